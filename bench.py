@@ -1,0 +1,192 @@
+"""GI frame benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1080p]
+                    [--form fp32|mixed|both] [--no-cpu] [--check]
+
+A step is one GI frame of the many-lights hot path on inputs already resident
+in HBM: VPL extraction from the RSM, ISM atlas clear + point splat, pull-push,
+and interleaved final gathering (mlgi_frame). The timed region is exactly K
+frames bracketed by a device synchronize on both sides; ms_per_step is wall
+time / K. Rank 0 prints one JSON line.
+
+The default workload is the 1920x1080, 4,096-VPL configuration (40k-triangle
+procedural room, 128^2 RSM, 1M splat points into 4,096 64^2 ISMs of a 4096^2
+atlas, 4 pull-push levels, 8x8 interleave, 16x16 tiles).
+
+roofline: the final gather dominates; its algorithmic work is 84 flops per
+lit pixel-VPL pair (pairs with a positive geometry factor, counted on the
+device), divided by the gather kernel's mean duration from HIP events on the
+context stream, against the MI355X FP32 vector peak.
+cpu_baseline: the oracle's FP32 instantiation of the same four stages run on
+the host for one full frame (final gathering threaded, the rest on one
+thread), scaled to ms/frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "many-lights-gi_amd"))
+
+FLOPS_PER_LIT_PAIR = 84
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(ctx, wl, inp, precision: int, steps: int, warmup: int):
+    import numpy as np
+    import mlgi
+
+    p = wl.params(precision=precision)
+    st = mlgi.GIStage(ctx, p, wl.n_vpl, wl.width, wl.height)
+    rsm = mlgi.rsm_desc(ctx.upload(inp.rsm_diffuse), ctx.upload(inp.rsm_normal), ctx.upload(inp.rsm_depth),
+                        inp.inv_bvp)
+    pts = mlgi.points_desc(ctx.upload(inp.points), ctx.upload(inp.bin_offsets))
+    gb = mlgi.gbuffer_desc(ctx.upload(inp.gb_depth), ctx.upload(inp.gb_normal), inp.vp_inv)
+    for _ in range(warmup):
+        st.process(rsm, pts, gb)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.process(rsm, pts, gb)
+    ctx.sync()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    # per-stage device times (separate, untimed pass with HIP events)
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(max(3, min(steps, 20))):
+        st.process(rsm, pts, gb)
+    stages = ctx.profile_read()
+    ctx.profile(False)
+    lit = st.count_lit_pairs(gb)
+    return {"st": st, "rsm": rsm, "pts": pts, "gb": gb, "wall_ms": wall_ms, "stages": stages, "lit": lit}
+
+
+def cpu_baseline(wl, inp, rec, r16, threads: int):
+    import numpy as np
+    import mlgi
+    import oracle
+
+    sh = mlgi.shuffle_indices(wl.n_vpl)
+    p = wl.params()
+    t = {}
+    t0 = time.perf_counter()
+    pos, nrm, pk, col = oracle.vpl_extract(inp.rsm_diffuse, inp.rsm_normal, inp.rsm_depth, inp.inv_bvp,
+                                           p.light_intensity, sh, wl.n_vpl, np.float32)
+    t["vpl"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    atlas = oracle.ism_splat(inp.points, inp.bin_offsets, pos, nrm, wl.atlas_side, p.z_far)
+    t["splat"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r16c = oracle.pullpush(atlas, wl.ism_side, wl.levels, p.z_far)
+    t["pullpush"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.final_gather(inp.gb_depth, inp.gb_normal, inp.vp_inv, pos, nrm, col, r16c, wl.atlas_side, p.z_far,
+                        p.gi_intensity_factor, p.vpl_clamping_value, wl.interleave, threads=threads,
+                        dtype=np.float32)
+    t["gather"] = time.perf_counter() - t0
+    total_ms = sum(t.values()) * 1e3
+    return {"value": round(total_ms, 3), "unit": "ms/frame", "cores": threads, "kind": "port",
+            "sample": (f"one full {wl.name} frame: oracle FP32 instantiation; final gathering on {threads} "
+                       "threads, VPL extraction/splat/pull-push on 1 thread"),
+            "stages_s": {k: round(v, 4) for k, v in t.items()}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="1080p")
+    ap.add_argument("--form", default="both", choices=["fp32", "mixed", "both"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--check", action="store_true", help="compare the frame with the FP64 oracle")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the sharding design is documented, not built).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import mlgi
+    from mlgi import workloads as W
+
+    wl = W.WORKLOADS[a.workload]
+    log(f"[bench] building inputs for {wl.name} ...")
+    t0 = time.perf_counter()
+    inp = W.build_inputs(wl)
+    log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s")
+    forms = {"fp32": [mlgi.PRECISION_FP32], "mixed": [mlgi.PRECISION_MIXED],
+             "both": [mlgi.PRECISION_FP32, mlgi.PRECISION_MIXED]}[a.form]
+    res = {}
+    with mlgi.Context(0) as ctx:
+        for prec in forms:
+            name = "fp32" if prec == mlgi.PRECISION_FP32 else "mixed"
+            log(f"[bench] form {name}: warmup {a.warmup}, steps {a.steps}")
+            r = run_form(ctx, wl, inp, prec, a.steps, a.warmup)
+            r["rec"] = r["st"].vpl_processor.host_records()
+            r["r16"] = r["st"].ism.pushpull_result.numpy()
+            r["img"] = r["st"].gi_buffer.numpy()
+            log(f"[bench] {name}: {r['wall_ms']:.4f} ms/frame, stages {r['stages']}")
+            res[name] = r
+    head = "mixed" if "mixed" in res else "fp32"
+    h = res[head]
+    fg_ms = h["stages"]["final_gather_ms"]
+    flops = FLOPS_PER_LIT_PAIR * h["lit"]
+    achieved = flops / (fg_ms * 1e-3) / 1e12 if fg_ms > 0 else 0.0
+    out = {
+        "metric": "GI ms/frame 1080p·4096 VPL (mixed vs FP32 HIP)",
+        "value": round(h["wall_ms"], 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(h["wall_ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16x2+fp32" if head == "mixed" else "fp32",
+        "data": "synthetic: seeded procedural room, ray-cast G-buffer/RSM, area-proportional points",
+        "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "vpls": wl.n_vpl,
+                   "interleave": wl.interleave, "points": wl.n_points, "atlas": wl.atlas_side,
+                   "ism": wl.ism_side, "pull_push_levels": wl.levels, "tile": f"{wl.tile[0]}x{wl.tile[1]}",
+                   "form": head, "triangles": wl.n_random + 12},
+        "forms": {k: {"ms_per_frame": round(v["wall_ms"], 4),
+                      "stages_ms": {s: round(x, 4) for s, x in v["stages"].items() if s != "frames"}}
+                  for k, v in res.items()},
+        "roofline": {"bound": "valu", "kernel": "final_gather", "achieved": round(achieved, 3),
+                     "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
+                     "lit_pairs": h["lit"], "flops_per_lit_pair": FLOPS_PER_LIT_PAIR,
+                     "kernel_ms": round(fg_ms, 4)},
+    }
+    if a.check:
+        import oracle
+        for k, v in res.items():
+            rec = v["rec"]
+            p = wl.params()
+            ref, _ = oracle.final_gather(inp.gb_depth, inp.gb_normal, inp.vp_inv, rec[:, 0:3], rec[:, 4:7],
+                                         rec[:, 13:16], v["r16"], wl.atlas_side, p.z_far, p.gi_intensity_factor,
+                                         p.vpl_clamping_value, wl.interleave, threads=a.cpu_threads)
+            out["forms"][k]["rel_l2_vs_fp64"] = float(np.linalg.norm(v["img"] - ref) / np.linalg.norm(ref))
+    if not a.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(wl, inp, h["rec"], h["r16"], a.cpu_threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

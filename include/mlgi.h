@@ -1,0 +1,220 @@
+/*
+ * mlgi -- many-lights global illumination hot path on MI355X (gfx950).
+ *
+ * C ABI of libmlgi.so. Plain pointers and sizes only; every device pointer
+ * is memory from mlgi_malloc (or any hipMalloc'd memory on the context's
+ * device). All compute entry points are asynchronous on the context's HIP
+ * stream; mlgi_sync() waits for them. A context is externally synchronised
+ * (one host thread at a time). Functions return mlgi_status; on failure
+ * mlgi_last_error(ctx) holds a message. No exception crosses this ABI.
+ *
+ * The stage entry points replace the reference's GL stage classes
+ * (karyon/many-lights-gi, source/mfs-painters/multiframepainter/):
+ *
+ *   mlgi_vpl_extract   VPLProcessor::process            VPLProcessor.h:22,
+ *                      VPLProcessor.cpp:73-101, gi/vpl_processor.comp
+ *   mlgi_ism_splat     ImperfectShadowmap::render (usePushPull branch:
+ *                      atlas clear + ism.comp dispatch)  ImperfectShadowmap.cpp:194-261
+ *   mlgi_ism_pullpush  ImperfectShadowmap::pullpush     ImperfectShadowmap.cpp:121-182
+ *   mlgi_ism_process   ImperfectShadowmap::process      ImperfectShadowmap.h:30-39
+ *   mlgi_final_gather  GIStage::compute_final_gathering GIStage.cpp:261-306,
+ *                      gi/final_gathering.comp
+ *   mlgi_frame         GIStage::process (VPLP -> ISM -> FG) GIStage.cpp:346-418
+ *
+ * The clustered light lists (ClusteredShading::process, ClusteredShading.h:30-39)
+ * are replaced inside mlgi_final_gather by per-tile culling that is exact
+ * with respect to the brute-force interleaved sum.
+ */
+#ifndef MLGI_H
+#define MLGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLGI_VERSION 1
+
+typedef enum mlgi_status {
+  MLGI_OK = 0,
+  MLGI_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, inconsistent shapes */
+  MLGI_ERR_OUT_OF_MEMORY = 2,    /* device allocation failed */
+  MLGI_ERR_HIP = 3,              /* HIP runtime error (message in mlgi_last_error) */
+  MLGI_ERR_UNSUPPORTED = 4,      /* valid in the reference, not implemented here */
+  MLGI_ERR_NO_DEVICE = 5,        /* no HIP device / device index out of range */
+  MLGI_ERR_OUT_OF_BUDGET = 6     /* admission refused by an HBM budget */
+} mlgi_status;
+
+typedef enum mlgi_precision {
+  MLGI_PRECISION_FP32 = 0,  /* plain FP32 final gathering */
+  MLGI_PRECISION_MIXED = 1  /* packed-FP16 geometry/flux terms, FP32 compensated sums */
+} mlgi_precision;
+
+typedef struct mlgi_context mlgi_context;
+
+/* Per-frame parameters: the GIStage members and uniforms
+ * (GIStage.h:52-100, defaults GIStage.cpp:215-232, uniforms GIStage.cpp:280-295,
+ * ImperfectShadowmap.cpp:34-35,188-190). */
+typedef struct mlgi_params {
+  float light_intensity;          /* VPL flux scale (lightIntensity, default 5) */
+  int32_t shuffle_lights;         /* permute VPLs with mt19937(1979982) shuffle */
+  float gi_intensity_factor;      /* default 3000 */
+  float vpl_clamping_value;       /* geometry-term clamp, default 1e-3 */
+  int32_t vpl_start_index;        /* vplStartIndex (normalisation only) */
+  int32_t vpl_end_index;          /* vplEndIndex; <=0 means vpl count */
+  int32_t scale_isms;             /* must be 0 (MLGI_ERR_UNSUPPORTED otherwise) */
+  int32_t points_only_into_scaled_isms; /* must be 0 */
+  float tess_level_factor;        /* accepted for interface parity; no effect */
+  int32_t use_push_pull;          /* must be 1: the point-splat ISM path */
+  int32_t enable_shadowing;       /* ENABLE_SHADOWING */
+  int32_t use_interleaving;       /* USE_INTERLEAVING */
+  int32_t interleave;             /* interleave side I (power of two, reference: 4) */
+  float z_far;                    /* ISM / camera zFar */
+  int32_t atlas_side;             /* ISM atlas side (reference totalIsmPixelSize 2048) */
+  int32_t pull_push_levels;       /* pull levels L (reference: 6) */
+  int32_t precision;              /* mlgi_precision */
+  int32_t tile_w, tile_h;         /* final-gather tile in interleave-class pixels; 0 = auto */
+} mlgi_params;
+
+/* RSM inputs (RasterizationStage outputs bound in VPLProcessor.cpp:84-86).
+ * Device pointers, row-major, row 0 = bottom (GL texel order). */
+typedef struct mlgi_rsm {
+  const float* diffuse;     /* width*height*3, RGB */
+  const float* face_normal; /* width*height*3, encoded n*0.5+0.5 */
+  const float* depth;       /* width*height, window-space depth in [0,1] */
+  int32_t width, height;
+  float inv_biased_view_projection[16]; /* column-major inverse(bias*P*V) */
+} mlgi_rsm;
+
+/* VPL records, MLGI_VPL_RECORD_FLOATS floats per VPL, 64-byte aligned:
+ *  [0..2] world position   [3] packed normal (pack3SNToFloat bits, u32)
+ *  [4..6] unpacked normal  [7..9] lookAtRH s row  [10..12] lookAtRH u row
+ *  [13..15] flux (diffuse * lightIntensity)
+ * (VPLProcessor vplBuffer + packedVplBuffer, VPLProcessor.cpp:29-41). */
+#define MLGI_VPL_RECORD_FLOATS 16
+typedef struct mlgi_vpls {
+  float* records; /* device, count * 16 floats */
+  int32_t count;
+} mlgi_vpls;
+
+/* Splat points grouped into bins (one bin per VPL, ism.geom:56-68). Each
+ * point is 4 floats: world xyz and, in the 4th slot, the u32 bits of
+ * packUnorm4x8(normal*0.5+0.5, radius/25). Bin b holds points
+ * [bin_offsets[b], bin_offsets[b+1]). */
+typedef struct mlgi_points {
+  const float* records;         /* device, count*4 */
+  const uint32_t* bin_offsets;  /* device, n_bins+1 */
+  int32_t count;
+  int32_t n_bins;
+} mlgi_points;
+
+/* ISM atlas ("ISM softrender", R32UI): packed (depth24 << 8) | radius8. */
+typedef struct mlgi_atlas {
+  uint32_t* texels;  /* device, side*side */
+  int32_t side;      /* totalIsmPixelSize, power of two */
+  int32_t ism_side;  /* ismPixelSize = side / mlgi_ism_indices_1d(ism count) */
+} mlgi_atlas;
+
+/* Pull-push result ("Pushpull result", R16 unorm). */
+typedef struct mlgi_ism_depth {
+  uint16_t* texels;  /* device, side*side */
+  int32_t side;
+  int32_t ism_side;
+} mlgi_ism_depth;
+
+/* Camera G-buffer (faceNormalBuffer, depthBuffer; MultiFramePainter.cpp:118-119). */
+typedef struct mlgi_gbuffer {
+  const float* depth;       /* width*height window depth */
+  const float* face_normal; /* width*height*3, encoded n*0.5+0.5 */
+  int32_t width, height;
+  float view_projection_inverse[16]; /* column-major viewInverted()*projectionInverted() */
+} mlgi_gbuffer;
+
+/* Irradiance output (giBuffer), FP32 RGB. */
+typedef struct mlgi_image {
+  float* rgb; /* device, width*height*3 */
+  int32_t width, height;
+} mlgi_image;
+
+/* Per-stage device times of the last profiled frame(s), milliseconds
+ * (stage names of the reference's PerfCounter: VPLP, ISM CS, pull-push, FG). */
+typedef struct mlgi_stage_times {
+  float vpl_ms;
+  float ism_clear_ms;
+  float ism_splat_ms;
+  float pullpush_ms;
+  float final_gather_ms;
+  float frame_ms;
+  int32_t frames; /* frames accumulated since the last reset */
+} mlgi_stage_times;
+
+/* ---------------------------------------------------------------- context */
+int32_t mlgi_version(void);
+int mlgi_device_count(int32_t* count);
+int mlgi_context_create(int32_t device, mlgi_context** out);
+void mlgi_context_destroy(mlgi_context* ctx);
+const char* mlgi_last_error(const mlgi_context* ctx);
+int mlgi_sync(mlgi_context* ctx);
+/* hipStream_t of the context, as an opaque pointer. */
+void* mlgi_stream(mlgi_context* ctx);
+
+/* Defaults of GIStage::initialize (GIStage.cpp:215-232) plus the atlas and
+ * pull-push constants of ImperfectShadowmap (2048 side, 6 levels). */
+void mlgi_params_default(mlgi_params* p);
+
+/* ----------------------------------------------------------------- memory */
+int mlgi_malloc(mlgi_context* ctx, size_t bytes, void** dptr);
+int mlgi_free(mlgi_context* ctx, void* dptr);
+int mlgi_memcpy_htod(mlgi_context* ctx, void* dst, const void* src, size_t bytes);
+int mlgi_memcpy_dtoh(mlgi_context* ctx, void* dst, const void* src, size_t bytes);
+
+/* ----------------------------------------------------------------- stages */
+/* VPL extraction: count = n VPLs regularly sampled from the RSM. */
+int mlgi_vpl_extract(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_params* p,
+                     mlgi_vpls* out);
+/* Clear the atlas to 0xFFFFFFFF and splat every point into <= 4 of the 16
+ * candidate ISMs of its bin with 32-bit atomicMin. Bit-exact. */
+int mlgi_ism_splat(mlgi_context* ctx, const mlgi_points* pts, const mlgi_vpls* vpls,
+                   const mlgi_params* p, mlgi_atlas* atlas);
+/* Pull-push hole filling, one workgroup per ISM, R16 result. Bit-exact.
+ * atlas->ism_side (ismPixelSize) must be one of 8, 16, 32, 64. */
+int mlgi_ism_pullpush(mlgi_context* ctx, const mlgi_atlas* atlas, const mlgi_params* p,
+                      mlgi_ism_depth* out);
+/* Splat + pull-push (ImperfectShadowmap::process). */
+int mlgi_ism_process(mlgi_context* ctx, const mlgi_points* pts, const mlgi_vpls* vpls,
+                     const mlgi_params* p, mlgi_atlas* atlas, mlgi_ism_depth* out);
+/* Interleaved final gathering with per-tile VPL culling. */
+int mlgi_final_gather(mlgi_context* ctx, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
+                      const mlgi_ism_depth* ism, const mlgi_params* p, mlgi_image* out);
+/* One GI frame: VPL extraction, ISM splat, pull-push, final gathering. The
+ * work buffers (vpls, atlas, ism) are caller-owned. */
+int mlgi_frame(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_points* pts,
+               const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls,
+               mlgi_atlas* atlas, mlgi_ism_depth* ism, mlgi_image* out);
+
+/* Number of (pixel, VPL-of-its-class) pairs with a positive geometry factor
+ * (the lit pairs the gather's algorithmic work is counted on). Synchronous. */
+int mlgi_count_lit_pairs(mlgi_context* ctx, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
+                         const mlgi_params* p, uint64_t* out);
+
+/* --------------------------------------------------------------- timing */
+/* Enable per-stage HIP event timing (PerfCounter analogue). */
+int mlgi_profile_enable(mlgi_context* ctx, int32_t on);
+int mlgi_profile_reset(mlgi_context* ctx);
+/* Averages over the frames profiled since the last reset. Synchronises. */
+int mlgi_profile_read(mlgi_context* ctx, mlgi_stage_times* out);
+
+/* ------------------------------------------------- host-side helpers */
+/* The fixed VPL permutation of VPLProcessor.cpp:58-62:
+ * std::shuffle(iota(n), std::mt19937(seed)). */
+int mlgi_shuffle_indices(int32_t n, uint32_t seed, int32_t* out);
+/* ISM grid side: int(pow(2, ceil(log2(ism_count) / 2))) (ImperfectShadowmap.cpp:189). */
+int32_t mlgi_ism_indices_1d(int32_t ism_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLGI_H */

@@ -1,0 +1,538 @@
+// C ABI of libmlgi.so: context, device memory, shape validation and the
+// stage entry points declared in include/mlgi.h.
+//
+// Every launch is preceded by host-side checks that the descriptors'
+// shapes match what the kernel grids assume, so a kernel never indexes past
+// a buffer the caller described.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/mlgi.h"
+#include "kernels.hpp"
+
+struct mlgi_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  std::map<int, int32_t*> shuffle;  // VPL count -> device permutation
+  unsigned long long* counter = nullptr;
+  // profiling (PerfCounter analogue): event pairs per stage
+  bool profile = false;
+  struct Span {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Span> spans;
+  std::vector<hipEvent_t> pool;
+  int frames = 0;
+};
+
+namespace {
+
+enum Stage { kVpl = 0, kClear, kSplat, kPullPush, kGather, kFrame, kStages };
+
+int fail(mlgi_context* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->error = buf;
+  }
+  return code;
+}
+
+int hip_check(mlgi_context* c, hipError_t e, const char* what) {
+  if (e == hipSuccess) return MLGI_OK;
+  return fail(c, e == hipErrorOutOfMemory ? MLGI_ERR_OUT_OF_MEMORY : MLGI_ERR_HIP, "%s: %s", what,
+              hipGetErrorString(e));
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+int log2i(int v) {
+  int k = 0;
+  while ((1 << k) < v) ++k;
+  return k;
+}
+
+hipEvent_t take_event(mlgi_context* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct Scope {  // records a timed span on the context stream when profiling
+  mlgi_context* c;
+  int stage;
+  hipEvent_t a = nullptr;
+  Scope(mlgi_context* ctx, int st) : c(ctx), stage(st) {
+    if (c->profile) {
+      a = take_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~Scope() {
+    if (c->profile && a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        (void)hipEventRecord(b, c->stream);
+        c->spans.push_back({stage, a, b});
+      } else {
+        c->pool.push_back(a);
+      }
+    }
+  }
+};
+
+mlgi::Mat4 to_mat(const float* m) {
+  mlgi::Mat4 r;
+  std::memcpy(r.m, m, sizeof r.m);
+  return r;
+}
+
+int get_shuffle(mlgi_context* c, int n, const int32_t** out) {
+  auto it = c->shuffle.find(n);
+  if (it != c->shuffle.end()) {
+    *out = it->second;
+    return MLGI_OK;
+  }
+  std::vector<int32_t> v(n);
+  mlgi_shuffle_indices(n, 1979982u, v.data());
+  int32_t* d = nullptr;
+  int rc = hip_check(c, hipMalloc(&d, sizeof(int32_t) * n), "hipMalloc(shuffle)");
+  if (rc) return rc;
+  rc = hip_check(c, hipMemcpy(d, v.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice),
+                 "hipMemcpy(shuffle)");
+  if (rc) {
+    (void)hipFree(d);
+    return rc;
+  }
+  c->shuffle[n] = d;
+  *out = d;
+  return MLGI_OK;
+}
+
+int check_params(mlgi_context* c, const mlgi_params* p) {
+  if (!p) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "params is NULL");
+  if (p->scale_isms || p->points_only_into_scaled_isms)
+    return fail(c, MLGI_ERR_UNSUPPORTED, "scaleISMs / pointsOnlyIntoScaledISMs are not supported");
+  if (!p->use_push_pull)
+    return fail(c, MLGI_ERR_UNSUPPORTED, "usePushPull=false (raster ISM path) is not supported");
+  if (!(p->z_far > 0.0f)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "z_far must be > 0");
+  return MLGI_OK;
+}
+
+int interleave_of(const mlgi_params* p) { return p->use_interleaving ? p->interleave : 1; }
+
+int check_vpls(mlgi_context* c, const mlgi_vpls* v) {
+  if (!v || !v->records) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "vpls is NULL");
+  if (v->count <= 0 || v->count > (1 << 20))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "vpl count %d out of range (1..2^20)", v->count);
+  return MLGI_OK;
+}
+
+// Atlas geometry: n1d = ism grid side, S = atlas side / n1d.
+int atlas_geometry(mlgi_context* c, int side, int n_vpl, int* n1d, int* ism_side) {
+  if (!is_pow2(side) || side > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas side %d must be a power of two <= 16384", side);
+  const int g = mlgi_ism_indices_1d(n_vpl);
+  if (side % g != 0 || side / g < 2)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas side %d too small for %d ISMs (grid %d)", side,
+                n_vpl, g);
+  *n1d = g;
+  *ism_side = side / g;
+  return MLGI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mlgi_version(void) { return MLGI_VERSION; }
+
+int mlgi_device_count(int32_t* count) {
+  if (!count) return MLGI_ERR_INVALID_ARGUMENT;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return MLGI_OK;
+}
+
+int mlgi_context_create(int32_t device, mlgi_context** out) {
+  if (!out) return MLGI_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MLGI_ERR_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return MLGI_ERR_HIP;
+  auto* c = new mlgi_context();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MLGI_ERR_HIP;
+  }
+  if (hipMalloc(&c->counter, sizeof(unsigned long long)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MLGI_ERR_OUT_OF_MEMORY;
+  }
+  *out = c;
+  return MLGI_OK;
+}
+
+void mlgi_context_destroy(mlgi_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->shuffle) (void)hipFree(kv.second);
+  for (auto& s : c->spans) {
+    (void)hipEventDestroy(s.a);
+    (void)hipEventDestroy(s.b);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  (void)hipFree(c->counter);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* mlgi_last_error(const mlgi_context* c) { return c ? c->error.c_str() : ""; }
+
+int mlgi_sync(mlgi_context* c) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  return hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+void* mlgi_stream(mlgi_context* c) { return c ? (void*)c->stream : nullptr; }
+
+void mlgi_params_default(mlgi_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->light_intensity = 5.0f;
+  p->shuffle_lights = 1;
+  p->gi_intensity_factor = 3000.0f;
+  p->vpl_clamping_value = 0.001f;
+  p->vpl_start_index = 0;
+  p->vpl_end_index = 0;  // = vpl count
+  p->scale_isms = 0;
+  p->points_only_into_scaled_isms = 0;
+  p->tess_level_factor = 2.0f;
+  p->use_push_pull = 1;
+  p->enable_shadowing = 1;
+  p->use_interleaving = 1;
+  p->interleave = 4;
+  p->z_far = 50.0f;
+  p->atlas_side = 2048;
+  p->pull_push_levels = 6;
+  p->precision = MLGI_PRECISION_FP32;
+  p->tile_w = 0;
+  p->tile_h = 0;
+}
+
+int mlgi_malloc(mlgi_context* c, size_t bytes, void** dptr) {
+  if (!c || !dptr) return MLGI_ERR_INVALID_ARGUMENT;
+  *dptr = nullptr;
+  if (bytes == 0) bytes = 256;
+  return hip_check(c, hipMalloc(dptr, bytes), "hipMalloc");
+}
+
+int mlgi_free(mlgi_context* c, void* d) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!d) return MLGI_OK;
+  (void)hipStreamSynchronize(c->stream);
+  return hip_check(c, hipFree(d), "hipFree");
+}
+
+int mlgi_memcpy_htod(mlgi_context* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!bytes) return MLGI_OK;
+  int rc = hip_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream),
+                     "hipMemcpyAsync(HtoD)");
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+int mlgi_memcpy_dtoh(mlgi_context* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!bytes) return MLGI_OK;
+  int rc = hip_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream),
+                     "hipMemcpyAsync(DtoH)");
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+int mlgi_shuffle_indices(int32_t n, uint32_t seed, int32_t* out) {
+  if (n < 0 || (!out && n)) return MLGI_ERR_INVALID_ARGUMENT;
+  std::vector<int> v(n);
+  std::iota(v.begin(), v.end(), 0);
+  std::mt19937 g(seed);
+  std::shuffle(v.begin(), v.end(), g);
+  for (int i = 0; i < n; ++i) out[i] = v[i];
+  return MLGI_OK;
+}
+
+int32_t mlgi_ism_indices_1d(int32_t ism_count) {
+  int k = 0;
+  while ((1LL << (2 * k)) < (long long)ism_count) ++k;
+  return 1 << k;
+}
+
+int mlgi_vpl_extract(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_params* p, mlgi_vpls* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!p) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "params is NULL");
+  if (!rsm || !rsm->diffuse || !rsm->face_normal || !rsm->depth)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "rsm buffers are NULL");
+  if (rsm->width <= 0 || rsm->height <= 0 || rsm->width > 16384 || rsm->height > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "rsm size %dx%d out of range", rsm->width, rsm->height);
+  int rc = check_vpls(c, out);
+  if (rc) return rc;
+  const int32_t* sh = nullptr;
+  if (p->shuffle_lights && (rc = get_shuffle(c, out->count, &sh))) return rc;
+  Scope sc(c, kVpl);
+  return hip_check(c,
+                   mlgi::launch_vpl_extract(c->stream, rsm->diffuse, rsm->face_normal, rsm->depth,
+                                            rsm->width, rsm->height,
+                                            to_mat(rsm->inv_biased_view_projection),
+                                            p->light_intensity, sh, out->count, out->records),
+                   "vpl_extract");
+}
+
+int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls,
+                   const mlgi_params* p, mlgi_atlas* atlas) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  int rc = check_params(c, p);
+  if (rc || (rc = check_vpls(c, vpls))) return rc;
+  if (!pts || !atlas || !atlas->texels)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "points / atlas is NULL");
+  if (pts->count < 0 || (pts->count > 0 && (!pts->records || !pts->bin_offsets)))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "points buffers are NULL");
+  if (pts->count > 0 && pts->n_bins != vpls->count)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "point bins (%d) must equal the VPL count (%d)",
+                pts->n_bins, vpls->count);
+  int n1d, s;
+  if ((rc = atlas_geometry(c, atlas->side, vpls->count, &n1d, &s))) return rc;
+  if (atlas->ism_side == 0) atlas->ism_side = s;
+  if (atlas->ism_side != s)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas ism_side %d != %d for %d VPLs in a %d atlas",
+                atlas->ism_side, s, vpls->count, atlas->side);
+  const size_t texels = (size_t)atlas->side * atlas->side;
+  {
+    Scope sc(c, kClear);
+    if ((rc = hip_check(c, mlgi::launch_atlas_clear(c->stream, atlas->texels, texels), "atlas_clear")))
+      return rc;
+  }
+  if (pts->count == 0) return MLGI_OK;
+  Scope sc(c, kSplat);
+  return hip_check(c,
+                   mlgi::launch_ism_splat(c->stream, pts->records, pts->bin_offsets, pts->n_bins,
+                                          pts->count, vpls->records, vpls->count, n1d, atlas->side,
+                                          p->z_far, atlas->texels),
+                   "ism_splat");
+}
+
+int mlgi_ism_pullpush(mlgi_context* c, const mlgi_atlas* atlas, const mlgi_params* p,
+                      mlgi_ism_depth* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!atlas || !atlas->texels || !out || !out->texels)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas / output is NULL");
+  if (out->side != atlas->side)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "output side %d != atlas side %d", out->side, atlas->side);
+  const int s = atlas->ism_side;
+  if (!is_pow2(atlas->side) || atlas->side > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas side %d must be a power of two <= 16384", atlas->side);
+  if (s != 8 && s != 16 && s != 32 && s != 64)
+    return fail(c, MLGI_ERR_UNSUPPORTED, "ISM side %d not in {8,16,32,64}", s);
+  if (atlas->side % s != 0)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "ISM side %d does not divide atlas side %d", s, atlas->side);
+  if (p->pull_push_levels < 1 || p->pull_push_levels > log2i(s))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "pull-push levels %d out of range 1..%d",
+                p->pull_push_levels, log2i(s));
+  out->ism_side = s;
+  Scope sc(c, kPullPush);
+  return hip_check(c,
+                   mlgi::launch_pullpush(c->stream, atlas->texels, atlas->side, s,
+                                         p->pull_push_levels, p->z_far, out->texels),
+                   "pullpush");
+}
+
+int mlgi_ism_process(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls,
+                     const mlgi_params* p, mlgi_atlas* atlas, mlgi_ism_depth* out) {
+  int rc = mlgi_ism_splat(c, pts, vpls, p, atlas);
+  if (rc) return rc;
+  return mlgi_ism_pullpush(c, atlas, p, out);
+}
+
+namespace {
+
+int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
+                      const mlgi_ism_depth* ism, const mlgi_params* p, mlgi::GatherArgs* a) {
+  int rc = check_params(c, p);
+  if (rc || (rc = check_vpls(c, vpls))) return rc;
+  if (!gb || !gb->depth || !gb->face_normal)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "gbuffer buffers are NULL");
+  if (gb->width <= 0 || gb->height <= 0 || gb->width > 16384 || gb->height > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "gbuffer size %dx%d out of range", gb->width, gb->height);
+  const int il = interleave_of(p);
+  if (!is_pow2(il) || il > 16)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "interleave %d must be a power of two <= 16", il);
+  const int classes = il * il;
+  if (vpls->count % classes != 0)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "vpl count %d not divisible by interleave classes %d",
+                vpls->count, classes);
+  const int vpl_end = p->vpl_end_index > 0 ? p->vpl_end_index : vpls->count;
+  if (vpl_end - p->vpl_start_index <= 0)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "vplEndIndex - vplStartIndex must be > 0");
+  std::memset(a, 0, sizeof *a);
+  if (p->enable_shadowing) {
+    if (!ism || !ism->texels) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "ism is NULL");
+    int n1d, s;
+    if ((rc = atlas_geometry(c, ism->side, vpls->count, &n1d, &s))) return rc;
+    if (ism->ism_side != s)
+      return fail(c, MLGI_ERR_INVALID_ARGUMENT, "ism side %d != %d for %d VPLs in a %d atlas",
+                  ism->ism_side, s, vpls->count, ism->side);
+    a->ism = ism->texels;
+    a->atlas_side = ism->side;
+    a->ism_side = s;
+    a->n1d = n1d;
+  } else {
+    a->atlas_side = 1;
+    a->ism_side = 1;
+    a->n1d = 1;
+  }
+  a->depth = gb->depth;
+  a->normal = gb->face_normal;
+  a->width = gb->width;
+  a->height = gb->height;
+  a->vp_inv = to_mat(gb->view_projection_inverse);
+  a->vpl = vpls->records;
+  a->n_vpl = vpls->count;
+  a->per_class = vpls->count / classes;
+  a->interleave = il;
+  a->interleave_log2 = log2i(il);
+  a->z_far = p->z_far;
+  a->clamp_value = p->vpl_clamping_value;
+  a->out_scale = p->gi_intensity_factor / (float)(vpl_end - p->vpl_start_index) * (float)classes;
+  a->shadowing = p->enable_shadowing ? 1 : 0;
+  return MLGI_OK;
+}
+
+void pick_tile(const mlgi_params* p, int* tw, int* th) {
+  *tw = p->tile_w > 0 ? p->tile_w : 8;
+  *th = p->tile_h > 0 ? p->tile_h : 8;
+}
+
+}  // namespace
+
+int mlgi_final_gather(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
+                      const mlgi_ism_depth* ism, const mlgi_params* p, mlgi_image* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  mlgi::GatherArgs a;
+  int rc = build_gather_args(c, gb, vpls, ism, p, &a);
+  if (rc) return rc;
+  if (!out || !out->rgb || out->width != gb->width || out->height != gb->height)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "output image must match the gbuffer size");
+  if (p->precision != MLGI_PRECISION_FP32 && p->precision != MLGI_PRECISION_MIXED)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "unknown precision %d", p->precision);
+  int tw, th;
+  pick_tile(p, &tw, &th);
+  if (!((tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 8) || (tw == 16 && th == 4)))
+    return fail(c, MLGI_ERR_UNSUPPORTED, "tile %dx%d not in {8x8, 16x16, 32x8, 16x4}", tw, th);
+  a.out = out->rgb;
+  Scope sc(c, kGather);
+  return hip_check(c, mlgi::launch_final_gather(c->stream, a, p->precision, tw, th), "final_gather");
+}
+
+int mlgi_frame(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_points* pts,
+               const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls, mlgi_atlas* atlas,
+               mlgi_ism_depth* ism, mlgi_image* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  Scope sc(c, kFrame);
+  int rc = mlgi_vpl_extract(c, rsm, p, vpls);
+  if (rc) return rc;
+  if ((rc = mlgi_ism_process(c, pts, vpls, p, atlas, ism))) return rc;
+  if ((rc = mlgi_final_gather(c, gb, vpls, ism, p, out))) return rc;
+  if (c->profile) c->frames++;
+  return MLGI_OK;
+}
+
+int mlgi_count_lit_pairs(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
+                         const mlgi_params* p, uint64_t* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!out) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "out is NULL");
+  mlgi_params q = *p;
+  q.enable_shadowing = 0;
+  mlgi::GatherArgs a;
+  int rc = build_gather_args(c, gb, vpls, nullptr, &q, &a);
+  if (rc) return rc;
+  if ((rc = hip_check(c, hipMemsetAsync(c->counter, 0, sizeof(unsigned long long), c->stream),
+                      "hipMemsetAsync")))
+    return rc;
+  if ((rc = hip_check(c, mlgi::launch_count_lit_pairs(c->stream, a, c->counter), "count_lit_pairs")))
+    return rc;
+  unsigned long long v = 0;
+  if ((rc = hip_check(c, hipMemcpyAsync(&v, c->counter, sizeof v, hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpyAsync")))
+    return rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"))) return rc;
+  *out = v;
+  return MLGI_OK;
+}
+
+int mlgi_profile_enable(mlgi_context* c, int32_t on) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  c->profile = on != 0;
+  return MLGI_OK;
+}
+
+int mlgi_profile_reset(mlgi_context* c) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& s : c->spans) {
+    c->pool.push_back(s.a);
+    c->pool.push_back(s.b);
+  }
+  c->spans.clear();
+  c->frames = 0;
+  return MLGI_OK;
+}
+
+int mlgi_profile_read(mlgi_context* c, mlgi_stage_times* out) {
+  if (!c || !out) return MLGI_ERR_INVALID_ARGUMENT;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (rc) return rc;
+  double sum[kStages] = {0};
+  int cnt[kStages] = {0};
+  for (auto& s : c->spans) {
+    float ms = 0.0f;
+    if ((rc = hip_check(c, hipEventElapsedTime(&ms, s.a, s.b), "hipEventElapsedTime"))) return rc;
+    sum[s.stage] += ms;
+    cnt[s.stage]++;
+  }
+  // averages per call of each stage (per frame when driven by mlgi_frame)
+  auto avg = [&](int st) { return cnt[st] ? (float)(sum[st] / cnt[st]) : 0.0f; };
+  out->vpl_ms = avg(kVpl);
+  out->ism_clear_ms = avg(kClear);
+  out->ism_splat_ms = avg(kSplat);
+  out->pullpush_ms = avg(kPullPush);
+  out->final_gather_ms = avg(kGather);
+  out->frame_ms = avg(kFrame);
+  out->frames = c->frames;
+  return MLGI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// Host-side launch wrappers of the gfx950 kernels. Each wrapper validates
+// nothing: the C-ABI layer (capi.cpp) checks shapes before calling, so every
+// grid and index below is known to be in bounds of its buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlgi {
+
+struct Mat4 {
+  float m[16];  // column-major
+};
+
+// vpl_extract.hip
+hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* normal,
+                              const float* depth, int w, int h, const Mat4& inv_bvp,
+                              float light_intensity, const int32_t* shuffle, int n,
+                              float* records);
+
+// ism_splat.hip
+hipError_t launch_atlas_clear(hipStream_t s, uint32_t* atlas, size_t texels);
+hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* bin_offsets,
+                            int n_bins, int n_points, const float* vpl_records, int n_vpl, int n1d,
+                            int atlas_side, float z_far, uint32_t* atlas);
+
+// pullpush.hip
+hipError_t launch_pullpush(hipStream_t s, const uint32_t* atlas, int atlas_side, int ism_side,
+                           int levels, float z_far, uint16_t* out);
+
+// final_gather.hip
+struct GatherArgs {
+  const float* depth;
+  const float* normal;
+  int width, height;
+  Mat4 vp_inv;
+  const float* vpl;  // records
+  int n_vpl;
+  int per_class;     // n_vpl / classes
+  int interleave;    // I
+  int interleave_log2;
+  const uint16_t* ism;
+  int atlas_side;
+  int ism_side;
+  int n1d;
+  float z_far;
+  float clamp_value;
+  float out_scale;   // giIntensityFactor / vplCount * I^2
+  int shadowing;
+  float* out;
+};
+hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision, int tile_w,
+                               int tile_h);
+hipError_t launch_count_lit_pairs(hipStream_t s, const GatherArgs& a,
+                                  unsigned long long* counter);
+
+}  // namespace mlgi

@@ -1,0 +1,620 @@
+// Host-side synthetic inputs (include/mlgi_scene.h): a seeded procedural
+// scene, a BVH ray caster that writes the G-buffer and RSM in the formats the
+// reference rasteriser produces (RasterizationStage.cpp:139-149,
+// model.frag:93 faceNormal = N*0.5+0.5), and the area-proportional splat
+// point set with the reference's point record and bin hash (ism.geom:49-68,
+// random.glsl:34-66). Deterministic for a given seed and independent of the
+// thread count.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/mlgi.h"
+#include "../../include/mlgi_scene.h"
+
+namespace {
+
+struct D3 {
+  double x, y, z;
+};
+inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double len(D3 a) { return std::sqrt(dot(a, a)); }
+inline D3 norm(D3 a) { return a * (1.0 / len(a)); }
+inline double comp(D3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// splitmix64: sequential generator and counter-based hash
+inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+struct Rng {
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s(seed) {}
+  double uniform() {
+    s += 0x9E3779B97F4A7C15ull;
+    return (double)(mix64(s) >> 11) * (1.0 / 9007199254740992.0);
+  }
+};
+inline double counter_uniform(uint64_t seed, uint64_t i, uint64_t k) {
+  uint64_t z = mix64(seed ^ mix64(i * 4 + k + 0x632BE59BD9B4E019ull));
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+struct Tri {
+  D3 v0, v1, v2, n;
+  float albedo[3];
+};
+
+struct Node {
+  double lo[3], hi[3];
+  int left, count;  // leaf: count > 0, left = first index; inner: left child, right = left+1
+};
+
+struct Hit {
+  double t;
+  int tri;
+};
+
+}  // namespace
+
+struct mlgi_scene {
+  std::vector<Tri> tris;
+  std::vector<int> order;  // BVH leaf order
+  std::vector<Node> nodes;
+  double area = 0.0;
+};
+
+namespace {
+
+const D3 kDefaultEye = {5.0, 5.0, 9.0};
+
+void build_bvh(mlgi_scene* s) {
+  const int n = (int)s->tris.size();
+  s->order.resize(n);
+  for (int i = 0; i < n; ++i) s->order[i] = i;
+  std::vector<D3> cen(n);
+  for (int i = 0; i < n; ++i) cen[i] = (s->tris[i].v0 + s->tris[i].v1 + s->tris[i].v2) * (1.0 / 3.0);
+  s->nodes.clear();
+  s->nodes.reserve(2 * n);
+  struct Item {
+    int node, begin, end;
+  };
+  std::vector<Item> stack;
+  s->nodes.push_back(Node{});
+  stack.push_back({0, 0, n});
+  while (!stack.empty()) {
+    Item it = stack.back();
+    stack.pop_back();
+    Node nd{};
+    for (int k = 0; k < 3; ++k) {
+      nd.lo[k] = 1e300;
+      nd.hi[k] = -1e300;
+    }
+    double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+    for (int i = it.begin; i < it.end; ++i) {
+      const Tri& t = s->tris[s->order[i]];
+      for (const D3* v : {&t.v0, &t.v1, &t.v2})
+        for (int k = 0; k < 3; ++k) {
+          nd.lo[k] = std::min(nd.lo[k], comp(*v, k));
+          nd.hi[k] = std::max(nd.hi[k], comp(*v, k));
+        }
+      for (int k = 0; k < 3; ++k) {
+        clo[k] = std::min(clo[k], comp(cen[s->order[i]], k));
+        chi[k] = std::max(chi[k], comp(cen[s->order[i]], k));
+      }
+    }
+    const int cnt = it.end - it.begin;
+    if (cnt <= 4) {
+      nd.left = it.begin;
+      nd.count = cnt;
+      s->nodes[it.node] = nd;
+      continue;
+    }
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+    const int mid = (it.begin + it.end) / 2;
+    std::nth_element(s->order.begin() + it.begin, s->order.begin() + mid, s->order.begin() + it.end,
+                     [&](int a, int b) {
+                       double ca = comp(cen[a], axis), cb = comp(cen[b], axis);
+                       return ca < cb || (ca == cb && a < b);
+                     });
+    nd.left = (int)s->nodes.size();
+    nd.count = 0;
+    s->nodes[it.node] = nd;
+    s->nodes.push_back(Node{});
+    s->nodes.push_back(Node{});
+    stack.push_back({nd.left, it.begin, mid});
+    stack.push_back({nd.left + 1, mid, it.end});
+  }
+}
+
+inline bool box_hit(const Node& nd, D3 o, D3 inv, double tmax, double* tnear) {
+  double t0 = 0.0, t1 = tmax;
+  for (int k = 0; k < 3; ++k) {
+    double a = (nd.lo[k] - comp(o, k)) * comp(inv, k);
+    double b = (nd.hi[k] - comp(o, k)) * comp(inv, k);
+    if (a > b) std::swap(a, b);
+    t0 = std::max(t0, a);
+    t1 = std::min(t1, b);
+    if (t0 > t1) return false;
+  }
+  *tnear = t0;
+  return true;
+}
+
+// Moller-Trumbore, two-sided
+inline bool tri_hit(const Tri& t, D3 o, D3 d, double tmax, double* th) {
+  D3 e1 = t.v1 - t.v0, e2 = t.v2 - t.v0;
+  D3 p = cross(d, e2);
+  double det = dot(e1, p);
+  if (std::fabs(det) < 1e-14) return false;
+  double inv = 1.0 / det;
+  D3 s = o - t.v0;
+  double u = dot(s, p) * inv;
+  if (u < 0.0 || u > 1.0) return false;
+  D3 q = cross(s, e1);
+  double v = dot(d, q) * inv;
+  if (v < 0.0 || u + v > 1.0) return false;
+  double tt = dot(e2, q) * inv;
+  if (tt <= 0.0 || tt >= tmax) return false;
+  *th = tt;
+  return true;
+}
+
+Hit trace(const mlgi_scene* s, D3 o, D3 d, double tmax) {
+  Hit h{tmax, -1};
+  D3 inv = {1.0 / (d.x != 0 ? d.x : 1e-300), 1.0 / (d.y != 0 ? d.y : 1e-300), 1.0 / (d.z != 0 ? d.z : 1e-300)};
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const Node& nd = s->nodes[stack[--sp]];
+    double tn;
+    if (!box_hit(nd, o, inv, h.t, &tn)) continue;
+    if (nd.count > 0) {
+      for (int i = nd.left; i < nd.left + nd.count; ++i) {
+        double th;
+        int ti = s->order[i];
+        if (tri_hit(s->tris[ti], o, d, h.t, &th) && (th < h.t || (th == h.t && ti < h.tri))) {
+          h.t = th;
+          h.tri = ti;
+        }
+      }
+    } else {
+      stack[sp++] = nd.left;
+      stack[sp++] = nd.left + 1;
+    }
+  }
+  return h;
+}
+
+// 4x4 column-major helpers in double
+struct M4 {
+  double m[16];
+};
+M4 mul(const M4& a, const M4& b) {
+  M4 r{};
+  for (int c = 0; c < 4; ++c)
+    for (int rr = 0; rr < 4; ++rr) {
+      double acc = 0;
+      for (int k = 0; k < 4; ++k) acc += a.m[k * 4 + rr] * b.m[c * 4 + k];
+      r.m[c * 4 + rr] = acc;
+    }
+  return r;
+}
+bool inverse(const M4& a, M4* out) {
+  double m[4][8];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      m[r][c] = a.m[c * 4 + r];
+      m[r][c + 4] = r == c ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 4; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+    if (std::fabs(m[piv][c]) < 1e-300) return false;
+    for (int k = 0; k < 8; ++k) std::swap(m[c][k], m[piv][k]);
+    double d = m[c][c];
+    for (int k = 0; k < 8; ++k) m[c][k] /= d;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      double f = m[r][c];
+      for (int k = 0; k < 8; ++k) m[r][k] -= f * m[c][k];
+    }
+  }
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) out->m[c * 4 + r] = m[r][c + 4];
+  return true;
+}
+void xform(const M4& a, const double v[4], double o[4]) {
+  for (int r = 0; r < 4; ++r) {
+    double acc = 0;
+    for (int c = 0; c < 4; ++c) acc += a.m[c * 4 + r] * v[c];
+    o[r] = acc;
+  }
+}
+
+void camera_matrices(const mlgi_camera* cam, M4* view, M4* proj) {
+  D3 eye = {cam->eye[0], cam->eye[1], cam->eye[2]};
+  D3 ctr = {cam->center[0], cam->center[1], cam->center[2]};
+  D3 up = {cam->up[0], cam->up[1], cam->up[2]};
+  D3 f = norm(ctr - eye);
+  D3 s = norm(cross(f, up));
+  D3 u = cross(s, f);
+  M4 v{};
+  v.m[0] = s.x; v.m[4] = s.y; v.m[8] = s.z;
+  v.m[1] = u.x; v.m[5] = u.y; v.m[9] = u.z;
+  v.m[2] = -f.x; v.m[6] = -f.y; v.m[10] = -f.z;
+  v.m[12] = -dot(s, eye); v.m[13] = -dot(u, eye); v.m[14] = dot(f, eye);
+  v.m[15] = 1.0;
+  *view = v;
+  const double th = std::tan((double)cam->fovy_deg * M_PI / 180.0 / 2.0);
+  const double n = cam->z_near, fa = cam->z_far;
+  M4 p{};
+  p.m[0] = 1.0 / ((double)cam->aspect * th);
+  p.m[5] = 1.0 / th;
+  p.m[10] = -(fa + n) / (fa - n);
+  p.m[11] = -1.0;
+  p.m[14] = -(2.0 * fa * n) / (fa - n);
+  *proj = p;
+}
+
+void to_float(const M4& a, float* o) {
+  for (int i = 0; i < 16; ++i) o[i] = (float)a.m[i];
+}
+
+int nthreads(int t) {
+  if (t > 0) return t;
+  unsigned h = std::thread::hardware_concurrency();
+  return h ? (int)std::min(h, 64u) : 4;
+}
+
+template <class F>
+void parallel_rows(int rows, int threads, F f) {
+  const int nt = std::max(1, std::min(threads, rows));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([=]() {
+      for (int y = t; y < rows; y += nt) f(y);
+    });
+  for (auto& th : pool) th.join();
+}
+
+struct RayCam {
+  M4 vp, inv_vp;
+  D3 eye, axis;
+};
+
+RayCam ray_cam(const mlgi_camera* cam) {
+  M4 v, p;
+  camera_matrices(cam, &v, &p);
+  RayCam rc;
+  rc.vp = mul(p, v);
+  inverse(rc.vp, &rc.inv_vp);
+  rc.eye = {cam->eye[0], cam->eye[1], cam->eye[2]};
+  rc.axis = norm(D3{cam->center[0], cam->center[1], cam->center[2]} - rc.eye);
+  return rc;
+}
+
+// Ray through the pixel centre from the near plane to the far plane.
+void pixel_ray(const RayCam& rc, int x, int y, int w, int h, D3* o, D3* d) {
+  double nx = ((x + 0.5) / w) * 2.0 - 1.0, ny = ((y + 0.5) / h) * 2.0 - 1.0;
+  double a[4] = {nx, ny, -1.0, 1.0}, b[4] = {nx, ny, 1.0, 1.0}, pa[4], pb[4];
+  xform(rc.inv_vp, a, pa);
+  xform(rc.inv_vp, b, pb);
+  D3 p0 = {pa[0] / pa[3], pa[1] / pa[3], pa[2] / pa[3]};
+  D3 p1 = {pb[0] / pb[3], pb[1] / pb[3], pb[2] / pb[3]};
+  *o = p0;
+  *d = p1 - p0;
+}
+
+double window_depth(const RayCam& rc, D3 p) {
+  double v[4] = {p.x, p.y, p.z, 1.0}, c[4];
+  xform(rc.vp, v, c);
+  return 0.5 * (c[2] / c[3]) + 0.5;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlgi_scene_create(uint64_t seed, int32_t n_random, mlgi_scene** out) {
+  if (!out || n_random < 0) return MLGI_ERR_INVALID_ARGUMENT;
+  auto* s = new mlgi_scene();
+  Rng rng(seed);
+  const double L = 10.0;
+  auto quad = [&](D3 a, D3 b, D3 c, D3 d, D3 n) {
+    for (int k = 0; k < 2; ++k) {
+      Tri t;
+      t.v0 = a;
+      t.v1 = k == 0 ? b : c;
+      t.v2 = k == 0 ? c : d;
+      t.n = n;
+      for (int i = 0; i < 3; ++i) t.albedo[i] = (float)(0.2 + 0.6 * rng.uniform());
+      s->tris.push_back(t);
+    }
+  };
+  quad({0, 0, 0}, {L, 0, 0}, {L, 0, L}, {0, 0, L}, {0, 1, 0});    // floor
+  quad({0, L, 0}, {L, L, 0}, {L, L, L}, {0, L, L}, {0, -1, 0});   // ceiling
+  quad({0, 0, 0}, {0, L, 0}, {0, L, L}, {0, 0, L}, {1, 0, 0});    // x = 0
+  quad({L, 0, 0}, {L, L, 0}, {L, L, L}, {L, 0, L}, {-1, 0, 0});   // x = L
+  quad({0, 0, 0}, {L, 0, 0}, {L, L, 0}, {0, L, 0}, {0, 0, 1});    // z = 0
+  quad({0, 0, L}, {L, 0, L}, {L, L, L}, {0, L, L}, {0, 0, -1});   // z = L
+  auto dir = [&]() {
+    double z = 2.0 * rng.uniform() - 1.0, ph = 2.0 * M_PI * rng.uniform();
+    double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+    return D3{r * std::cos(ph), r * std::sin(ph), z};
+  };
+  int made = 0;
+  while (made < n_random) {
+    D3 v0 = {0.5 + 9.0 * rng.uniform(), 0.5 + 9.0 * rng.uniform(), 0.5 + 9.0 * rng.uniform()};
+    D3 e1 = dir() * (0.1 + 0.4 * rng.uniform());
+    D3 e2 = dir() * (0.1 + 0.4 * rng.uniform());
+    float alb[3];
+    for (int i = 0; i < 3; ++i) alb[i] = (float)(0.2 + 0.6 * rng.uniform());
+    D3 c = cross(e1, e2);
+    if (len(c) < 1e-4) continue;
+    if (len(v0 - kDefaultEye) < 1.0) continue;  // keep the default camera clear
+    Tri t;
+    t.v0 = v0;
+    t.v1 = v0 + e1;
+    t.v2 = v0 + e2;
+    t.n = norm(c);
+    std::memcpy(t.albedo, alb, sizeof alb);
+    s->tris.push_back(t);
+    ++made;
+  }
+  for (const Tri& t : s->tris) s->area += 0.5 * len(cross(t.v1 - t.v0, t.v2 - t.v0));
+  build_bvh(s);
+  *out = s;
+  return MLGI_OK;
+}
+
+void mlgi_scene_destroy(mlgi_scene* s) { delete s; }
+
+int32_t mlgi_scene_triangle_count(const mlgi_scene* s) { return s ? (int32_t)s->tris.size() : 0; }
+
+double mlgi_scene_total_area(const mlgi_scene* s) { return s ? s->area : 0.0; }
+
+int mlgi_scene_triangles(const mlgi_scene* s, float* v, float* n, float* a) {
+  if (!s) return MLGI_ERR_INVALID_ARGUMENT;
+  for (size_t i = 0; i < s->tris.size(); ++i) {
+    const Tri& t = s->tris[i];
+    if (v) {
+      const D3* p[3] = {&t.v0, &t.v1, &t.v2};
+      for (int k = 0; k < 3; ++k) {
+        v[i * 9 + k * 3 + 0] = (float)p[k]->x;
+        v[i * 9 + k * 3 + 1] = (float)p[k]->y;
+        v[i * 9 + k * 3 + 2] = (float)p[k]->z;
+      }
+    }
+    if (n) {
+      n[i * 3 + 0] = (float)t.n.x;
+      n[i * 3 + 1] = (float)t.n.y;
+      n[i * 3 + 2] = (float)t.n.z;
+    }
+    if (a) std::memcpy(a + i * 3, t.albedo, sizeof t.albedo);
+  }
+  return MLGI_OK;
+}
+
+void mlgi_scene_default_camera(float aspect, mlgi_camera* c) {
+  if (!c) return;
+  const float eye[3] = {(float)kDefaultEye.x, (float)kDefaultEye.y, (float)kDefaultEye.z};
+  const float ctr[3] = {5.0f, 4.0f, 0.0f};
+  const float up[3] = {0.0f, 1.0f, 0.0f};
+  std::memcpy(c->eye, eye, sizeof eye);
+  std::memcpy(c->center, ctr, sizeof ctr);
+  std::memcpy(c->up, up, sizeof up);
+  c->fovy_deg = 60.0f;
+  c->aspect = aspect;
+  c->z_near = 0.05f;
+  c->z_far = 50.0f;
+}
+
+void mlgi_scene_light_pose(uint64_t seed, mlgi_camera* c) {
+  if (!c) return;
+  Rng rng(seed ^ 0x5107A11C0FFEEull);
+  c->eye[0] = (float)(2.0 + 6.0 * rng.uniform());
+  c->eye[1] = (float)(8.5 + 1.0 * rng.uniform());
+  c->eye[2] = (float)(2.0 + 6.0 * rng.uniform());
+  c->center[0] = (float)(1.0 + 8.0 * rng.uniform());
+  c->center[1] = (float)(0.0 + 2.0 * rng.uniform());
+  c->center[2] = (float)(1.0 + 8.0 * rng.uniform());
+  c->up[0] = 0.0f;
+  c->up[1] = 0.0f;
+  c->up[2] = 1.0f;
+  c->fovy_deg = 45.0f;
+  c->aspect = 1.0f;
+  c->z_near = 0.05f;
+  c->z_far = 50.0f;
+}
+
+int mlgi_camera_matrices(const mlgi_camera* cam, float* view16, float* proj16, float* vp_inv16) {
+  if (!cam) return MLGI_ERR_INVALID_ARGUMENT;
+  M4 v, p;
+  camera_matrices(cam, &v, &p);
+  if (view16) to_float(v, view16);
+  if (proj16) to_float(p, proj16);
+  if (vp_inv16) {
+    M4 inv;
+    if (!inverse(mul(p, v), &inv)) return MLGI_ERR_INVALID_ARGUMENT;
+    to_float(inv, vp_inv16);
+  }
+  return MLGI_OK;
+}
+
+int mlgi_scene_render_gbuffer(const mlgi_scene* s, const mlgi_camera* cam, int32_t w, int32_t h,
+                              int32_t threads, float* depth, float* normal) {
+  if (!s || !cam || w <= 0 || h <= 0 || !depth || !normal) return MLGI_ERR_INVALID_ARGUMENT;
+  const RayCam rc = ray_cam(cam);
+  parallel_rows(h, nthreads(threads), [&](int y) {
+    for (int x = 0; x < w; ++x) {
+      D3 o, d;
+      pixel_ray(rc, x, y, w, h, &o, &d);
+      Hit hit = trace(s, o, d, 1.0);
+      size_t p = (size_t)y * w + x;
+      if (hit.tri < 0) {
+        depth[p] = 1.0f;
+        normal[p * 3] = normal[p * 3 + 1] = normal[p * 3 + 2] = 0.5f;
+        continue;
+      }
+      D3 n = s->tris[hit.tri].n;
+      if (dot(n, d) > 0) n = n * -1.0;
+      depth[p] = (float)window_depth(rc, o + d * hit.t);
+      normal[p * 3 + 0] = (float)n.x * 0.5f + 0.5f;
+      normal[p * 3 + 1] = (float)n.y * 0.5f + 0.5f;
+      normal[p * 3 + 2] = (float)n.z * 0.5f + 0.5f;
+    }
+  });
+  return MLGI_OK;
+}
+
+int mlgi_scene_render_rsm(const mlgi_scene* s, const mlgi_camera* light, int32_t w, int32_t h,
+                          int32_t threads, float* diffuse, float* normal, float* depth,
+                          float* inv_bvp16) {
+  if (!s || !light || w <= 0 || h <= 0 || !diffuse || !normal || !depth)
+    return MLGI_ERR_INVALID_ARGUMENT;
+  const RayCam rc = ray_cam(light);
+  const double cos_half = std::cos((double)light->fovy_deg * M_PI / 180.0 / 2.0);
+  parallel_rows(h, nthreads(threads), [&](int y) {
+    for (int x = 0; x < w; ++x) {
+      D3 o, d;
+      pixel_ray(rc, x, y, w, h, &o, &d);
+      Hit hit = trace(s, o, d, 1.0);
+      size_t p = (size_t)y * w + x;
+      if (hit.tri < 0) {
+        depth[p] = 1.0f;
+        for (int k = 0; k < 3; ++k) {
+          normal[p * 3 + k] = 0.5f;
+          diffuse[p * 3 + k] = 0.0f;
+        }
+        continue;
+      }
+      const Tri& t = s->tris[hit.tri];
+      D3 n = t.n;
+      if (dot(n, d) > 0) n = n * -1.0;
+      const bool in_cone = dot(norm(d), rc.axis) >= cos_half;
+      depth[p] = (float)window_depth(rc, o + d * hit.t);
+      normal[p * 3 + 0] = (float)n.x * 0.5f + 0.5f;
+      normal[p * 3 + 1] = (float)n.y * 0.5f + 0.5f;
+      normal[p * 3 + 2] = (float)n.z * 0.5f + 0.5f;
+      for (int k = 0; k < 3; ++k) diffuse[p * 3 + k] = in_cone ? t.albedo[k] : 0.0f;
+    }
+  });
+  if (inv_bvp16) {
+    M4 bias{};
+    bias.m[0] = 0.5; bias.m[5] = 0.5; bias.m[10] = 0.5;
+    bias.m[12] = 0.5; bias.m[13] = 0.5; bias.m[14] = 0.5; bias.m[15] = 1.0;
+    M4 inv;
+    if (!inverse(mul(bias, rc.vp), &inv)) return MLGI_ERR_INVALID_ARGUMENT;
+    to_float(inv, inv_bvp16);
+  }
+  return MLGI_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+inline uint32_t f2u(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+inline float u2f(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+// random.glsl:34-66: random(vec3) = floatConstruct(hash(uvec3 bits))
+inline uint32_t jhash(uint32_t x) {
+  x += (x << 10u);
+  x ^= (x >> 6u);
+  x += (x << 3u);
+  x ^= (x >> 11u);
+  x += (x << 15u);
+  return x;
+}
+inline float glsl_random3(float a, float b, float c) {
+  uint32_t m = jhash(f2u(a) ^ jhash(f2u(b)) ^ jhash(f2u(c)));
+  m &= 0x007FFFFFu;
+  m |= 0x3F800000u;
+  return u2f(m) - 1.0f;
+}
+// packUnorm4x8 component: round-to-nearest-even of clamp(c, 0, 1) * 255
+inline uint32_t unorm8(float c) {
+  float v = std::min(std::max(c, 0.0f), 1.0f) * 255.0f;
+  return (uint32_t)std::nearbyint(v);
+}
+
+}  // namespace
+
+extern "C" int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, int32_t n_bins,
+                                        int32_t ism_count, uint64_t seed, int32_t threads,
+                                        float* records, uint32_t* bin_offsets) {
+  if (!s || n_points < 0 || n_points > (1ll << 31) - 1 || n_bins <= 0 || ism_count <= 0 ||
+      (!records && n_points) || !bin_offsets)
+    return MLGI_ERR_INVALID_ARGUMENT;
+  const size_t nt = s->tris.size();
+  std::vector<double> cdf(nt);
+  double acc = 0.0;
+  for (size_t i = 0; i < nt; ++i) {
+    const Tri& t = s->tris[i];
+    acc += 0.5 * len(cross(t.v1 - t.v0, t.v2 - t.v0));
+    cdf[i] = acc;
+  }
+  const double total = acc;
+  // each point stands for ism_count points: radius boosted by sqrt(ism_count) (ism.geom:60-63)
+  const double r_world = std::sqrt(total / ((double)std::max<int64_t>(n_points, 1) * M_PI)) *
+                         std::sqrt((double)ism_count);
+  const float r_packed = (float)(r_world / 25.0);
+  std::vector<float> tmp((size_t)n_points * 4);
+  std::vector<int32_t> bin((size_t)n_points);
+  const int nth = nthreads(threads);
+  const int64_t chunk = (n_points + nth - 1) / std::max(nth, 1);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nth; ++t) {
+    pool.emplace_back([&, t]() {
+      const int64_t b = t * chunk, e = std::min<int64_t>(n_points, b + chunk);
+      for (int64_t i = b; i < e; ++i) {
+        double u0 = counter_uniform(seed, (uint64_t)i, 0) * total;
+        size_t ti = (size_t)(std::upper_bound(cdf.begin(), cdf.end(), u0) - cdf.begin());
+        if (ti >= nt) ti = nt - 1;
+        const Tri& tr = s->tris[ti];
+        double r1 = std::sqrt(counter_uniform(seed, (uint64_t)i, 1));
+        double r2 = counter_uniform(seed, (uint64_t)i, 2);
+        D3 p = tr.v0 * (1.0 - r1) + tr.v1 * (r1 * (1.0 - r2)) + tr.v2 * (r1 * r2);
+        float px = (float)p.x, py = (float)p.y, pz = (float)p.z;
+        uint32_t bits = unorm8((float)tr.n.x * 0.5f + 0.5f) | (unorm8((float)tr.n.y * 0.5f + 0.5f) << 8) |
+                        (unorm8((float)tr.n.z * 0.5f + 0.5f) << 16) | (unorm8(r_packed) << 24);
+        tmp[i * 4 + 0] = px;
+        tmp[i * 4 + 1] = py;
+        tmp[i * 4 + 2] = pz;
+        tmp[i * 4 + 3] = u2f(bits);
+        int bi = (int)(glsl_random3(px, py, pz) * (float)n_bins);
+        bin[i] = std::min(std::max(bi, 0), n_bins - 1);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  std::vector<uint32_t> cnt(n_bins + 1, 0);
+  for (int64_t i = 0; i < n_points; ++i) cnt[bin[i] + 1]++;
+  for (int b = 0; b < n_bins; ++b) cnt[b + 1] += cnt[b];
+  std::memcpy(bin_offsets, cnt.data(), sizeof(uint32_t) * (n_bins + 1));
+  std::vector<uint32_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t i = 0; i < n_points; ++i) {
+    uint32_t dst = pos[bin[i]]++;
+    std::memcpy(records + (size_t)dst * 4, &tmp[i * 4], 16);
+  }
+  return MLGI_OK;
+}

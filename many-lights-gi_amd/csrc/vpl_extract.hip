@@ -1,0 +1,80 @@
+// VPL extraction: regular RSM sampling into VPL records.
+// Follows gi/vpl_processor.comp:41-72 and VPLProcessor.cpp:73-101.
+// Compiled with -ffp-contract=off: bit-exact against the FP32 oracle twin.
+//
+// One lane per VPL. Each lane reads its RSM texel (RGB diffuse, encoded
+// normal, depth), unprojects with inverse(bias*P*V) using an explicit fused
+// multiply-add chain, quantises the normal through the 10:10:10 packing, and
+// also stores the lookAtRH basis so the splat and gather never recompute it.
+#include "kernels.hpp"
+#include "mlgi_device.hpp"
+
+namespace mlgi {
+namespace {
+
+using dev::f3;
+
+__global__ __launch_bounds__(64) void vpl_extract_kernel(
+    const float* __restrict__ diffuse, const float* __restrict__ normal,
+    const float* __restrict__ depth, int w, int h, Mat4 m, float light_intensity,
+    const int32_t* __restrict__ shuffle, int n, uint32_t axis_x, uint32_t axis_y,
+    float* __restrict__ records) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint32_t)n) return;
+  // uvec2(resultIndex % x, resultIndex / x) * samplerSize / lightsOnAxis
+  const uint32_t tx = (i % axis_x) * (uint32_t)w / axis_x;
+  const uint32_t ty = (i / axis_x) * (uint32_t)h / axis_y;
+  const float uvx = (float)tx / (float)w;
+  const float uvy = (float)ty / (float)h;
+  float dr = 0.f, dg = 0.f, db = 0.f, nx = 0.f, ny = 0.f, nz = 0.f, d = 0.f;
+  if (tx < (uint32_t)w && ty < (uint32_t)h) {  // out-of-range texelFetch reads 0
+    const size_t p = (size_t)ty * (uint32_t)w + tx;
+    dr = diffuse[p * 3 + 0];
+    dg = diffuse[p * 3 + 1];
+    db = diffuse[p * 3 + 2];
+    nx = normal[p * 3 + 0];
+    ny = normal[p * 3 + 1];
+    nz = normal[p * 3 + 2];
+    d = depth[p];
+  }
+  nx = nx * 2.0f - 1.0f;
+  ny = ny * 2.0f - 1.0f;
+  nz = nz * 2.0f - 1.0f;
+  float r[4];
+#pragma unroll
+  for (int row = 0; row < 4; ++row) {
+    // depth column first: it cancels against the constant column for a
+    // perspective light, and one fused operation keeps that exact
+    float acc = __builtin_fmaf(m.m[2 * 4 + row], d, m.m[3 * 4 + row]);
+    acc = __builtin_fmaf(m.m[1 * 4 + row], uvy, acc);
+    acc = __builtin_fmaf(m.m[0 * 4 + row], uvx, acc);
+    r[row] = acc;
+  }
+  const uint32_t dst = shuffle ? (uint32_t)shuffle[i] : i;
+  const uint32_t pk = dev::pack3sn(nx, ny, nz);
+  const f3 un = dev::unpack3sn(pk);
+  f3 s, u;
+  dev::look_at_rh(un, s, u);
+  float4* out = reinterpret_cast<float4*>(records + (size_t)dst * 16);
+  out[0] = make_float4(r[0] / r[3], r[1] / r[3], r[2] / r[3], __uint_as_float(pk));
+  out[1] = make_float4(un.x, un.y, un.z, s.x);
+  out[2] = make_float4(s.y, s.z, u.x, u.y);
+  out[3] = make_float4(u.z, dr * light_intensity, dg * light_intensity, db * light_intensity);
+}
+
+}  // namespace
+
+hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* normal,
+                              const float* depth, int w, int h, const Mat4& inv_bvp,
+                              float light_intensity, const int32_t* shuffle, int n,
+                              float* records) {
+  uint32_t s1d = 0;
+  while ((uint64_t)s1d * s1d < (uint64_t)n) ++s1d;  // uint(ceil(sqrt(totalVplCount)))
+  const uint32_t axis_y = (uint32_t)n / s1d;          // totalWorkCount / rsmSamples1d
+  const int blocks = (n + 63) / 64;
+  hipLaunchKernelGGL(vpl_extract_kernel, dim3(blocks), dim3(64), 0, s, diffuse, normal, depth, w,
+                     h, inv_bvp, light_intensity, shuffle, n, s1d, axis_y, records);
+  return hipGetLastError();
+}
+
+}  // namespace mlgi

@@ -1,0 +1,392 @@
+"""Python mirror of the mlgi C ABI (libmlgi.so).
+
+Device memory and streams live behind the C ABI; this module only moves
+numpy arrays in and out and names things the way the reference does:
+
+    VPLProcessor.process(rsm, light_intensity, shuffle_lights)
+        VPLProcessor.h:22, VPLProcessor.cpp:73-101
+    ImperfectShadowmap.process(points, vpl_processor, vpl_start_index,
+        vpl_end_index, scale_isms, points_only_into_scaled_isms,
+        tess_level_factor, use_push_pull, z_far)
+        ImperfectShadowmap.h:30-39, ImperfectShadowmap.cpp:184-192
+    GIStage.compute_final_gathering() / GIStage.process()
+        GIStage.cpp:261-306, :346-418
+
+Errors raise MlgiError with the library's message; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from ._native import MlgiError, Params, PRECISION_FP32, PRECISION_MIXED, VPL_RECORD_FLOATS
+
+__all__ = [
+    "MlgiError", "Params", "PRECISION_FP32", "PRECISION_MIXED", "VPL_RECORD_FLOATS",
+    "Context", "DeviceArray", "Scene", "Camera", "VPLProcessor", "ImperfectShadowmap",
+    "GIStage", "default_params", "shuffle_indices", "ism_indices_1d", "device_count",
+]
+
+lib = N.lib
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib.mlgi_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def shuffle_indices(n: int, seed: int = 1979982) -> np.ndarray:
+    out = np.empty(n, np.int32)
+    rc = lib.mlgi_shuffle_indices(n, seed, out.ctypes.data)
+    if rc:
+        raise MlgiError(rc, "mlgi_shuffle_indices")
+    return out
+
+
+def ism_indices_1d(n: int) -> int:
+    return int(lib.mlgi_ism_indices_1d(n))
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    lib.mlgi_device_count(C.byref(n))
+    return n.value
+
+
+class Context:
+    """One HIP device + stream (mlgi_context)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib.mlgi_context_create(device, C.byref(h))
+        if rc:
+            raise MlgiError(rc, f"mlgi_context_create(device={device}) failed")
+        self.h = h
+        self._live: set[int] = set()
+
+    def check(self, rc: int, what: str = "") -> None:
+        if rc:
+            msg = lib.mlgi_last_error(self.h)
+            raise MlgiError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self) -> None:
+        if self.h:
+            for ptr in list(self._live):
+                lib.mlgi_free(self.h, C.c_void_p(ptr))
+            self._live.clear()
+            lib.mlgi_context_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- memory
+    def empty(self, shape, dtype) -> "DeviceArray":
+        return DeviceArray(self, shape, dtype)
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.copy_from(a)
+        return d
+
+    def sync(self) -> None:
+        self.check(lib.mlgi_sync(self.h), "mlgi_sync")
+
+    # -- profiling (PerfCounter analogue)
+    def profile(self, on: bool = True) -> None:
+        self.check(lib.mlgi_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self) -> None:
+        self.check(lib.mlgi_profile_reset(self.h))
+
+    def profile_read(self) -> dict:
+        t = N.StageTimes()
+        self.check(lib.mlgi_profile_read(self.h, C.byref(t)), "mlgi_profile_read")
+        return {k: getattr(t, k) for k, _ in N.StageTimes._fields_}
+
+
+class DeviceArray:
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = C.c_void_p()
+        ctx.check(lib.mlgi_malloc(ctx.h, max(self.nbytes, 1), C.byref(h)), "mlgi_malloc")
+        self.ptr = h.value
+        ctx._live.add(self.ptr)
+
+    def copy_from(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch {a.nbytes} != {self.nbytes}")
+        self.ctx.check(lib.mlgi_memcpy_htod(self.ctx.h, C.c_void_p(self.ptr), a.ctypes.data, a.nbytes),
+                       "mlgi_memcpy_htod")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(lib.mlgi_memcpy_dtoh(self.ctx.h, out.ctypes.data, C.c_void_p(self.ptr), self.nbytes),
+                       "mlgi_memcpy_dtoh")
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.ctx.h:
+            self.ctx._live.discard(self.ptr)
+            lib.mlgi_free(self.ctx.h, C.c_void_p(self.ptr))
+        self.ptr = None
+
+
+# ------------------------------------------------------------------ scene
+@dataclass
+class Camera:
+    eye: tuple
+    center: tuple
+    up: tuple
+    fovy_deg: float
+    aspect: float
+    z_near: float
+    z_far: float
+
+    def to_c(self) -> N.Camera:
+        c = N.Camera()
+        c.eye[:] = self.eye
+        c.center[:] = self.center
+        c.up[:] = self.up
+        c.fovy_deg, c.aspect, c.z_near, c.z_far = self.fovy_deg, self.aspect, self.z_near, self.z_far
+        return c
+
+    @staticmethod
+    def from_c(c: N.Camera) -> "Camera":
+        return Camera(tuple(c.eye), tuple(c.center), tuple(c.up), c.fovy_deg, c.aspect, c.z_near, c.z_far)
+
+    @staticmethod
+    def default(aspect: float) -> "Camera":
+        c = N.Camera()
+        lib.mlgi_scene_default_camera(aspect, C.byref(c))
+        return Camera.from_c(c)
+
+    @staticmethod
+    def light(seed: int) -> "Camera":
+        c = N.Camera()
+        lib.mlgi_scene_light_pose(seed, C.byref(c))
+        return Camera.from_c(c)
+
+    def matrices(self):
+        v = np.empty(16, np.float32)
+        p = np.empty(16, np.float32)
+        inv = np.empty(16, np.float32)
+        rc = lib.mlgi_camera_matrices(C.byref(self.to_c()), v.ctypes.data, p.ctypes.data, inv.ctypes.data)
+        if rc:
+            raise MlgiError(rc, "mlgi_camera_matrices")
+        return v, p, inv
+
+
+class Scene:
+    """Seeded procedural scene (host): room + random triangles."""
+
+    def __init__(self, seed: int, n_random: int):
+        h = C.c_void_p()
+        rc = lib.mlgi_scene_create(seed, n_random, C.byref(h))
+        if rc:
+            raise MlgiError(rc, "mlgi_scene_create")
+        self.h = h
+        self.seed = seed
+        self.n_random = n_random
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.mlgi_scene_destroy(self.h)
+            self.h = None
+
+    @property
+    def triangle_count(self) -> int:
+        return int(lib.mlgi_scene_triangle_count(self.h))
+
+    @property
+    def total_area(self) -> float:
+        return float(lib.mlgi_scene_total_area(self.h))
+
+    def triangles(self):
+        n = self.triangle_count
+        v = np.empty((n, 3, 3), np.float32)
+        nr = np.empty((n, 3), np.float32)
+        a = np.empty((n, 3), np.float32)
+        lib.mlgi_scene_triangles(self.h, v.ctypes.data, nr.ctypes.data, a.ctypes.data)
+        return v, nr, a
+
+    def render_gbuffer(self, cam: Camera, width: int, height: int, threads: int = 0):
+        depth = np.empty((height, width), np.float32)
+        normal = np.empty((height, width, 3), np.float32)
+        rc = lib.mlgi_scene_render_gbuffer(self.h, C.byref(cam.to_c()), width, height, threads,
+                                           depth.ctypes.data, normal.ctypes.data)
+        if rc:
+            raise MlgiError(rc, "mlgi_scene_render_gbuffer")
+        return depth, normal
+
+    def render_rsm(self, light: Camera, width: int, height: int, threads: int = 0):
+        diffuse = np.empty((height, width, 3), np.float32)
+        normal = np.empty((height, width, 3), np.float32)
+        depth = np.empty((height, width), np.float32)
+        inv = np.empty(16, np.float32)
+        rc = lib.mlgi_scene_render_rsm(self.h, C.byref(light.to_c()), width, height, threads,
+                                       diffuse.ctypes.data, normal.ctypes.data, depth.ctypes.data,
+                                       inv.ctypes.data)
+        if rc:
+            raise MlgiError(rc, "mlgi_scene_render_rsm")
+        return diffuse, normal, depth, inv
+
+    def sample_points(self, n_points: int, n_bins: int, ism_count: int, seed: int, threads: int = 0):
+        rec = np.empty((n_points, 4), np.float32)
+        off = np.empty(n_bins + 1, np.uint32)
+        rc = lib.mlgi_scene_sample_points(self.h, n_points, n_bins, ism_count, seed, threads,
+                                          rec.ctypes.data, off.ctypes.data)
+        if rc:
+            raise MlgiError(rc, "mlgi_scene_sample_points")
+        return rec, off
+
+
+# ------------------------------------------------------------ descriptors
+def rsm_desc(diffuse: DeviceArray, normal: DeviceArray, depth: DeviceArray, inv_bvp) -> N.Rsm:
+    r = N.Rsm()
+    r.diffuse, r.face_normal, r.depth = diffuse.ptr, normal.ptr, depth.ptr
+    r.height, r.width = depth.shape
+    r.inv_biased_view_projection[:] = [float(x) for x in np.asarray(inv_bvp, np.float32)]
+    return r
+
+
+def gbuffer_desc(depth: DeviceArray, normal: DeviceArray, vp_inv) -> N.GBuffer:
+    g = N.GBuffer()
+    g.depth, g.face_normal = depth.ptr, normal.ptr
+    g.height, g.width = depth.shape
+    g.view_projection_inverse[:] = [float(x) for x in np.asarray(vp_inv, np.float32)]
+    return g
+
+
+def points_desc(records: DeviceArray, bin_offsets: DeviceArray) -> N.Points:
+    p = N.Points()
+    p.records, p.bin_offsets = records.ptr, bin_offsets.ptr
+    p.count = records.shape[0]
+    p.n_bins = bin_offsets.shape[0] - 1
+    return p
+
+
+# --------------------------------------------------------- stage classes
+class VPLProcessor:
+    """Owns the VPL buffer (vplBuffer + packedVplBuffer of the reference)."""
+
+    def __init__(self, ctx: Context, vpl_count: int):
+        self.ctx = ctx
+        self.count = vpl_count
+        self.records = ctx.empty((vpl_count, VPL_RECORD_FLOATS), np.float32)
+
+    def desc(self) -> N.Vpls:
+        v = N.Vpls()
+        v.records, v.count = self.records.ptr, self.count
+        return v
+
+    def process(self, rsm: N.Rsm, light_intensity: float, shuffle_lights: bool, params: Params | None = None):
+        p = params if params is not None else default_params()
+        p.light_intensity = light_intensity
+        p.shuffle_lights = 1 if shuffle_lights else 0
+        self.ctx.check(lib.mlgi_vpl_extract(self.ctx.h, C.byref(rsm), C.byref(p), C.byref(self.desc())),
+                       "mlgi_vpl_extract")
+
+    def host_records(self) -> np.ndarray:
+        return self.records.numpy()
+
+
+class ImperfectShadowmap:
+    """Owns the atlas ("ISM softrender") and the pull-push result."""
+
+    def __init__(self, ctx: Context, atlas_side: int, vpl_count: int):
+        self.ctx = ctx
+        self.side = atlas_side
+        self.ism_side = atlas_side // ism_indices_1d(vpl_count)
+        self.softrender = ctx.empty((atlas_side, atlas_side), np.uint32)
+        self.pushpull_result = ctx.empty((atlas_side, atlas_side), np.uint16)
+
+    def atlas_desc(self) -> N.Atlas:
+        a = N.Atlas()
+        a.texels, a.side, a.ism_side = self.softrender.ptr, self.side, self.ism_side
+        return a
+
+    def depth_desc(self) -> N.IsmDepth:
+        d = N.IsmDepth()
+        d.texels, d.side, d.ism_side = self.pushpull_result.ptr, self.side, self.ism_side
+        return d
+
+    def render(self, points: N.Points, vpl_processor: VPLProcessor, params: Params) -> None:
+        self.ctx.check(lib.mlgi_ism_splat(self.ctx.h, C.byref(points), C.byref(vpl_processor.desc()),
+                                          C.byref(params), C.byref(self.atlas_desc())), "mlgi_ism_splat")
+
+    def pullpush(self, params: Params) -> None:
+        self.ctx.check(lib.mlgi_ism_pullpush(self.ctx.h, C.byref(self.atlas_desc()), C.byref(params),
+                                             C.byref(self.depth_desc())), "mlgi_ism_pullpush")
+
+    def process(self, points: N.Points, vpl_processor: VPLProcessor, vpl_start_index: int,
+                vpl_end_index: int, scale_isms: bool, points_only_into_scaled_isms: bool,
+                tess_level_factor: float, use_push_pull: bool, z_far: float,
+                params: Params | None = None) -> None:
+        p = params if params is not None else default_params()
+        p.vpl_start_index, p.vpl_end_index = vpl_start_index, vpl_end_index
+        p.scale_isms = int(scale_isms)
+        p.points_only_into_scaled_isms = int(points_only_into_scaled_isms)
+        p.tess_level_factor = tess_level_factor
+        p.use_push_pull = int(use_push_pull)
+        p.z_far = z_far
+        p.atlas_side = self.side
+        self.ctx.check(lib.mlgi_ism_process(self.ctx.h, C.byref(points), C.byref(vpl_processor.desc()),
+                                            C.byref(p), C.byref(self.atlas_desc()),
+                                            C.byref(self.depth_desc())), "mlgi_ism_process")
+
+
+class GIStage:
+    """Per-frame GI driver: VPLP -> ISM (splat + pull-push) -> FG."""
+
+    def __init__(self, ctx: Context, params: Params, vpl_count: int, width: int, height: int):
+        self.ctx = ctx
+        self.params = params
+        self.vpl_processor = VPLProcessor(ctx, vpl_count)
+        self.ism = ImperfectShadowmap(ctx, params.atlas_side, vpl_count)
+        self.gi_buffer = ctx.empty((height, width, 3), np.float32)
+        self.width, self.height = width, height
+
+    def image_desc(self) -> N.Image:
+        im = N.Image()
+        im.rgb, im.width, im.height = self.gi_buffer.ptr, self.width, self.height
+        return im
+
+    def compute_final_gathering(self, gbuffer: N.GBuffer) -> None:
+        self.ctx.check(lib.mlgi_final_gather(self.ctx.h, C.byref(gbuffer), C.byref(self.vpl_processor.desc()),
+                                             C.byref(self.ism.depth_desc()), C.byref(self.params),
+                                             C.byref(self.image_desc())), "mlgi_final_gather")
+
+    def process(self, rsm: N.Rsm, points: N.Points, gbuffer: N.GBuffer) -> None:
+        v, a, d, im = self.vpl_processor.desc(), self.ism.atlas_desc(), self.ism.depth_desc(), self.image_desc()
+        self.ctx.check(lib.mlgi_frame(self.ctx.h, C.byref(rsm), C.byref(points), C.byref(gbuffer),
+                                      C.byref(self.params), C.byref(v), C.byref(a), C.byref(d),
+                                      C.byref(im)), "mlgi_frame")
+
+    def count_lit_pairs(self, gbuffer: N.GBuffer) -> int:
+        out = C.c_uint64(0)
+        self.ctx.check(lib.mlgi_count_lit_pairs(self.ctx.h, C.byref(gbuffer), C.byref(self.vpl_processor.desc()),
+                                                C.byref(self.params), C.byref(out)), "mlgi_count_lit_pairs")
+        return int(out.value)

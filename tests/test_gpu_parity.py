@@ -1,0 +1,121 @@
+"""GPU parity: every stage through the C ABI against the oracle.
+
+Bars (stage-wise: each oracle stage consumes the GPU's previous stage):
+  * VPL records: bit-exact vs the FP32 twin (positions, packed and unpacked
+    normals, flux, lookAtRH basis); positions within 1e-5 of FP64.
+  * ISM atlas: bit-exact vs the FP32 twin (integer atomicMin is order-free).
+  * Pull-push R16: bit-exact vs the FP32 twin.
+  * Irradiance: rel-L2 <= 1e-4 vs the FP64 transcription (plain FP32 form).
+"""
+import numpy as np
+import pytest
+
+import mlgi
+import oracle
+from mlgi import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+REL_L2_FP32 = 1e-4
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+class Frame:
+    """Runs one frame stage by stage and keeps host copies of every output."""
+
+    def __init__(self, ctx, wl, **pkw):
+        self.wl = wl
+        self.inp = inp = W.build_inputs(wl)
+        self.params = p = wl.params(**pkw)
+        self.stage = st = mlgi.GIStage(ctx, p, wl.n_vpl, wl.width, wl.height)
+        self.rsm = mlgi.rsm_desc(ctx.upload(inp.rsm_diffuse), ctx.upload(inp.rsm_normal),
+                                 ctx.upload(inp.rsm_depth), inp.inv_bvp)
+        self.pts = mlgi.points_desc(ctx.upload(inp.points), ctx.upload(inp.bin_offsets))
+        self.gb = mlgi.gbuffer_desc(ctx.upload(inp.gb_depth), ctx.upload(inp.gb_normal), inp.vp_inv)
+        print(f"[frame] {wl.name} {wl.width}x{wl.height} N={wl.n_vpl} I={wl.interleave} "
+              f"P={wl.n_points} params={pkw}", flush=True)
+        st.vpl_processor.process(self.rsm, p.light_intensity, bool(p.shuffle_lights), p)
+        self.rec = st.vpl_processor.host_records()
+        st.ism.render(self.pts, st.vpl_processor, p)
+        self.atlas = st.ism.softrender.numpy()
+        st.ism.pullpush(p)
+        self.r16 = st.ism.pushpull_result.numpy()
+        st.compute_final_gathering(self.gb)
+        self.img = st.gi_buffer.numpy()
+
+    @property
+    def vpl(self):
+        r = self.rec
+        return r[:, 0:3], r[:, 4:7], r[:, 13:16]
+
+
+@pytest.fixture(scope="module", params=["tiny", "small", "1080p"])
+def frame(request, ctx):
+    return Frame(ctx, W.WORKLOADS[request.param])
+
+
+def test_vpl_extract_bit_exact(frame):
+    inp, wl = frame.inp, frame.wl
+    sh = mlgi.shuffle_indices(wl.n_vpl)
+    pos, nrm, pk, col = oracle.vpl_extract(inp.rsm_diffuse, inp.rsm_normal, inp.rsm_depth, inp.inv_bvp,
+                                           frame.params.light_intensity, sh, wl.n_vpl, np.float32)
+    r = frame.rec
+    assert np.array_equal(r[:, 0:3], pos)
+    assert np.array_equal(r[:, 3].view(np.uint32), pk)
+    assert np.array_equal(r[:, 4:7], nrm)
+    assert np.array_equal(r[:, 13:16], col)
+    for i in range(0, wl.n_vpl, max(1, wl.n_vpl // 64)):
+        s, u = oracle.look_at_rh(r[i, 4:7], np.float32)
+        assert np.array_equal(r[i, 7:10], s) and np.array_equal(r[i, 10:13], u), i
+    pos64, _, pk64, _ = oracle.vpl_extract(inp.rsm_diffuse, inp.rsm_normal, inp.rsm_depth, inp.inv_bvp,
+                                           frame.params.light_intensity, sh, wl.n_vpl, np.float64)
+    assert np.abs(r[:, 0:3] - pos64).max() <= 1e-5 * 10.0
+
+
+def test_ism_splat_bit_exact(frame):
+    pos, nrm, _ = frame.vpl
+    ref = oracle.ism_splat(frame.inp.points, frame.inp.bin_offsets, pos, nrm, frame.wl.atlas_side,
+                           frame.params.z_far)
+    assert (ref != 0xFFFFFFFF).any()
+    mism = int((frame.atlas != ref).sum())
+    assert mism == 0, f"{mism} atlas texels differ"
+
+
+def test_pullpush_bit_exact(frame):
+    ref = oracle.pullpush(frame.atlas, frame.wl.ism_side, frame.wl.levels, frame.params.z_far)
+    mism = int((frame.r16 != ref).sum())
+    assert mism == 0, f"{mism} R16 texels differ"
+
+
+def test_final_gather_fp32_vs_fp64(frame):
+    pos, nrm, col = frame.vpl
+    p = frame.params
+    ref, lit = oracle.final_gather(frame.inp.gb_depth, frame.inp.gb_normal, frame.inp.vp_inv, pos, nrm, col,
+                                   frame.r16, frame.wl.atlas_side, p.z_far, p.gi_intensity_factor,
+                                   p.vpl_clamping_value, frame.wl.interleave)
+    assert lit > 0
+    err = rel_l2(frame.img, ref)
+    print(f"[fg] {frame.wl.name} rel-L2 {err:.3e} lit pairs {lit}")
+    assert err <= REL_L2_FP32
+
+
+def test_frame_matches_stages(frame, ctx):
+    st = frame.stage
+    st.process(frame.rsm, frame.pts, frame.gb)
+    ctx.sync()
+    assert np.array_equal(st.ism.softrender.numpy(), frame.atlas)
+    assert np.array_equal(st.ism.pushpull_result.numpy(), frame.r16)
+    assert np.array_equal(st.gi_buffer.numpy(), frame.img)
+
+
+def test_count_lit_pairs(frame):
+    pos, nrm, col = frame.vpl
+    p = frame.params
+    _, lit = oracle.final_gather(frame.inp.gb_depth, frame.inp.gb_normal, frame.inp.vp_inv, pos, nrm, col,
+                                 None, frame.wl.atlas_side, p.z_far, p.gi_intensity_factor,
+                                 p.vpl_clamping_value, frame.wl.interleave, shadowing=False)
+    got = frame.stage.count_lit_pairs(frame.gb)
+    assert abs(got - lit) <= max(2, 1e-5 * lit), (got, lit)
