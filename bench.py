@@ -178,7 +178,7 @@ def main() -> int:
             rec = v["rec"]
             p = wl.params()
             ref, _ = oracle.final_gather(inp.gb_depth, inp.gb_normal, inp.vp_inv, rec[:, 0:3], rec[:, 4:7],
-                                         rec[:, 13:16], v["r16"], wl.atlas_side, p.z_far, p.gi_intensity_factor,
+                                         rec[:, 16:19], v["r16"], wl.atlas_side, p.z_far, p.gi_intensity_factor,
                                          p.vpl_clamping_value, wl.interleave, threads=a.cpu_threads)
             out["forms"][k]["rel_l2_vs_fp64"] = float(np.linalg.norm(v["img"] - ref) / np.linalg.norm(ref))
     if not a.no_cpu:

@@ -76,6 +76,7 @@ typedef struct mlgi_params {
   int32_t pull_push_levels;       /* pull levels L (reference: 6) */
   int32_t precision;              /* mlgi_precision */
   int32_t tile_w, tile_h;         /* final-gather tile in interleave-class pixels; 0 = auto */
+  int32_t gather_fetch;           /* final-gather VPL fetch variant 0..2; -1 = auto */
 } mlgi_params;
 
 /* RSM inputs (RasterizationStage outputs bound in VPLProcessor.cpp:84-86).
@@ -88,14 +89,15 @@ typedef struct mlgi_rsm {
   float inv_biased_view_projection[16]; /* column-major inverse(bias*P*V) */
 } mlgi_rsm;
 
-/* VPL records, MLGI_VPL_RECORD_FLOATS floats per VPL, 64-byte aligned:
- *  [0..2] world position   [3] packed normal (pack3SNToFloat bits, u32)
- *  [4..6] unpacked normal  [7..9] lookAtRH s row  [10..12] lookAtRH u row
- *  [13..15] flux (diffuse * lightIntensity)
+/* VPL records, MLGI_VPL_RECORD_FLOATS floats (128 bytes) per VPL:
+ *  [0..2]   world position          [3]  plane offset n . position
+ *  [4..6]   unpacked normal         [7]  packed normal (pack3SNToFloat bits, u32)
+ *  [8..10]  lookAtRH s row          [12..14] lookAtRH u row
+ *  [16..18] flux (diffuse * lightIntensity); all other slots are 0.
  * (VPLProcessor vplBuffer + packedVplBuffer, VPLProcessor.cpp:29-41). */
-#define MLGI_VPL_RECORD_FLOATS 16
+#define MLGI_VPL_RECORD_FLOATS 32
 typedef struct mlgi_vpls {
-  float* records; /* device, count * 16 floats */
+  float* records; /* device, count * MLGI_VPL_RECORD_FLOATS floats */
   int32_t count;
 } mlgi_vpls;
 

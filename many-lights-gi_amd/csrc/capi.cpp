@@ -240,6 +240,7 @@ void mlgi_params_default(mlgi_params* p) {
   p->precision = MLGI_PRECISION_FP32;
   p->tile_w = 0;
   p->tile_h = 0;
+  p->gather_fetch = -1;
 }
 
 int mlgi_malloc(mlgi_context* c, size_t bytes, void** dptr) {
@@ -409,6 +410,7 @@ int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
     a->atlas_side = ism->side;
     a->ism_side = s;
     a->n1d = n1d;
+    a->n1d_log2 = log2i(n1d);
   } else {
     a->atlas_side = 1;
     a->ism_side = 1;
@@ -425,6 +427,10 @@ int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
   a->interleave = il;
   a->interleave_log2 = log2i(il);
   a->z_far = p->z_far;
+  // smoothstep(1 - bias, 1, 1 - (d/zFar - occ)) = smoothstep of t = 1 + (occ - d/zFar) / bias
+  const double bias = (double)0.02f;
+  a->shadow_k1 = (float)(1.0 / (bias * (double)p->z_far));
+  a->shadow_k2 = (float)(1.0 / (bias * 65535.0));
   a->clamp_value = p->vpl_clamping_value;
   a->out_scale = p->gi_intensity_factor / (float)(vpl_end - p->vpl_start_index) * (float)classes;
   a->shadowing = p->enable_shadowing ? 1 : 0;
@@ -452,9 +458,11 @@ int mlgi_final_gather(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
   pick_tile(p, &tw, &th);
   if (!((tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 8) || (tw == 16 && th == 4)))
     return fail(c, MLGI_ERR_UNSUPPORTED, "tile %dx%d not in {8x8, 16x16, 32x8, 16x4}", tw, th);
+  const int fetch = p->gather_fetch >= 0 ? p->gather_fetch : 0;
+  if (fetch > 2) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "gather_fetch %d not in 0..2", fetch);
   a.out = out->rgb;
   Scope sc(c, kGather);
-  return hip_check(c, mlgi::launch_final_gather(c->stream, a, p->precision, tw, th), "final_gather");
+  return hip_check(c, mlgi::launch_final_gather(c->stream, a, p->precision, tw, th, fetch), "final_gather");
 }
 
 int mlgi_frame(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_points* pts,

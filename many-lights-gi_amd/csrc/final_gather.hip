@@ -6,15 +6,27 @@
 // k = (x mod I) + (y mod I) * I and sums the VPLs [k*N/I^2, (k+1)*N/I^2).
 // A workgroup owns a TW x TH tile of one class's pixels (pixels I apart on
 // screen, like the reference's 8x8 workgroups); each wave64 of it owns 64 of
-// those pixels, so every lane of a wave walks the same VPL list and each VPL
-// record is wave-uniform: it is fetched with scalar loads and used from SGPRs.
+// those pixels, so every lane of a wave walks the same VPL list. The class's
+// VPLs are staged in LDS 64 at a time, one record per lane of the workgroup;
+// the waves then read each record with uniform (broadcast) LDS reads, the next
+// one fetched while the current one is processed.
 //
 // Culling (replaces the clustered light lists, light_lists.comp:104-119).
-// Each wave reduces the world-space AABB of its pixels; lane j tests VPL j of
-// a 64-VPL chunk: if the VPL's tangent plane has the whole box behind it by a
-// margin far above FP32 rounding, every pixel of the wave would compute
-// angleFactor == 0 for it, so dropping it leaves the sum bit-identical. A
-// ballot turns the tests into a 64-bit survivor mask the wave iterates.
+// Each wave reduces the bounds of its pixels' positions, normals and plane
+// offsets; lane j tests VPL j of a 64-VPL chunk against them:
+//   * the VPL's tangent plane has the whole position box behind it, or
+//   * the VPL lies behind the tangent plane of every pixel,
+// each by a margin far above FP32 rounding. Either way every pixel would get
+// angleFactor == 0 from that VPL. A ballot turns the tests into a 64-bit
+// survivor mask the wave iterates.
+//
+// Per pair, the hemisphere tests come first and use only fused multiply-adds
+// against the VPL's plane offset n.p (stored in the record) and the pixel's
+// own n.p; the reciprocal square root, projection and ISM gather run only for
+// lit pairs. The geometry term is written as -(n_v.d)(n_p.d)/|d|^6, and the
+// paraboloid coordinate (s.d/|d|) / (1 - v.z) as (s.d) / (|d| + n_v.d), one
+// reciprocal for both axes, evaluated in ISM-local texels so that no FP32
+// rounding at atlas scale enters the NEAREST decision.
 //
 // Workgroup order is XCD-aware: consecutive workgroups go to different XCDs
 // (round robin), so the linear work index is remapped such that the I^2
@@ -27,12 +39,19 @@ namespace {
 
 constexpr int kXcds = 8;
 
-struct VplRec {
-  float px, py, pz, packed;
+// VPL fields the gather uses, as staged in LDS (64 bytes).
+struct VplLds {
+  float px, py, pz, plane;
   float nx, ny, nz, sx;
   float sy, sz, ux, uy;
   float uz, cr, cg, cb;
 };
+
+__device__ __forceinline__ VplLds load_vpl(const float* rec) {
+  const float4* r = reinterpret_cast<const float4*>(rec);
+  const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
+  return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
+}
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -51,10 +70,11 @@ struct Pixel {
   float depth;
   float fx, fy, fz;  // world position
   float nx, ny, nz;  // decoded face normal
+  float plane;       // n_p . position
 };
 
-// Pixel of lane (lx, ly) of tile (tile_x, tile_y) in class k, and its
-// reconstruction: ndc = vec4(uv, depth, 1)*2-1, world = VPinv*ndc / w.
+// Pixel (cx, cy) of class (kx, ky) and its reconstruction:
+// ndc = vec4(uv, depth, 1)*2-1, world = VPinv*ndc / w (depth column first).
 __device__ __forceinline__ Pixel load_pixel(const GatherArgs& a, int cx, int cy, int kx, int ky) {
   Pixel p;
   p.x = cx * a.interleave + kx;
@@ -79,24 +99,57 @@ __device__ __forceinline__ Pixel load_pixel(const GatherArgs& a, int cx, int cy,
   p.nx = n0 * 2.0f - 1.0f;
   p.ny = n1 * 2.0f - 1.0f;
   p.nz = n2 * 2.0f - 1.0f;
+  p.plane = __builtin_fmaf(p.nx, p.fx, __builtin_fmaf(p.ny, p.fy, p.nz * p.fz));
   return p;
 }
 
-// Survivor mask of the 64-VPL chunk starting at VPL `first` (count valid).
-__device__ __forceinline__ unsigned long long cull_chunk(const GatherArgs& a, int first, int count,
-                                                         float cx, float cy, float cz, float hx,
-                                                         float hy, float hz) {
+struct WaveBounds {
+  float cx, cy, cz, hx, hy, hz;        // position box centre / half extent
+  float nlx, nly, nlz, nhx, nhy, nhz;  // normal box
+  float plane_lo;                      // min of n_p . p
+  float scale;                         // coordinate magnitude for the margins
+};
+
+// Survivor mask of the `count` staged VPLs of the current chunk.
+__device__ __forceinline__ unsigned long long cull_chunk(const VplLds* tile, int count,
+                                                         const WaveBounds& w) {
   const int lane = threadIdx.x & 63;
   bool keep = false;
   if (lane < count) {
-    const float4* r = reinterpret_cast<const float4*>(a.vpl + (size_t)(first + lane) * 16);
-    const float4 r0 = r[0], r1 = r[1];
-    const float dx = cx - r0.x, dy = cy - r0.y, dz = cz - r0.z;
-    const float ax = fabsf(r1.x), ay = fabsf(r1.y), az = fabsf(r1.z);
-    // max over the box of n . (p - p_vpl) = n . (c - p_vpl) + |n| . h
-    const float support = r1.x * dx + r1.y * dy + r1.z * dz + (ax * hx + ay * hy + az * hz);
-    const float mag = ax * fabsf(dx) + ay * fabsf(dy) + az * fabsf(dz) + (ax * hx + ay * hy + az * hz);
-    keep = !(support < -1e-4f * mag - 1e-20f);
+    const VplLds& v = tile[lane];
+    // (1) max over the box of n . (p - p_vpl) = n . (c - p_vpl) + |n| . h
+    const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
+    const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
+    const float spread = ax * w.hx + ay * w.hy + az * w.hz;
+    const float support = v.nx * dx + v.ny * dy + v.nz * dz + spread;
+    const float mag = ax * fabsf(dx) + ay * fabsf(dy) + az * fabsf(dz) + spread;
+    const float tol = 1e-4f * mag + 1e-6f * (w.scale + fabsf(v.plane));
+    const bool behind_vpl = support < -tol;
+    // (2) min over pixels of n_p . (p - p_vpl) >= plane_lo - max_{n in box} n . p_vpl
+    const float mx = fmaxf(w.nlx * v.px, w.nhx * v.px) + fmaxf(w.nly * v.py, w.nhy * v.py) +
+                     fmaxf(w.nlz * v.pz, w.nhz * v.pz);
+    const float tol2 = 1e-6f * (w.scale + fabsf(v.px) + fabsf(v.py) + fabsf(v.pz));
+    const bool behind_pixels = w.plane_lo - mx > tol2;
+    keep = !(behind_vpl || behind_pixels);
+  }
+  return __ballot(keep);
+}
+
+// Same test on a record held by the lane itself.
+__device__ __forceinline__ unsigned long long cull_one(const VplLds& v, bool present,
+                                                       const WaveBounds& w) {
+  bool keep = false;
+  if (present) {
+    const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
+    const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
+    const float spread = ax * w.hx + ay * w.hy + az * w.hz;
+    const float support = v.nx * dx + v.ny * dy + v.nz * dz + spread;
+    const float mag = ax * fabsf(dx) + ay * fabsf(dy) + az * fabsf(dz) + spread;
+    const float tol = 1e-4f * mag + 1e-6f * (w.scale + fabsf(v.plane));
+    const float mx = fmaxf(w.nlx * v.px, w.nhx * v.px) + fmaxf(w.nly * v.py, w.nhy * v.py) +
+                     fmaxf(w.nlz * v.pz, w.nhz * v.pz);
+    const float tol2 = 1e-6f * (w.scale + fabsf(v.px) + fabsf(v.py) + fabsf(v.pz));
+    keep = !(support < -tol || w.plane_lo - mx > tol2);
   }
   return __ballot(keep);
 }
@@ -105,10 +158,10 @@ __device__ __forceinline__ unsigned long long cull_chunk(const GatherArgs& a, in
 // FP32 local coordinate lies within kTie of a texel edge, closer than its
 // rounding error can be trusted. Rebuilds the pixel position, the lookAtRH
 // basis and the projection from the FP32 inputs in double.
-constexpr float kTie = 1e-4f;
+constexpr float kTie = 2.5e-4f;
 
 __device__ __forceinline__ void refine_texel(const GatherArgs& a, int x, int y, float depth,
-                                             const VplRec& v, float* tx, float* ty) {
+                                             const VplLds& v, int* tx, int* ty) {
   const float* m = a.vp_inv.m;
   const double c0 = (double)x / (double)a.width * 2.0 - 1.0;
   const double c1 = (double)y / (double)a.height * 2.0 - 1.0;
@@ -130,11 +183,23 @@ __device__ __forceinline__ void refine_texel(const GatherArgs& a, int x, int y, 
   const double lx = (sx * dx + sy * dy + sz * dz) / dist / den;
   const double ly = (ux * dx + uy * dy + uz * dz) / dist / den;
   const double S = (double)a.ism_side;
-  *tx = (float)floor((lx + 1.0) * 0.5 * S);
-  *ty = (float)floor((ly + 1.0) * 0.5 * S);
+  *tx = (int)floor((lx + 1.0) * 0.5 * S);
+  *ty = (int)floor((ly + 1.0) * 0.5 * S);
 }
 
-template <int TW, int TH, bool kShadow>
+// Record fetch variants (picked per shape by the autotuner):
+//   kFetchLds     chunk staged in LDS, full next record prefetched
+//   kFetchLdsTest chunk staged in LDS, only the hemisphere-test fields
+//                 prefetched, the rest read when a lane is lit
+//   kFetchScalar  records read from global with wave-uniform (scalar) loads,
+//                 next record prefetched into SGPRs
+enum FetchVariant { kFetchLds = 0, kFetchLdsTest = 1, kFetchScalar = 2 };
+
+struct VplTest {
+  float px, py, pz, plane, nx, ny, nz;
+};
+
+template <int TW, int TH, bool kShadow, int kFetch>
 __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int tiles_x,
                                                               int num_work, int per_xcd) {
   const int b = blockIdx.x;
@@ -148,69 +213,123 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
   const Pixel p = load_pixel(a, tile_x * TW + lx, tile_y * TH + ly, kx, ky);
 
-  // wave-level bounds of the pixels' world positions
+  // ---- wave bounds for culling
   const float inf = __builtin_inff();
+  WaveBounds wb;
   const float mnx = wave_min(p.valid ? p.fx : inf), mxx = wave_max(p.valid ? p.fx : -inf);
-  const float mny = wave_min(p.valid ? p.fy : inf), mxy = wave_max(p.valid ? p.fy : -inf);
-  const float mnz = wave_min(p.valid ? p.fz : inf), mxz = wave_max(p.valid ? p.fz : -inf);
-  if (!(mnx <= mxx)) return;  // no pixel of this wave is on screen (wave-uniform)
-  const float bcx = 0.5f * (mnx + mxx), bcy = 0.5f * (mny + mxy), bcz = 0.5f * (mnz + mxz);
-  const float bhx = 0.5f * (mxx - mnx) * 1.0001f, bhy = 0.5f * (mxy - mny) * 1.0001f,
-              bhz = 0.5f * (mxz - mnz) * 1.0001f;
+  const bool wave_live = mnx <= mxx;  // some pixel of this wave is on screen (wave-uniform)
+  {
+    const float mny = wave_min(p.valid ? p.fy : inf), mxy = wave_max(p.valid ? p.fy : -inf);
+    const float mnz = wave_min(p.valid ? p.fz : inf), mxz = wave_max(p.valid ? p.fz : -inf);
+    wb.cx = 0.5f * (mnx + mxx);
+    wb.cy = 0.5f * (mny + mxy);
+    wb.cz = 0.5f * (mnz + mxz);
+    wb.hx = 0.5f * (mxx - mnx) * 1.0001f;
+    wb.hy = 0.5f * (mxy - mny) * 1.0001f;
+    wb.hz = 0.5f * (mxz - mnz) * 1.0001f;
+    wb.nlx = wave_min(p.valid ? p.nx : inf);
+    wb.nhx = wave_max(p.valid ? p.nx : -inf);
+    wb.nly = wave_min(p.valid ? p.ny : inf);
+    wb.nhy = wave_max(p.valid ? p.ny : -inf);
+    wb.nlz = wave_min(p.valid ? p.nz : inf);
+    wb.nhz = wave_max(p.valid ? p.nz : -inf);
+    wb.plane_lo = wave_min(p.valid ? p.plane : inf);
+    wb.scale = 4.0f * fmaxf(fmaxf(fmaxf(fabsf(mnx), fabsf(mxx)), fmaxf(fabsf(mny), fabsf(mxy))),
+                            fmaxf(fabsf(mnz), fabsf(mxz)));
+  }
 
+  __shared__ VplLds vtile[64];
   const int first = k * a.per_class;
-  const float S = (float)a.ism_side;
-  const float half_s = 0.5f * S;
+  const float hs = 0.5f * (float)a.ism_side;  // ((p + 1) / 2) * S = p * S/2 + S/2
   const int amask = a.atlas_side - 1;
-  const float inv_zfar = 1.0f / a.z_far;
-  const float e0 = 1.0f - 0.02f;  // smoothstep(1.0 - shadowBias, 1.0, .)
-  const float inv_width = 1.0f / (1.0f - e0);
+  const int alog = 31 - __builtin_clz(a.atlas_side);
+  const float k1 = a.shadow_k1;  // smoothstep slope / zFar
+  const float k2 = a.shadow_k2;  // smoothstep slope / 65535
   float ar = 0.0f, ag = 0.0f, ab = 0.0f;
 
   for (int chunk = 0; chunk < a.per_class; chunk += 64) {
     const int count = min(64, a.per_class - chunk);
-    unsigned long long mask = cull_chunk(a, first + chunk, count, bcx, bcy, bcz, bhx, bhy, bhz);
-    while (mask) {
-      const int j = __builtin_ctzll(mask);
+    const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
+    if (kFetch != kFetchScalar) {
+      // ---- stage the chunk's VPLs in LDS (one record per lane)
+      if (chunk > 0) __syncthreads();
+      if ((int)threadIdx.x < count) vtile[threadIdx.x] = load_vpl(crec + (size_t)threadIdx.x * kVplFloats);
+      __syncthreads();
+    }
+    if (!wave_live) continue;
+    unsigned long long mask;
+    if (kFetch != kFetchScalar) {
+      mask = cull_chunk(vtile, count, wb);
+    } else {
+      const int lane = threadIdx.x & 63;
+      VplLds mine{};
+      if (lane < count) mine = load_vpl(crec + (size_t)lane * kVplFloats);
+      mask = cull_one(mine, lane < count, wb);
+    }
+    if (!mask) continue;
+    int j = __builtin_ctzll(mask);
+    auto fetch_test = [&](int jj) -> VplTest {
+      if (kFetch == kFetchScalar) {
+        const float4* r = reinterpret_cast<const float4*>(crec + (size_t)jj * kVplFloats);
+        const float4 a0 = r[0], b0 = r[1];
+        return VplTest{a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, b0.z};
+      }
+      const VplLds& t = vtile[jj];
+      return VplTest{t.px, t.py, t.pz, t.plane, t.nx, t.ny, t.nz};
+    };
+    auto fetch_full = [&](int jj) -> VplLds {
+      if (kFetch == kFetchScalar) return load_vpl(crec + (size_t)jj * kVplFloats);
+      return vtile[jj];
+    };
+    VplTest vt = fetch_test(j);
+    VplLds vf = (kFetch == kFetchLdsTest) ? VplLds{} : fetch_full(j);
+    while (true) {
       mask &= mask - 1ull;
+      const int jn = mask ? __builtin_ctzll(mask) : j;
+      // next record, fetched while this one is processed
+      const VplTest vtn = fetch_test(jn);
+      const VplLds vfn = (kFetch == kFetchLdsTest) ? VplLds{} : fetch_full(jn);
       const int vi = first + chunk + j;
-      const VplRec v = *reinterpret_cast<const VplRec*>(a.vpl + (size_t)vi * 16);
-      const float dx = p.fx - v.px, dy = p.fy - v.py, dz = p.fz - v.pz;
-      const float d2 = dx * dx + dy * dy + dz * dz;
-      const float inv = __builtin_amdgcn_rsqf(d2);
-      const float ndot = v.nx * dx + v.ny * dy + v.nz * dz;
-      const float cv = ndot * inv;
-      const float cp = -(p.nx * dx + p.ny * dy + p.nz * dz) * inv;
-      const float angle = fmaxf(cv, 0.0f) * fmaxf(cp, 0.0f);
-      if (angle > 0.0f) {
+      // n_vpl . (p - p_vpl) and n_p . (p - p_vpl) from the plane offsets
+      const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
+      const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
+      if (ndot > 0.0f && pdot < 0.0f) {
+        const VplLds v = (kFetch == kFetchLdsTest) ? vtile[j] : vf;
+        const float dx = p.fx - vt.px, dy = p.fy - vt.py, dz = p.fz - vt.pz;
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float inv = __builtin_amdgcn_rsqf(d2);
         const float inv2 = inv * inv;
-        float g = fminf(angle * (inv2 * inv2), a.clamp_value);
+        // cos_v cos_p / d^4 = -(ndot pdot) / d^6
+        float g = fminf(-(ndot * pdot) * inv2 * (inv2 * inv2), a.clamp_value);
         if (kShadow) {
-          // paraboloid_project(diff, dist, n, zFar, vi, n1d, false), evaluated
-          // in ISM-local texel coordinates: ((p+1)/2) * S = p*S/2 + S/2
-          const float lxv = (v.sx * dx + v.sy * dy + v.sz * dz) * inv;
-          const float lyv = (v.ux * dx + v.uy * dy + v.uz * dz) * inv;
-          const float rden = __builtin_amdgcn_rcpf(1.0f + cv);  // 1 / (1 - v.z)
-          const float cx = __builtin_fmaf(lxv * rden, half_s, half_s);
-          const float cy = __builtin_fmaf(lyv * rden, half_s, half_s);
-          float tx = floorf(cx), ty = floorf(cy);
-          const float rx = cx - tx, ry = cy - ty;
-          if (fminf(fminf(rx, 1.0f - rx), fminf(ry, 1.0f - ry)) < kTie)
+          const float dist = d2 * inv;
+          const float rh = hs * __builtin_amdgcn_rcpf(dist + ndot);
+          const float sd = __builtin_fmaf(v.sx, dx, __builtin_fmaf(v.sy, dy, v.sz * dz));
+          const float ud = __builtin_fmaf(v.ux, dx, __builtin_fmaf(v.uy, dy, v.uz * dz));
+          const float cx = __builtin_fmaf(sd, rh, hs);
+          const float cy = __builtin_fmaf(ud, rh, hs);
+          const float fx = __builtin_amdgcn_fractf(cx), fy = __builtin_amdgcn_fractf(cy);
+          int tx = (int)(cx - fx), ty = (int)(cy - fy);
+          if (fmaxf(fabsf(fx - 0.5f), fabsf(fy - 0.5f)) > 0.5f - kTie)
             refine_texel(a, p.x, p.y, p.depth, v, &tx, &ty);
-          const int tile_y = vi / a.n1d;
-          const int tile_xi = vi - tile_y * a.n1d;
-          const int gx = (tile_xi * a.ism_side + (int)tx) & amask;  // GL_REPEAT wrap
-          const int gy = (tile_y * a.ism_side + (int)ty) & amask;
-          const float occ = (float)a.ism[(size_t)gy * a.atlas_side + gx] * (1.0f / 65535.0f);
-          const float sv = d2 * inv * inv_zfar - occ;
-          float t = (1.0f - sv - e0) * inv_width;
-          t = fminf(fmaxf(t, 0.0f), 1.0f);
-          g *= t * t * (3.0f - 2.0f * t);
+          const int tile_y = vi >> a.n1d_log2;
+          const int tile_xi = vi & (a.n1d - 1);
+          const int gx = (tile_xi * a.ism_side + tx) & amask;  // GL_REPEAT wrap
+          const int gy = (tile_y * a.ism_side + ty) & amask;
+          const float occ = (float)a.ism[(uint32_t)((gy << alog) | gx)];
+          // smoothstep(1 - bias, 1, 1 - (dist/zFar - occ)): t = 1 + (occ - dist/zFar) / bias
+          float t = __builtin_fmaf(occ, k2, __builtin_fmaf(-dist, k1, 1.0f));
+          t = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
+          g *= t * t * __builtin_fmaf(-2.0f, t, 3.0f);
         }
         ar = __builtin_fmaf(v.cr, g, ar);
         ag = __builtin_fmaf(v.cg, g, ag);
         ab = __builtin_fmaf(v.cb, g, ab);
       }
+      if (!mask) break;
+      vt = vtn;
+      vf = vfn;
+      j = jn;
     }
   }
   if (p.valid) {
@@ -221,7 +340,8 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   }
 }
 
-// Lit-pair counter: pairs (pixel, VPL of its class) with angleFactor > 0.
+// Lit-pair counter: pairs (pixel, VPL of its class) with angleFactor > 0,
+// evaluated directly as max(0, n_v.d) * max(0, -n_p.d) > 0.
 __global__ __launch_bounds__(64) void count_lit_kernel(GatherArgs a, int tiles_x, int num_work,
                                                        unsigned long long* counter) {
   const int w = blockIdx.x;
@@ -235,12 +355,11 @@ __global__ __launch_bounds__(64) void count_lit_kernel(GatherArgs a, int tiles_x
   unsigned int lit = 0;
   const int first = k * a.per_class;
   for (int j = 0; j < a.per_class; ++j) {
-    const VplRec v = *reinterpret_cast<const VplRec*>(a.vpl + (size_t)(first + j) * 16);
+    const VplLds v = load_vpl(a.vpl + (size_t)(first + j) * kVplFloats);
     const float dx = p.fx - v.px, dy = p.fy - v.py, dz = p.fz - v.pz;
-    const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
-    const float cv = (v.nx * dx + v.ny * dy + v.nz * dz) * inv;
-    const float cp = -(p.nx * dx + p.ny * dy + p.nz * dz) * inv;
-    lit += (p.valid && fmaxf(cv, 0.0f) * fmaxf(cp, 0.0f) > 0.0f) ? 1u : 0u;
+    const float ndot = v.nx * dx + v.ny * dy + v.nz * dz;
+    const float pdot = p.nx * dx + p.ny * dy + p.nz * dz;
+    lit += (p.valid && ndot > 0.0f && pdot < 0.0f) ? 1u : 0u;
   }
   unsigned long long s = lit;
 #pragma unroll
@@ -249,7 +368,7 @@ __global__ __launch_bounds__(64) void count_lit_kernel(GatherArgs a, int tiles_x
     __hip_atomic_fetch_add(counter, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int TW, int TH>
+template <int TW, int TH, int kFetch>
 hipError_t launch_tiled(hipStream_t s, const GatherArgs& a) {
   const int cw = (a.width + a.interleave - 1) / a.interleave;
   const int ch = (a.height + a.interleave - 1) / a.interleave;
@@ -258,23 +377,33 @@ hipError_t launch_tiled(hipStream_t s, const GatherArgs& a) {
   const int per_xcd = (num_work + kXcds - 1) / kXcds;
   const dim3 grid(per_xcd * kXcds), block(TW * TH);
   if (a.shadowing)
-    hipLaunchKernelGGL((final_gather_kernel<TW, TH, true>), grid, block, 0, s, a, tiles_x, num_work,
-                       per_xcd);
+    hipLaunchKernelGGL((final_gather_kernel<TW, TH, true, kFetch>), grid, block, 0, s, a, tiles_x,
+                       num_work, per_xcd);
   else
-    hipLaunchKernelGGL((final_gather_kernel<TW, TH, false>), grid, block, 0, s, a, tiles_x,
+    hipLaunchKernelGGL((final_gather_kernel<TW, TH, false, kFetch>), grid, block, 0, s, a, tiles_x,
                        num_work, per_xcd);
   return hipGetLastError();
+}
+
+template <int TW, int TH>
+hipError_t launch_fetch(hipStream_t s, const GatherArgs& a, int fetch) {
+  switch (fetch) {
+    case kFetchLds: return launch_tiled<TW, TH, kFetchLds>(s, a);
+    case kFetchLdsTest: return launch_tiled<TW, TH, kFetchLdsTest>(s, a);
+    case kFetchScalar: return launch_tiled<TW, TH, kFetchScalar>(s, a);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
 
 hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision, int tile_w,
-                               int tile_h) {
+                               int tile_h, int fetch) {
   (void)precision;
-  if (tile_w == 8 && tile_h == 8) return launch_tiled<8, 8>(s, a);
-  if (tile_w == 16 && tile_h == 16) return launch_tiled<16, 16>(s, a);
-  if (tile_w == 32 && tile_h == 8) return launch_tiled<32, 8>(s, a);
-  if (tile_w == 16 && tile_h == 4) return launch_tiled<16, 4>(s, a);
+  if (tile_w == 8 && tile_h == 8) return launch_fetch<8, 8>(s, a, fetch);
+  if (tile_w == 16 && tile_h == 16) return launch_fetch<16, 16>(s, a, fetch);
+  if (tile_w == 32 && tile_h == 8) return launch_fetch<32, 8>(s, a, fetch);
+  if (tile_w == 16 && tile_h == 4) return launch_fetch<16, 4>(s, a, fetch);
   return hipErrorInvalidValue;
 }
 

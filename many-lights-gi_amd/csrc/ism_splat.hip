@@ -47,13 +47,13 @@ __global__ __launch_bounds__(64) void ism_splat_kernel(
   const int lane = threadIdx.x;
   if (lane < kMaxTest) {
     const int idx = (b + lane) % n_vpl;  // index %= totalVplCount
-    const float4* r = reinterpret_cast<const float4*>(vpl + (size_t)idx * 16);
-    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    const float4* r = reinterpret_cast<const float4*>(vpl + (size_t)idx * kVplFloats);
+    const float4 r0 = r[kVplPos / 4], r1 = r[kVplNormal / 4], r2 = r[kVplS / 4], r3 = r[kVplU / 4];
     Candidate c;
     c.px = r0.x; c.py = r0.y; c.pz = r0.z;
     c.nx = r1.x; c.ny = r1.y; c.nz = r1.z;
-    c.sx = r1.w; c.sy = r2.x; c.sz = r2.y;
-    c.ux = r2.z; c.uy = r2.w; c.uz = r3.x;
+    c.sx = r2.x; c.sy = r2.y; c.sz = r2.z;
+    c.ux = r3.x; c.uy = r3.y; c.uz = r3.z;
     c.ism = idx;
     // int(ismIndex / ismIndices1d), exact for idx < 2^24
     c.tile_y = idx / n1d;

@@ -12,6 +12,14 @@ struct Mat4 {
   float m[16];  // column-major
 };
 
+// VPL record: 32 floats (128 bytes) per VPL, see include/mlgi.h.
+constexpr int kVplFloats = 32;
+constexpr int kVplPos = 0;       // xyz, [3] plane offset n . position
+constexpr int kVplNormal = 4;    // xyz, [7] packed normal bits
+constexpr int kVplS = 8;         // lookAtRH s row
+constexpr int kVplU = 12;        // lookAtRH u row
+constexpr int kVplFlux = 16;     // rgb
+
 // vpl_extract.hip
 hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* normal,
                               const float* depth, int w, int h, const Mat4& inv_bvp,
@@ -43,14 +51,17 @@ struct GatherArgs {
   int atlas_side;
   int ism_side;
   int n1d;
+  int n1d_log2;
   float z_far;
+  float shadow_k1;   // 1 / (bias * zFar), bias = 0.02 (final_gathering.comp:124)
+  float shadow_k2;   // 1 / (bias * 65535): R16 texel value to smoothstep units
   float clamp_value;
   float out_scale;   // giIntensityFactor / vplCount * I^2
   int shadowing;
   float* out;
 };
 hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision, int tile_w,
-                               int tile_h);
+                               int tile_h, int fetch);
 hipError_t launch_count_lit_pairs(hipStream_t s, const GatherArgs& a,
                                   unsigned long long* counter);
 

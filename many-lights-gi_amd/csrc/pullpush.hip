@@ -13,8 +13,24 @@
 // the push border test gives zero weight to every texel of another ISM, so
 // ISMs are independent and nothing outside the tile is read. Zero-weight
 // terms contribute +-0 to the push sums, so skipping them is exact.
+//
+// Most of an atlas is empty (0xFFFFFFFF), and an empty texel decodes to a
+// fixed, valid far sample, so whole regions of every pull level hold one
+// constant texel E_l (computed on the host by the same code and passed as an
+// argument): a pull output whose four inputs all equal E_{l-1} is E_l.
+//
+// Push runs over the reference's invocations: the (up to) four fine pixels
+// that share the same four coarse neighbours (push.comp:46-67), so loads,
+// half unpacking and per-neighbour constants are formed once per quad.
+// The radius tests compare a correctly rounded square root with a float,
+// sqrt_rn(x) > R. Because sqrt(x) of a float x can never equal the midpoint m
+// between R and the next float (m has 25 significant bits, m^2 then has more
+// than 48, x has 24), that is exactly x > m^2, evaluated in FP64 where m and
+// m^2 are exact: the same decision as the FP32 twin for a fraction of the cost.
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
+
+#include <cmath>
 
 namespace mlgi {
 namespace {
@@ -25,6 +41,12 @@ struct __align__(16) Texel {
 };
 
 constexpr int kThreads = 256;
+constexpr int kMaxLevels = 7;
+
+struct PullConsts {
+  Texel e[kMaxLevels + 1];  // e[l]: pull level l of an all-empty block (l >= 1)
+  int pull_fast;            // every E_l has zero displacement (value position-free)
+};
 
 template <int S>
 constexpr int level_offset(int l) {  // texels of pull levels 1..l-1
@@ -33,14 +55,307 @@ constexpr int level_offset(int l) {  // texels of pull levels 1..l-1
   return o;
 }
 
+__host__ __device__ inline uint32_t f2u(float f) {
+  union {
+    float f;
+    uint32_t u;
+  } x;
+  x.f = f;
+  return x.u;
+}
+
+// packHalf2x16 (round to nearest even). The device uses the hardware
+// conversion; the host (constants of empty regions) the textbook bit method.
+__host__ __device__ inline uint16_t f32_to_f16_rne(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+#else
+  const uint32_t x = f2u(f);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u));
+  if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);
+  if (ax < 0x38800000u) {
+    if (ax < 0x33000000u) return (uint16_t)sign;
+    const uint32_t e = ax >> 23, m = (ax & 0x7FFFFFu) | 0x800000u, sh = 126u - e;
+    uint32_t q = m >> sh;
+    const uint32_t rem = m & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t q = (((ax >> 23) - 112u) << 10) | ((ax & 0x7FFFFFu) >> 13);
+  const uint32_t rem = ax & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;
+  return (uint16_t)(sign | q);
+#endif
+}
+
+__host__ __device__ inline uint32_t pack_half2_hd(float x, float y) {
+  return (uint32_t)f32_to_f16_rne(x) | ((uint32_t)f32_to_f16_rne(y) << 16);
+}
+
+__host__ __device__ inline float f16_to_f32(uint16_t h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (float)__builtin_bit_cast(_Float16, h);
+#else
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  union {
+    uint32_t u;
+    float f;
+  } r;
+  if (e == 0) {
+    const float v = std::ldexp((float)m, -24);
+    return sign ? -v : v;
+  }
+  r.u = (e == 31) ? (sign | 0x7F800000u | (m << 13)) : (sign | ((e + 112u) << 23) | (m << 13));
+  return r.f;
+#endif
+}
+
+__host__ __device__ inline void unpack_half2_hd(uint32_t u, float& x, float& y) {
+  x = f16_to_f32((uint16_t)(u & 0xFFFFu));
+  y = f16_to_f32((uint16_t)(u >> 16));
+}
+
+// m^2 for the test sqrt_rn(x) > R, R >= 0 finite (see the file header).
+__host__ __device__ inline double sqrt_gt_bound(float R) {
+  union {
+    float f;
+    uint32_t u;
+  } n;
+  n.f = R;
+  n.u += 1u;  // next float above R
+  const double m = ((double)R + (double)n.f) * 0.5;
+  return m * m;
+}
+
+// Level-0 decode of one atlas texel (pull.comp:37-39, :51-66): depth,
+// maxDepth and the radius in ISM pixels. r = float(s & 255) / 10 and
+// madd = r * 2 / zFar * 0.6 are passed in (tabulated per radius byte).
+struct Decoded {
+  float depth, max_depth, radius;
+};
+
+__host__ __device__ inline Decoded decode_texel(uint32_t s, float r, float madd, float z_far,
+                                                int S) {
+  Decoded d;
+  d.depth = (float)(s >> 8) / 16777216.0f;
+  d.max_depth = d.depth + madd;
+  const float dist = d.depth * z_far;
+  float radius = r / dist / 3.14f * (float)S;
+  radius = radius * 1.3f;
+  d.radius = fminf(radius, 15.0f);
+  return d;
+}
+
+// pull.comp:84-125: combine four inputs (depth, maxDepth, radius, displacement
+// to the output centre, validity) into one output texel.
+__host__ __device__ inline Texel pull_combine(const float* ds, const float* mds, const float* rs,
+                                              const float* dx, const float* dy, bool* valid) {
+  float minimum = __builtin_inff();
+  float max_depth = 0.0f;
+  bool any = false;
+  for (int i = 0; i < 4; ++i) {
+    if (!valid[i]) continue;
+    any = true;
+    minimum = fminf(ds[i], minimum);
+    if (minimum == ds[i]) max_depth = mds[i];
+  }
+  Texel res{1.0f, 0.0f, 0.0f, 0u};
+  if (!any) return res;
+  for (int i = 0; i < 4; ++i)
+    if (ds[i] > max_depth) valid[i] = false;
+  float dacc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f, mvm = 0.0f;
+  uint32_t nv = 0;
+  for (int i = 0; i < 4; ++i) {
+    if (!valid[i]) continue;
+    dacc = dacc + ds[i];
+    xacc = xacc + dx[i];
+    yacc = yacc + dy[i];
+    racc = racc + rs[i];
+    mvm = fmaxf(mvm, mds[i]);
+    ++nv;
+  }
+  if (nv > 0) {
+    const float fn = (float)nv;
+    res.depth = dacc / fn;
+    res.max_depth = mvm;
+    res.radius = racc / fn;
+    res.disp = pack_half2_hd(xacc / fn, yacc / fn);
+  }
+  return res;
+}
+
+__device__ __forceinline__ bool same_texel(const Texel& a, const Texel& b) {
+  return f2u(a.depth) == f2u(b.depth) && f2u(a.max_depth) == f2u(b.max_depth) &&
+         f2u(a.radius) == f2u(b.radius) && a.disp == b.disp;
+}
+
+// Generic pull of output (ox, oy) at level l >= 1 (pull.comp:20-126).
+template <int S>
+__device__ __forceinline__ Texel pull_generic(int l, int ox, int oy, int ism_x, int ism_y,
+                                              const uint32_t* base, const Texel* src,
+                                              const float* rtab, const float* mtab,
+                                              const Decoded& empty, float z_far) {
+  const int so = S >> l, si = so * 2;
+  const float scale = __builtin_ldexpf(1.0f, -l);  // pow(2, -level)
+  const int gox = ism_x * so + ox, goy = ism_y * so + oy;  // global output texel
+  float ds[4], mds[4], rs[4], dx[4], dy[4];
+  bool valid[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    // offsets {0,0},{0,1},{1,0},{1,1}
+    const int offx = i >> 1, offy = i & 1;
+    const int ix = ox * 2 + offx, iy = oy * 2 + offy;
+    if (l == 1) {
+      const uint32_t s = base[iy * S + ix];
+      const uint32_t b = s & 0xFFu;
+      const Decoded d = (s == 0xFFFFFFFFu) ? empty : decode_texel(s, rtab[b], mtab[b], z_far, S);
+      ds[i] = d.depth;
+      mds[i] = d.max_depth;
+      rs[i] = d.radius;
+      // zero displacement: (in + 0.5) / 2 - (out + 0.5) is exactly +-0.25,
+      // and sqrt(0.125) rounds to 0.35355339
+      dx[i] = offx ? 0.25f : -0.25f;
+      dy[i] = offy ? 0.25f : -0.25f;
+      valid[i] = ds[i] != 1.0f && 0.35355339f <= rs[i] * scale;
+    } else {
+      const Texel tx = src[iy * si + ix];
+      float disp_x, disp_y;
+      unpack_half2_hd(tx.disp, disp_x, disp_y);
+      const float gix = (float)(gox * 2 + offx), giy = (float)(goy * 2 + offy);
+      dx[i] = (gix + 0.5f + disp_x) / 2.0f - ((float)gox + 0.5f);
+      dy[i] = (giy + 0.5f + disp_y) / 2.0f - ((float)goy + 0.5f);
+      ds[i] = tx.depth;
+      mds[i] = tx.max_depth;
+      rs[i] = tx.radius;
+      // sqrt(dx^2 + dy^2) <= radius * scale, decided exactly without the root
+      const float x = dx[i] * dx[i] + dy[i] * dy[i];
+      valid[i] = ds[i] != 1.0f && !((double)x > sqrt_gt_bound(rs[i] * scale));
+    }
+  }
+  return pull_combine(ds, mds, rs, dx, dy, valid);
+}
+
+__device__ __forceinline__ uint16_t to_r16(float depth) {
+  const float r = fminf(fmaxf(depth, 0.0f), 1.0f);
+  return (uint16_t)rintf(r * 65535.0f);
+}
+
+// One push invocation (push.comp:44-165): quad g = (gx, gy) of level l
+// (fine side so) writes fine pixels 2g - 1 + {0,1}^2 that lie in the ISM.
+// Level 0 reads the originals from the atlas tile and leaves the unorm16
+// result in the same slot; other levels update their texels in place.
+__device__ __forceinline__ void push_quad(int l, int gx, int gy, int so, const Texel* coarse,
+                                          Texel* same, uint32_t* base) {
+  const int sc = so / 2;
+  const float scale = __builtin_ldexpf(1.0f, -l);
+  // offsets {0,0},{0,1},{1,1},{1,0} for both neighbours and fine pixels
+  const int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
+  bool usable[4];
+  float cd[4], cmd[4], cr[4], cdx2[4], cdy2[4];
+  double bound[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
+    // ISM border test: coarse neighbours outside this ISM get weight 0
+    const bool inside = cx >= 0 && cy >= 0 && cx < sc && cy < sc;
+    Texel c{1.0f, 0.0f, 0.0f, 0u};
+    if (inside) c = coarse[cy * sc + cx];
+    cd[i] = c.depth;
+    cmd[i] = c.max_depth;
+    cr[i] = c.radius;
+    float dxh, dyh;
+    unpack_half2_hd(c.disp, dxh, dyh);
+    cdx2[i] = dxh * 2.0f;
+    cdy2[i] = dyh * 2.0f;
+    usable[i] = inside && c.depth != 1.0f;
+    bound[i] = sqrt_gt_bound(c.radius * scale);
+  }
+#pragma unroll
+  for (int op = 0; op < 4; ++op) {
+    const int px = 2 * gx - 1 + ox[op], py = 2 * gy - 1 + oy[op];
+    if (px < 0 || py < 0 || px >= so || py >= so) continue;
+    int w[4];
+    float ddx[4], ddy[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
+      w[i] = usable[i] ? ((wt == 0) ? 9 : (wt == 2 ? 1 : 3)) : 0;
+      // (coarse + 0.5) * 2 - (fine + 0.5) is exact
+      const float ctx = ((float)(gx - 1 + ox[i]) + 0.5f) * 2.0f;
+      const float cty = ((float)(gy - 1 + oy[i]) + 0.5f) * 2.0f;
+      const float ttx = (float)px + 0.5f, tty = (float)py + 0.5f;
+      ddx[i] = ctx - ttx + cdx2[i];
+      ddy[i] = cty - tty + cdy2[i];
+      const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
+      if ((double)x > bound[i]) w[i] = 0;  // sqrt(x) > radius * 2^-level
+    }
+    float minimum = 9001.0f;
+    float max_depth = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (w[i] == 0) continue;
+      minimum = fminf(cd[i], minimum);
+      if (minimum == cd[i]) max_depth = cmd[i];
+    }
+    int wacc = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (w[i] != 0 && cd[i] > max_depth) w[i] = 0;
+      wacc += w[i];
+    }
+    const int t = py * so + px;
+    Texel orig;
+    if (l == 0)
+      orig = Texel{(float)(base[t] >> 8) / 16777216.0f, 0.0f, 0.0f, 0u};
+    else
+      orig = same[t];
+    const bool invalid = orig.depth == 1.0f;
+    bool occluded = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) occluded = occluded || (w[i] > 0 && orig.depth > cmd[i]);
+    Texel res = orig;
+    if (wacc > 0 && (invalid || occluded)) {
+      const float fw = (float)wacc;
+      float dacc = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (w[i] != 0) dacc = dacc + cd[i] * (float)w[i];
+      res.depth = dacc / fw;
+      if (l > 0) {
+        float macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (w[i] == 0) continue;
+          const float fwi = (float)w[i];
+          macc = macc + cmd[i] * fwi;
+          racc = racc + cr[i] * fwi;
+          xacc = xacc + ddx[i] * fwi;
+          yacc = yacc + ddy[i] * fwi;
+        }
+        res.max_depth = macc / fw;
+        res.radius = racc / fw;
+        res.disp = pack_half2_hd(xacc / fw, yacc / fw);
+      }
+    }
+    if (l == 0)
+      base[t] = to_r16(res.depth);  // the tile slot now holds the result
+    else
+      same[t] = res;
+  }
+}
+
 template <int S>
 __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __restrict__ atlas,
                                                             int atlas_side, int levels,
-                                                            float z_far,
+                                                            float z_far, PullConsts pc,
                                                             uint16_t* __restrict__ out) {
   constexpr int kLog = (S == 8) ? 3 : (S == 16) ? 4 : (S == 32) ? 5 : (S == 64) ? 6 : 7;
   __shared__ uint32_t base[S * S];
   __shared__ Texel lv[level_offset<S>(kLog + 1)];
+  __shared__ float rtab[256], mtab[256];
 
   const int tid = threadIdx.x;
   const int n1d = atlas_side / S;
@@ -48,6 +363,12 @@ __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __re
   const int ism_y = blockIdx.x / n1d;
   const size_t a0 = (size_t)ism_y * S * atlas_side + (size_t)ism_x * S;
 
+  // ---- radius-byte tables (same expressions as the per-texel decode)
+  for (int b = tid; b < 256; b += kThreads) {
+    const float r = (float)b / 10.0f;
+    rtab[b] = r;
+    mtab[b] = r * 2.0f / z_far * 0.6f;
+  }
   // ---- load the ISM's atlas tile (S*S u32) with 16-byte loads
   for (int i = tid; i < S * S / 4; i += kThreads) {
     const int row = (i * 4) / S, col = (i * 4) % S;
@@ -55,209 +376,119 @@ __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __re
     *reinterpret_cast<uint4*>(&base[row * S + col]) = v;
   }
   __syncthreads();
+  // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
+  const Decoded empty = decode_texel(0xFFFFFFFFu, rtab[255], mtab[255], z_far, S);
 
-  // ---- pull (pull.comp), level l written from level l-1
+  // ---- pull, level l written from level l-1
   for (int l = 1; l <= levels; ++l) {
-    const int so = S >> l, si = so * 2;
-    const float scale = __builtin_ldexpf(1.0f, -l);  // pow(2, -level)
+    const int so = S >> l, n = so * so;
     Texel* dst = lv + level_offset<S>(l);
     const Texel* src = lv + level_offset<S>(l - 1);
-    for (int t = tid; t < so * so; t += kThreads) {
+    for (int t = tid; t < n; t += kThreads) {
       const int ox = t % so, oy = t / so;
-      const int gox = ism_x * so + ox, goy = ism_y * so + oy;  // global output texel
-      float ds[4], mds[4], rs[4], dx[4], dy[4];
-      bool valid[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // offsets {0,0},{0,1},{1,0},{1,1}
-        const int offx = i >> 1, offy = i & 1;
-        const int ix = ox * 2 + offx, iy = oy * 2 + offy;
-        float depth, max_depth, radius, disp_x = 0.0f, disp_y = 0.0f;
-        if (l == 1) {
-          const uint32_t s = base[iy * S + ix];
-          depth = (float)(s >> 8) / 16777216.0f;
-          radius = (float)(s & 0xFFu) / 10.0f;
-          max_depth = depth + radius * 2.0f / z_far * 0.6f;
-          const float dist = depth * z_far;
-          radius = radius / dist / 3.14f * (float)S;
-          radius = radius * 1.3f;
-          radius = fminf(radius, 15.0f);
-        } else {
-          const Texel tx = src[iy * si + ix];
-          depth = tx.depth;
-          max_depth = tx.max_depth;
-          dev::unpack_half2(tx.disp, disp_x, disp_y);
-          radius = tx.radius;
-        }
-        const float gix = (float)(gox * 2 + offx), giy = (float)(goy * 2 + offy);
-        const float ndx = (gix + 0.5f + disp_x) / 2.0f - ((float)gox + 0.5f);
-        const float ndy = (giy + 0.5f + disp_y) / 2.0f - ((float)goy + 0.5f);
-        const float dist = sqrtf(ndx * ndx + ndy * ndy);
-        ds[i] = depth;
-        mds[i] = max_depth;
-        rs[i] = radius;
-        dx[i] = ndx;
-        dy[i] = ndy;
-        valid[i] = depth != 1.0f && dist <= radius * scale;
+      bool all_empty = false;
+      if (!pc.pull_fast) {
+      } else if (l == 1) {
+        const uint2 a = *reinterpret_cast<const uint2*>(&base[(2 * oy) * S + 2 * ox]);
+        const uint2 b = *reinterpret_cast<const uint2*>(&base[(2 * oy + 1) * S + 2 * ox]);
+        all_empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
+      } else {
+        const int si = so * 2;
+        const Texel ep = pc.e[l - 1];
+        all_empty = same_texel(src[(2 * oy) * si + 2 * ox], ep) &&
+                    same_texel(src[(2 * oy) * si + 2 * ox + 1], ep) &&
+                    same_texel(src[(2 * oy + 1) * si + 2 * ox], ep) &&
+                    same_texel(src[(2 * oy + 1) * si + 2 * ox + 1], ep);
       }
-      float minimum = __builtin_inff();
-      float max_depth = 0.0f;
-      bool any = false;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (!valid[i]) continue;
-        any = true;
-        minimum = fminf(ds[i], minimum);
-        if (minimum == ds[i]) max_depth = mds[i];
-      }
-      Texel res{1.0f, 0.0f, 0.0f, 0u};
-      if (any) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (ds[i] > max_depth) valid[i] = false;
-        float dacc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f, mvm = 0.0f;
-        uint32_t nv = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (!valid[i]) continue;
-          dacc = dacc + ds[i];
-          xacc = xacc + dx[i];
-          yacc = yacc + dy[i];
-          racc = racc + rs[i];
-          mvm = fmaxf(mvm, mds[i]);
-          ++nv;
-        }
-        if (nv > 0) {
-          const float fn = (float)nv;
-          res.depth = dacc / fn;
-          res.max_depth = mvm;
-          res.radius = racc / fn;
-          res.disp = dev::pack_half2(xacc / fn, yacc / fn);
-        }
-      }
-      dst[oy * so + ox] = res;
+      dst[t] = all_empty ? pc.e[l]
+                         : pull_generic<S>(l, ox, oy, ism_x, ism_y, base, src, rtab, mtab, empty, z_far);
     }
     __syncthreads();
   }
 
-  // ---- push (push.comp), level l written from level l+1, in place
-  const size_t o0 = a0;  // same tile origin in the R16 result
+  // ---- push, level l written from level l+1 in place, one quad per lane
   for (int l = levels - 1; l >= 0; --l) {
-    const int so = S >> l, sc = so / 2;
-    const float scale = __builtin_ldexpf(1.0f, -l);
+    const int so = S >> l, nq1 = so / 2 + 1;
     const Texel* coarse = lv + level_offset<S>(l + 1);
     Texel* same = lv + level_offset<S>(l);
-    for (int t = tid; t < so * so; t += kThreads) {
-      const int px = t % so, py = t / so;
-      // invocation g covers fine pixels 2g-1+{0,1}; offsets {0,0},{0,1},{1,1},{1,0}
-      const int gx = (px + 1) >> 1, gy = (py + 1) >> 1;
-      const int opx = px + 1 - 2 * gx, opy = py + 1 - 2 * gy;
-      const int op = opx ? (opy ? 2 : 3) : (opy ? 1 : 0);
-      const int clx = gx - 1, cly = gy - 1;
-      float cd[4], cmd[4], cr[4], ddx[4], ddy[4];
-      int w[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int offx = (i == 2 || i == 3) ? 1 : 0;
-        const int offy = (i == 1 || i == 2) ? 1 : 0;
-        const int cx = clx + offx, cy = cly + offy;
-        const int wtab = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
-        w[i] = (wtab == 0) ? 9 : (wtab == 2 ? 1 : 3);
-        cd[i] = cmd[i] = cr[i] = ddx[i] = ddy[i] = 0.0f;
-        // ISM border test: coarse neighbours outside this ISM get weight 0
-        if (cx < 0 || cy < 0 || cx >= sc || cy >= sc) {
-          w[i] = 0;
-          continue;
-        }
-        const Texel c = coarse[cy * sc + cx];
-        float cdx, cdy;
-        dev::unpack_half2(c.disp, cdx, cdy);
-        cd[i] = c.depth;
-        cmd[i] = c.max_depth;
-        cr[i] = c.radius;
-        if (c.depth == 1.0f) w[i] = 0;
-        const float ctx = ((float)cx + 0.5f) * 2.0f, cty = ((float)cy + 0.5f) * 2.0f;
-        const float ttx = (float)px + 0.5f, tty = (float)py + 0.5f;
-        ddx[i] = ctx - ttx + cdx * 2.0f;
-        ddy[i] = cty - tty + cdy * 2.0f;
-        const float dist = sqrtf(ddx[i] * ddx[i] + ddy[i] * ddy[i]);
-        if (dist > c.radius * scale) w[i] = 0;
-      }
-      float minimum = 9001.0f;
-      float max_depth = 0.0f;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (w[i] == 0) continue;
-        minimum = fminf(cd[i], minimum);
-        if (minimum == cd[i]) max_depth = cmd[i];
-      }
-      int wacc = w[0] + w[1] + w[2] + w[3];
-      if (wacc > 0) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (cd[i] > max_depth) w[i] = 0;
-      }
-      float dacc = 0.0f, macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
-      wacc = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (w[i] == 0) continue;
-        const float fw = (float)w[i];
-        dacc = dacc + cd[i] * fw;
-        macc = macc + cmd[i] * fw;
-        racc = racc + cr[i] * fw;
-        xacc = xacc + ddx[i] * fw;
-        yacc = yacc + ddy[i] * fw;
-        wacc += w[i];
-      }
-      Texel res{0.0f, 0.0f, 0.0f, 0u};
-      if (wacc > 0) {
-        const float fw = (float)wacc;
-        res.depth = dacc / fw;
-        res.max_depth = macc / fw;
-        res.radius = racc / fw;
-        res.disp = dev::pack_half2(xacc / fw, yacc / fw);
-      }
-      Texel orig;
-      if (l == 0) {
-        orig = Texel{(float)(base[py * S + px] >> 8) / 16777216.0f, 0.0f, 0.0f, 0u};
-      } else {
-        orig = same[py * so + px];
-      }
-      const bool invalid = orig.depth == 1.0f;
-      bool occluded = false;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) occluded = occluded || (w[i] > 0 && orig.depth > cmd[i]);
-      if (wacc <= 0 || (!invalid && !occluded)) res = orig;
-      if (l == 0) {
-        const float r = fminf(fmaxf(res.depth, 0.0f), 1.0f);
-        out[o0 + (size_t)py * atlas_side + px] = (uint16_t)rintf(r * 65535.0f);
-      } else {
-        same[py * so + px] = res;
-      }
-    }
+    for (int q = tid; q < nq1 * nq1; q += kThreads) push_quad(l, q % nq1, q / nq1, so, coarse, same, base);
     __syncthreads();
   }
+
+  // ---- write the unorm16 tile (held in the low halves of the tile slots)
+  for (int i = tid; i < S * S / 4; i += kThreads) {
+    const int row = (i * 4) / S, col = (i * 4) % S;
+    const uint4 v = *reinterpret_cast<const uint4*>(&base[row * S + col]);
+    *reinterpret_cast<uint2*>(out + a0 + (size_t)row * atlas_side + col) =
+        make_uint2(v.x | (v.y << 16), v.z | (v.w << 16));
+  }
+}
+
+// Host-side constants: the pull levels of an all-empty ISM, computed with
+// the same functions the kernel uses.
+PullConsts make_consts(int S, int levels, float z_far) {
+  PullConsts pc{};
+  const float r = (float)255 / 10.0f;
+  const float madd = r * 2.0f / z_far * 0.6f;
+  const Decoded e0 = decode_texel(0xFFFFFFFFu, r, madd, z_far, S);
+  float ds[4], mds[4], rs[4], dx[4], dy[4];
+  bool valid[4];
+  for (int l = 1; l <= levels && l <= kMaxLevels; ++l) {
+    const float scale = std::ldexp(1.0f, -l);
+    for (int i = 0; i < 4; ++i) {
+      const int offx = i >> 1, offy = i & 1;
+      float dist;
+      if (l == 1) {
+        ds[i] = e0.depth;
+        mds[i] = e0.max_depth;
+        rs[i] = e0.radius;
+        dx[i] = offx ? 0.25f : -0.25f;
+        dy[i] = offy ? 0.25f : -0.25f;
+        dist = 0.35355339f;
+      } else {
+        const Texel& t = pc.e[l - 1];
+        float disp_x, disp_y;
+        unpack_half2_hd(t.disp, disp_x, disp_y);
+        // output (0, 0): the value is the same at every position while the
+        // displacement is zero, which holds for every E_l
+        dx[i] = ((float)offx + 0.5f + disp_x) / 2.0f - 0.5f;
+        dy[i] = ((float)offy + 0.5f + disp_y) / 2.0f - 0.5f;
+        dist = sqrtf(dx[i] * dx[i] + dy[i] * dy[i]);
+        ds[i] = t.depth;
+        mds[i] = t.max_depth;
+        rs[i] = t.radius;
+      }
+      valid[i] = ds[i] != 1.0f && dist <= rs[i] * scale;
+    }
+    pc.e[l] = pull_combine(ds, mds, rs, dx, dy, valid);
+  }
+  // the all-empty shortcut is position-free only while the displacement is 0
+  pc.pull_fast = 1;
+  for (int l = 1; l <= levels && l <= kMaxLevels; ++l)
+    if (pc.e[l].disp != 0u) pc.pull_fast = 0;
+  return pc;
 }
 
 }  // namespace
 
 hipError_t launch_pullpush(hipStream_t s, const uint32_t* atlas, int atlas_side, int ism_side,
                            int levels, float z_far, uint16_t* out) {
+  if (levels < 1 || levels > kMaxLevels) return hipErrorInvalidValue;
   const int n1d = atlas_side / ism_side;
   const dim3 grid(n1d * n1d), block(kThreads);
+  const PullConsts pc = make_consts(ism_side, levels, z_far);
   switch (ism_side) {
     case 8:
-      hipLaunchKernelGGL(pullpush_kernel<8>, grid, block, 0, s, atlas, atlas_side, levels, z_far, out);
+      hipLaunchKernelGGL(pullpush_kernel<8>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
       break;
     case 16:
-      hipLaunchKernelGGL(pullpush_kernel<16>, grid, block, 0, s, atlas, atlas_side, levels, z_far, out);
+      hipLaunchKernelGGL(pullpush_kernel<16>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
       break;
     case 32:
-      hipLaunchKernelGGL(pullpush_kernel<32>, grid, block, 0, s, atlas, atlas_side, levels, z_far, out);
+      hipLaunchKernelGGL(pullpush_kernel<32>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
       break;
     case 64:
-      hipLaunchKernelGGL(pullpush_kernel<64>, grid, block, 0, s, atlas, atlas_side, levels, z_far, out);
+      hipLaunchKernelGGL(pullpush_kernel<64>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
       break;
     default:
       return hipErrorInvalidValue;

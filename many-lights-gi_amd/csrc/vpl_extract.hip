@@ -55,11 +55,18 @@ __global__ __launch_bounds__(64) void vpl_extract_kernel(
   const f3 un = dev::unpack3sn(pk);
   f3 s, u;
   dev::look_at_rh(un, s, u);
-  float4* out = reinterpret_cast<float4*>(records + (size_t)dst * 16);
-  out[0] = make_float4(r[0] / r[3], r[1] / r[3], r[2] / r[3], __uint_as_float(pk));
-  out[1] = make_float4(un.x, un.y, un.z, s.x);
-  out[2] = make_float4(s.y, s.z, u.x, u.y);
-  out[3] = make_float4(u.z, dr * light_intensity, dg * light_intensity, db * light_intensity);
+  const float px = r[0] / r[3], py = r[1] / r[3], pz = r[2] / r[3];
+  const float plane = un.x * px + un.y * py + un.z * pz;  // n . position
+  float4* out = reinterpret_cast<float4*>(records + (size_t)dst * kVplFloats);
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  out[0] = make_float4(px, py, pz, plane);
+  out[1] = make_float4(un.x, un.y, un.z, __uint_as_float(pk));
+  out[2] = make_float4(s.x, s.y, s.z, 0.f);
+  out[3] = make_float4(u.x, u.y, u.z, 0.f);
+  out[4] = make_float4(dr * light_intensity, dg * light_intensity, db * light_intensity, 0.f);
+  out[5] = zero;
+  out[6] = zero;
+  out[7] = zero;
 }
 
 }  // namespace

@@ -28,9 +28,23 @@ __all__ = [
     "MlgiError", "Params", "PRECISION_FP32", "PRECISION_MIXED", "VPL_RECORD_FLOATS",
     "Context", "DeviceArray", "Scene", "Camera", "VPLProcessor", "ImperfectShadowmap",
     "GIStage", "default_params", "shuffle_indices", "ism_indices_1d", "device_count",
+    "vpl_fields",
 ]
 
 lib = N.lib
+
+
+def vpl_fields(rec: np.ndarray) -> dict:
+    """Named views of a host copy of VPL records (count x VPL_RECORD_FLOATS)."""
+    return {
+        "position": rec[:, 0:3],
+        "plane": rec[:, 3],
+        "normal": rec[:, 4:7],
+        "packed_normal": rec[:, 7].view(np.uint32),
+        "s": rec[:, 8:11],
+        "u": rec[:, 12:15],
+        "flux": rec[:, 16:19],
+    }
 
 
 def default_params(**kw) -> Params:

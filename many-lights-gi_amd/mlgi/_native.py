@@ -22,7 +22,7 @@ MLGI_ERR_OUT_OF_BUDGET = 6
 PRECISION_FP32 = 0
 PRECISION_MIXED = 1
 
-VPL_RECORD_FLOATS = 16
+VPL_RECORD_FLOATS = 32
 
 
 class Params(C.Structure):
@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("precision", C.c_int32),
         ("tile_w", C.c_int32),
         ("tile_h", C.c_int32),
+        ("gather_fetch", C.c_int32),
     ]
 
 

@@ -49,7 +49,7 @@ class Frame:
     @property
     def vpl(self):
         r = self.rec
-        return r[:, 0:3], r[:, 4:7], r[:, 13:16]
+        return r[:, 0:3], r[:, 4:7], r[:, 16:19]
 
 
 @pytest.fixture(scope="module", params=["tiny", "small", "1080p"])
@@ -64,12 +64,16 @@ def test_vpl_extract_bit_exact(frame):
                                            frame.params.light_intensity, sh, wl.n_vpl, np.float32)
     r = frame.rec
     assert np.array_equal(r[:, 0:3], pos)
-    assert np.array_equal(r[:, 3].view(np.uint32), pk)
+    assert np.array_equal(r[:, 7].view(np.uint32), pk)
     assert np.array_equal(r[:, 4:7], nrm)
-    assert np.array_equal(r[:, 13:16], col)
+    assert np.array_equal(r[:, 16:19], col)
+    f = mlgi.vpl_fields(r)
+    n, x = f["normal"], f["position"]
+    plane = n[:, 0] * x[:, 0] + n[:, 1] * x[:, 1] + n[:, 2] * x[:, 2]  # float32, unfused
+    assert np.array_equal(f["plane"], plane)
     for i in range(0, wl.n_vpl, max(1, wl.n_vpl // 64)):
         s, u = oracle.look_at_rh(r[i, 4:7], np.float32)
-        assert np.array_equal(r[i, 7:10], s) and np.array_equal(r[i, 10:13], u), i
+        assert np.array_equal(r[i, 8:11], s) and np.array_equal(r[i, 12:15], u), i
     pos64, _, pk64, _ = oracle.vpl_extract(inp.rsm_diffuse, inp.rsm_normal, inp.rsm_depth, inp.inv_bvp,
                                            frame.params.light_intensity, sh, wl.n_vpl, np.float64)
     assert np.abs(r[:, 0:3] - pos64).max() <= 1e-5 * 10.0
