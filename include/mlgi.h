@@ -77,6 +77,7 @@ typedef struct mlgi_params {
   int32_t precision;              /* mlgi_precision */
   int32_t tile_w, tile_h;         /* final-gather tile in interleave-class pixels; 0 = auto */
   int32_t gather_fetch;           /* final-gather VPL fetch variant 0..2; -1 = auto */
+  int32_t splat_block;            /* splat workgroup size 64/128/256; 0 = auto */
 } mlgi_params;
 
 /* RSM inputs (RasterizationStage outputs bound in VPLProcessor.cpp:84-86).
@@ -195,6 +196,19 @@ int mlgi_final_gather(mlgi_context* ctx, const mlgi_gbuffer* gb, const mlgi_vpls
 int mlgi_frame(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_points* pts,
                const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls,
                mlgi_atlas* atlas, mlgi_ism_depth* ism, mlgi_image* out);
+
+/* ------------------------------------------------------------ HIP graphs */
+typedef struct mlgi_graph mlgi_graph;
+/* Capture one mlgi_frame with these descriptors into an instantiated HIP
+ * graph (kernel nodes only). Replays read the buffers' current contents, so
+ * new RSM / G-buffer data can be uploaded between replays. */
+int mlgi_graph_capture_frame(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_points* pts,
+                             const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls,
+                             mlgi_atlas* atlas, mlgi_ism_depth* ism, mlgi_image* out,
+                             mlgi_graph** graph);
+/* Enqueue one replay on the context stream (asynchronous). */
+int mlgi_graph_launch(mlgi_context* ctx, mlgi_graph* graph);
+void mlgi_graph_destroy(mlgi_graph* graph);
 
 /* Number of (pixel, VPL-of-its-class) pairs with a positive geometry factor
  * (the lit pairs the gather's algorithmic work is counted on). Synchronous. */

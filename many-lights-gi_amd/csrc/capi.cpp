@@ -241,6 +241,7 @@ void mlgi_params_default(mlgi_params* p) {
   p->tile_w = 0;
   p->tile_h = 0;
   p->gather_fetch = -1;
+  p->splat_block = 0;
 }
 
 int mlgi_malloc(mlgi_context* c, size_t bytes, void** dptr) {
@@ -323,6 +324,8 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
   if (pts->count > 0 && pts->n_bins != vpls->count)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "point bins (%d) must equal the VPL count (%d)",
                 pts->n_bins, vpls->count);
+  if (p->splat_block != 0 && p->splat_block != 64 && p->splat_block != 128 && p->splat_block != 256)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "splat_block %d not in {0, 64, 128, 256}", p->splat_block);
   int n1d, s;
   if ((rc = atlas_geometry(c, atlas->side, vpls->count, &n1d, &s))) return rc;
   if (atlas->ism_side == 0) atlas->ism_side = s;
@@ -340,7 +343,8 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
   return hip_check(c,
                    mlgi::launch_ism_splat(c->stream, pts->records, pts->bin_offsets, pts->n_bins,
                                           pts->count, vpls->records, vpls->count, n1d, atlas->side,
-                                          p->z_far, atlas->texels),
+                                          p->z_far, atlas->texels,
+                                          p->splat_block > 0 ? p->splat_block : 256),
                    "ism_splat");
 }
 
@@ -476,6 +480,74 @@ int mlgi_frame(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_points* pts,
   if ((rc = mlgi_final_gather(c, gb, vpls, ism, p, out))) return rc;
   if (c->profile) c->frames++;
   return MLGI_OK;
+}
+
+}  // extern "C"
+
+struct mlgi_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int mlgi_graph_capture_frame(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_points* pts,
+                             const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls,
+                             mlgi_atlas* atlas, mlgi_ism_depth* ism, mlgi_image* out,
+                             mlgi_graph** graph) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!graph) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "graph is NULL");
+  *graph = nullptr;
+  int rc = check_vpls(c, vpls);
+  if (rc) return rc;
+  if (!p) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "params is NULL");
+  // everything that allocates or copies happens before the capture
+  const int32_t* sh = nullptr;
+  if (p->shuffle_lights && (rc = get_shuffle(c, vpls->count, &sh))) return rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"))) return rc;
+  const bool prof = c->profile;
+  c->profile = false;  // no event nodes in the graph
+  rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                 "hipStreamBeginCapture");
+  if (rc) {
+    c->profile = prof;
+    return rc;
+  }
+  const int frame_rc = mlgi_frame(c, rsm, pts, gb, p, vpls, atlas, ism, out);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(c->stream, &g);
+  c->profile = prof;
+  if (frame_rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return frame_rc;
+  }
+  if ((rc = hip_check(c, end, "hipStreamEndCapture"))) return rc;
+  auto* out_graph = new mlgi_graph();
+  out_graph->graph = g;
+  if ((rc = hip_check(c, hipGraphInstantiate(&out_graph->exec, g, nullptr, nullptr, 0),
+                      "hipGraphInstantiate"))) {
+    (void)hipGraphDestroy(g);
+    delete out_graph;
+    return rc;
+  }
+  *graph = out_graph;
+  return MLGI_OK;
+}
+
+int mlgi_graph_launch(mlgi_context* c, mlgi_graph* graph) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!graph || !graph->exec) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "graph is NULL");
+  Scope sc(c, kFrame);
+  int rc = hip_check(c, hipGraphLaunch(graph->exec, c->stream), "hipGraphLaunch");
+  if (!rc && c->profile) c->frames++;
+  return rc;
+}
+
+void mlgi_graph_destroy(mlgi_graph* graph) {
+  if (!graph) return;
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
 }
 
 int mlgi_count_lit_pairs(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,

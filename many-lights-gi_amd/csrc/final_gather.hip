@@ -160,20 +160,24 @@ __device__ __forceinline__ unsigned long long cull_one(const VplLds& v, bool pre
 // basis and the projection from the FP32 inputs in double.
 constexpr float kTie = 2.5e-4f;
 
-__device__ __forceinline__ void refine_texel(const GatherArgs& a, int x, int y, float depth,
-                                             const VplLds& v, int* tx, int* ty) {
-  const float* m = a.vp_inv.m;
-  const double c0 = (double)x / (double)a.width * 2.0 - 1.0;
-  const double c1 = (double)y / (double)a.height * 2.0 - 1.0;
+// Out of line: the FP64 temporaries would otherwise raise the register
+// count of the whole kernel for a path taken by well under 1% of pairs.
+// Returns the texel as (ty << 16) | tx.
+__device__ __noinline__ int refine_texel(Mat4 m4, int width, int height, int ism_side, int x, int y,
+                                         float depth, float vpx, float vpy, float vpz, float vnx,
+                                         float vny, float vnz) {
+  const float* m = m4.m;
+  const double c0 = (double)x / (double)width * 2.0 - 1.0;
+  const double c1 = (double)y / (double)height * 2.0 - 1.0;
   const double c2 = (double)depth * 2.0 - 1.0;
   double r[4];
   for (int row = 0; row < 4; ++row)
     r[row] = (double)m[row] * c0 + (double)m[4 + row] * c1 + ((double)m[8 + row] * c2 + (double)m[12 + row]);
-  const double dx = r[0] / r[3] - (double)v.px;
-  const double dy = r[1] / r[3] - (double)v.py;
-  const double dz = r[2] / r[3] - (double)v.pz;
+  const double dx = r[0] / r[3] - (double)vpx;
+  const double dy = r[1] / r[3] - (double)vpy;
+  const double dz = r[2] / r[3] - (double)vpz;
   const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-  const double nx = v.nx, ny = v.ny, nz = v.nz;
+  const double nx = vnx, ny = vny, nz = vnz;
   const double upz = (double)0.01f;  // lookAtRH up = (0, 1, 0.01)
   const double qx = ny * upz - nz, qy = -nx * upz, qz = nx;
   const double ql = sqrt(qx * qx + qy * qy + qz * qz);
@@ -182,9 +186,10 @@ __device__ __forceinline__ void refine_texel(const GatherArgs& a, int x, int y, 
   const double den = 1.0 + (nx * dx + ny * dy + nz * dz) / dist;
   const double lx = (sx * dx + sy * dy + sz * dz) / dist / den;
   const double ly = (ux * dx + uy * dy + uz * dz) / dist / den;
-  const double S = (double)a.ism_side;
-  *tx = (int)floor((lx + 1.0) * 0.5 * S);
-  *ty = (int)floor((ly + 1.0) * 0.5 * S);
+  const double S = (double)ism_side;
+  const int tx = (int)floor((lx + 1.0) * 0.5 * S);
+  const int ty = (int)floor((ly + 1.0) * 0.5 * S);
+  return ((ty & 0xFFFF) << 16) | (tx & 0xFFFF);
 }
 
 // Record fetch variants (picked per shape by the autotuner):
@@ -310,8 +315,12 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
           const float cy = __builtin_fmaf(ud, rh, hs);
           const float fx = __builtin_amdgcn_fractf(cx), fy = __builtin_amdgcn_fractf(cy);
           int tx = (int)(cx - fx), ty = (int)(cy - fy);
-          if (fmaxf(fabsf(fx - 0.5f), fabsf(fy - 0.5f)) > 0.5f - kTie)
-            refine_texel(a, p.x, p.y, p.depth, v, &tx, &ty);
+          if (fmaxf(fabsf(fx - 0.5f), fabsf(fy - 0.5f)) > 0.5f - kTie) {
+            const int t2 = refine_texel(a.vp_inv, a.width, a.height, a.ism_side, p.x, p.y, p.depth,
+                                        vt.px, vt.py, vt.pz, vt.nx, vt.ny, vt.nz);
+            tx = (int)(int16_t)(t2 & 0xFFFF);
+            ty = t2 >> 16;
+          }
           const int tile_y = vi >> a.n1d_log2;
           const int tile_xi = vi & (a.n1d - 1);
           const int gx = (tile_xi * a.ism_side + tx) & amask;  // GL_REPEAT wrap

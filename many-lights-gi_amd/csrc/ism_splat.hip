@@ -3,8 +3,10 @@
 // ImperfectShadowmap.cpp:212. Compiled with -ffp-contract=off: the atlas is
 // bit-exact against the FP32 oracle twin (integer atomicMin is order-free).
 //
-// One wave64 workgroup per point bin b. The 16 candidate VPLs (b+i) mod N are
-// staged in LDS once per workgroup (1 KiB); each lane owns one point per
+// One workgroup per point bin b (256 lanes by default, so a bin of a few
+// hundred points is one pass and every SIMD holds enough waves to cover the
+// latency of the point loads and atomics). The 16 candidate VPLs (b+i) mod N
+// are staged in LDS once per workgroup (1 KiB); each lane owns one point per
 // iteration, tests all 16 candidates from broadcast LDS reads (uniform
 // addresses, no bank conflicts), keeps the first four that pass both
 // hemisphere tests as a bit mask, and projects only those. The projected
@@ -38,7 +40,8 @@ struct Candidate {
   int pad;
 };
 
-__global__ __launch_bounds__(64) void ism_splat_kernel(
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void ism_splat_kernel(
     const float4* __restrict__ points, const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
     const float* __restrict__ vpl, int n_vpl, int n1d, int atlas_side, float z_far,
     uint32_t* __restrict__ atlas) {
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(64) void ism_splat_kernel(
   const uint32_t begin = min(bin_offsets[b], end);
   const float inv_n1d = 1.0f / (float)n1d;  // exact: n1d is a power of two
   const float side = (float)atlas_side;
-  for (uint32_t p = begin + lane; p < end; p += 64) {
+  for (uint32_t p = begin + lane; p < end; p += kThreads) {
     const float4 rec = points[p];
     const f3 pos = dev::make(rec.x, rec.y, rec.z);
     const uint32_t bits = __float_as_uint(rec.w);
@@ -146,12 +149,18 @@ hipError_t launch_atlas_clear(hipStream_t s, uint32_t* atlas, size_t texels) {
 
 hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* bin_offsets,
                             int n_bins, int n_points, const float* vpl_records, int n_vpl, int n1d,
-                            int atlas_side, float z_far, uint32_t* atlas) {
+                            int atlas_side, float z_far, uint32_t* atlas, int block) {
   if (n_bins <= 0) return hipSuccess;
-  hipLaunchKernelGGL(ism_splat_kernel, dim3(n_bins), dim3(64), 0, s,
-                     reinterpret_cast<const float4*>(points), bin_offsets, (uint32_t)n_points,
-                     vpl_records, n_vpl,
-                     n1d, atlas_side, z_far, atlas);
+  const float4* pts = reinterpret_cast<const float4*>(points);
+  if (block == 64)
+    hipLaunchKernelGGL(ism_splat_kernel<64>, dim3(n_bins), dim3(64), 0, s, pts, bin_offsets,
+                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas);
+  else if (block == 128)
+    hipLaunchKernelGGL(ism_splat_kernel<128>, dim3(n_bins), dim3(128), 0, s, pts, bin_offsets,
+                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas);
+  else
+    hipLaunchKernelGGL(ism_splat_kernel<256>, dim3(n_bins), dim3(256), 0, s, pts, bin_offsets,
+                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas);
   return hipGetLastError();
 }
 

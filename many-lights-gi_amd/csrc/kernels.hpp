@@ -30,7 +30,7 @@ hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* 
 hipError_t launch_atlas_clear(hipStream_t s, uint32_t* atlas, size_t texels);
 hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* bin_offsets,
                             int n_bins, int n_points, const float* vpl_records, int n_vpl, int n1d,
-                            int atlas_side, float z_far, uint32_t* atlas);
+                            int atlas_side, float z_far, uint32_t* atlas, int block);
 
 // pullpush.hip
 hipError_t launch_pullpush(hipStream_t s, const uint32_t* atlas, int atlas_side, int ism_side,

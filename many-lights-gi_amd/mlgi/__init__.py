@@ -399,8 +399,35 @@ class GIStage:
                                       C.byref(self.params), C.byref(v), C.byref(a), C.byref(d),
                                       C.byref(im)), "mlgi_frame")
 
+    def capture(self, rsm: N.Rsm, points: N.Points, gbuffer: N.GBuffer) -> "FrameGraph":
+        """Capture one frame (process()) into a replayable HIP graph."""
+        v, a, d, im = self.vpl_processor.desc(), self.ism.atlas_desc(), self.ism.depth_desc(), self.image_desc()
+        h = C.c_void_p()
+        self.ctx.check(lib.mlgi_graph_capture_frame(self.ctx.h, C.byref(rsm), C.byref(points), C.byref(gbuffer),
+                                                    C.byref(self.params), C.byref(v), C.byref(a), C.byref(d),
+                                                    C.byref(im), C.byref(h)), "mlgi_graph_capture_frame")
+        return FrameGraph(self.ctx, h)
+
     def count_lit_pairs(self, gbuffer: N.GBuffer) -> int:
         out = C.c_uint64(0)
         self.ctx.check(lib.mlgi_count_lit_pairs(self.ctx.h, C.byref(gbuffer), C.byref(self.vpl_processor.desc()),
                                                 C.byref(self.params), C.byref(out)), "mlgi_count_lit_pairs")
         return int(out.value)
+
+
+class FrameGraph:
+    """An instantiated HIP graph of one GI frame."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def launch(self) -> None:
+        self.ctx.check(lib.mlgi_graph_launch(self.ctx.h, self.h), "mlgi_graph_launch")
+
+    def close(self) -> None:
+        if self.h:
+            lib.mlgi_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

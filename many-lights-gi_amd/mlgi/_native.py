@@ -47,6 +47,7 @@ class Params(C.Structure):
         ("tile_w", C.c_int32),
         ("tile_h", C.c_int32),
         ("gather_fetch", C.c_int32),
+        ("splat_block", C.c_int32),
     ]
 
 
@@ -146,6 +147,10 @@ SIGNATURES = {
     "mlgi_frame": (C.c_int, [vp, P(Rsm), P(Points), P(GBuffer), P(Params), P(Vpls), P(Atlas),
                              P(IsmDepth), P(Image)]),
     "mlgi_count_lit_pairs": (C.c_int, [vp, P(GBuffer), P(Vpls), P(Params), P(C.c_uint64)]),
+    "mlgi_graph_capture_frame": (C.c_int, [vp, P(Rsm), P(Points), P(GBuffer), P(Params), P(Vpls),
+                                           P(Atlas), P(IsmDepth), P(Image), P(vp)]),
+    "mlgi_graph_launch": (C.c_int, [vp, vp]),
+    "mlgi_graph_destroy": (None, [vp]),
     "mlgi_profile_enable": (C.c_int, [vp, i32]),
     "mlgi_profile_reset": (C.c_int, [vp]),
     "mlgi_profile_read": (C.c_int, [vp, P(StageTimes)]),
