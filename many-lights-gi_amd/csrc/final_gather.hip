@@ -53,16 +53,46 @@ __device__ __forceinline__ VplLds load_vpl(const float* rec) {
   return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
+// Wave64 min / max, result wave-uniform. Floats are mapped to unsigned keys
+// of the same order, so each step is one unsigned min/max with a DPP operand
+// (no NaN canonicalisation). DPP steps: swap within pairs and quads, mirror
+// within half-rows and rows (every lane then holds its row's value), then
+// broadcast lane 15 into rows 1 and 3 and lane 31 into rows 2 and 3; lane 63
+// ends with the whole wave's value. Lanes of rows a broadcast skips read the
+// op's identity. Every lane must be active.
+__device__ __forceinline__ uint32_t order_key(float f) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, f);
+  return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
 }
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
+__device__ __forceinline__ float from_order_key(uint32_t k) {
+  return __builtin_bit_cast(float, k ^ ((uint32_t)((int32_t)~k >> 31) | 0x80000000u));
 }
+template <int kCtrl, int kRowMask, bool kMax>
+__device__ __forceinline__ uint32_t dpp_step(uint32_t v) {
+  const uint32_t identity = kMax ? 0u : 0xFFFFFFFFu;
+  const uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, kCtrl, kRowMask, 0xF, false);
+  return kMax ? (v > o ? v : o) : (v < o ? v : o);
+}
+constexpr int kDppQuadSwap1 = 0xB1;     // quad_perm [1,0,3,2]
+constexpr int kDppQuadSwap2 = 0x4E;     // quad_perm [2,3,0,1]
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template <bool kMax>
+__device__ __forceinline__ float wave_reduce(float f) {
+  uint32_t v = order_key(f);
+  v = dpp_step<kDppQuadSwap1, 0xF, kMax>(v);
+  v = dpp_step<kDppQuadSwap2, 0xF, kMax>(v);
+  v = dpp_step<kDppRowHalfMirror, 0xF, kMax>(v);
+  v = dpp_step<kDppRowMirror, 0xF, kMax>(v);
+  v = dpp_step<kDppRowBcast15, 0xA, kMax>(v);
+  v = dpp_step<kDppRowBcast31, 0xC, kMax>(v);
+  return from_order_key((uint32_t)__builtin_amdgcn_readlane((int)v, 63));
+}
+__device__ __forceinline__ float wave_min(float v) { return wave_reduce<false>(v); }
+__device__ __forceinline__ float wave_max(float v) { return wave_reduce<true>(v); }
 
 struct Pixel {
   int x, y;
@@ -193,9 +223,9 @@ __device__ __noinline__ int refine_texel(Mat4 m4, int width, int height, int ism
 }
 
 // Record fetch variants (picked per shape by the autotuner):
-//   kFetchLds     chunk staged in LDS, full next record prefetched
-//   kFetchLdsTest chunk staged in LDS, only the hemisphere-test fields
-//                 prefetched, the rest read when a lane is lit
+//   kFetchLds     chunk staged in LDS, full record read per iteration
+//   kFetchLdsTest chunk staged in LDS, the hemisphere-test fields read per
+//                 iteration, the rest only when a lane is lit
 //   kFetchScalar  records read from global with wave-uniform (scalar) loads,
 //                 next record prefetched into SGPRs
 enum FetchVariant { kFetchLds = 0, kFetchLdsTest = 1, kFetchScalar = 2 };
@@ -272,7 +302,6 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
       mask = cull_one(mine, lane < count, wb);
     }
     if (!mask) continue;
-    int j = __builtin_ctzll(mask);
     auto fetch_test = [&](int jj) -> VplTest {
       if (kFetch == kFetchScalar) {
         const float4* r = reinterpret_cast<const float4*>(crec + (size_t)jj * kVplFloats);
@@ -282,18 +311,27 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
       const VplLds& t = vtile[jj];
       return VplTest{t.px, t.py, t.pz, t.plane, t.nx, t.ny, t.nz};
     };
-    auto fetch_full = [&](int jj) -> VplLds {
-      if (kFetch == kFetchScalar) return load_vpl(crec + (size_t)jj * kVplFloats);
-      return vtile[jj];
-    };
+    // LDS variants read the record at the top of each iteration (other waves
+    // hide the latency; a one-ahead copy would cost a register move per
+    // field and iteration); the scalar variant keeps the next record in SGPRs.
+    int j = __builtin_ctzll(mask);
     VplTest vt = fetch_test(j);
-    VplLds vf = (kFetch == kFetchLdsTest) ? VplLds{} : fetch_full(j);
+    VplLds vf = (kFetch == kFetchScalar) ? load_vpl(crec + (size_t)j * kVplFloats) : VplLds{};
     while (true) {
       mask &= mask - 1ull;
       const int jn = mask ? __builtin_ctzll(mask) : j;
-      // next record, fetched while this one is processed
-      const VplTest vtn = fetch_test(jn);
-      const VplLds vfn = (kFetch == kFetchLdsTest) ? VplLds{} : fetch_full(jn);
+      VplTest vtn{};
+      VplLds vfn{};
+      if (kFetch == kFetchScalar) {
+        vtn = fetch_test(jn);
+        vfn = load_vpl(crec + (size_t)jn * kVplFloats);
+      }
+      if (kFetch == kFetchLds) {
+        vf = vtile[j];
+        vt = VplTest{vf.px, vf.py, vf.pz, vf.plane, vf.nx, vf.ny, vf.nz};
+      } else if (kFetch == kFetchLdsTest) {
+        vt = fetch_test(j);
+      }
       const int vi = first + chunk + j;
       // n_vpl . (p - p_vpl) and n_p . (p - p_vpl) from the plane offsets
       const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
@@ -336,8 +374,10 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
         ab = __builtin_fmaf(v.cb, g, ab);
       }
       if (!mask) break;
-      vt = vtn;
-      vf = vfn;
+      if (kFetch == kFetchScalar) {
+        vt = vtn;
+        vf = vfn;
+      }
       j = jn;
     }
   }

@@ -68,7 +68,12 @@ __host__ __device__ inline uint32_t f2u(float f) {
 // conversion; the host (constants of empty regions) the textbook bit method.
 __host__ __device__ inline uint16_t f32_to_f16_rne(float f) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const _Float16 h = (_Float16)f;
+  // The empty asm pins f as a rounded float: without it the backend folds a
+  // producing fma and this conversion into one fma_mix with a single f16
+  // rounding, which is not packHalf2x16 of the float result.
+  float v = f;
+  asm volatile("" : "+v"(v));
+  const _Float16 h = (_Float16)v;
   return __builtin_bit_cast(uint16_t, h);
 #else
   const uint32_t x = f2u(f);
@@ -130,6 +135,25 @@ __host__ __device__ inline double sqrt_gt_bound(float R) {
   return m * m;
 }
 
+// x / b correctly rounded from y = RN(1 / b): q = RN(x y) lies within an ulp
+// of x / b, the remainder r = x - q b is exact under an FMA, and RN(q + r y)
+// is then RN(x / b) (Markstein's theorem). Holds for every normal x and the
+// divisors used here (the integers 1..16 and 3.14), as an exhaustive check
+// over all float significands confirms; three operations instead of the
+// scaled IEEE division sequence.
+__host__ __device__ inline float div_by_recip(float x, float b, float y) {
+  const float q = x * y;
+  const float r = __builtin_fmaf(-q, b, x);
+  return __builtin_fmaf(r, y, q);
+}
+
+// RN(1 / w) for w = 0..16 (entry 0 unused); compile-time IEEE divisions.
+constexpr float kRcp[17] = {0.0f,        1.0f,        1.0f / 2.0f,  1.0f / 3.0f,  1.0f / 4.0f,
+                            1.0f / 5.0f, 1.0f / 6.0f, 1.0f / 7.0f,  1.0f / 8.0f,  1.0f / 9.0f,
+                            1.0f / 10.0f, 1.0f / 11.0f, 1.0f / 12.0f, 1.0f / 13.0f, 1.0f / 14.0f,
+                            1.0f / 15.0f, 1.0f / 16.0f};
+constexpr float kRcp314 = 1.0f / 3.14f;
+
 // Level-0 decode of one atlas texel (pull.comp:37-39, :51-66): depth,
 // maxDepth and the radius in ISM pixels. r = float(s & 255) / 10 and
 // madd = r * 2 / zFar * 0.6 are passed in (tabulated per radius byte).
@@ -143,7 +167,7 @@ __host__ __device__ inline Decoded decode_texel(uint32_t s, float r, float madd,
   d.depth = (float)(s >> 8) / 16777216.0f;
   d.max_depth = d.depth + madd;
   const float dist = d.depth * z_far;
-  float radius = r / dist / 3.14f * (float)S;
+  float radius = div_by_recip(r / dist, 3.14f, kRcp314) * (float)S;
   radius = radius * 1.3f;
   d.radius = fminf(radius, 15.0f);
   return d;
@@ -179,22 +203,23 @@ __host__ __device__ inline Texel pull_combine(const float* ds, const float* mds,
   }
   if (nv > 0) {
     const float fn = (float)nv;
-    res.depth = dacc / fn;
+    const float y = nv == 3 ? kRcp[3] : (nv == 4 ? 0.25f : (nv == 2 ? 0.5f : 1.0f));
+    res.depth = div_by_recip(dacc, fn, y);
     res.max_depth = mvm;
-    res.radius = racc / fn;
-    res.disp = pack_half2_hd(xacc / fn, yacc / fn);
+    res.radius = div_by_recip(racc, fn, y);
+    res.disp = pack_half2_hd(div_by_recip(xacc, fn, y), div_by_recip(yacc, fn, y));
   }
   return res;
 }
 
-__device__ __forceinline__ bool same_texel(const Texel& a, const Texel& b) {
+__host__ __device__ __forceinline__ bool same_texel(const Texel& a, const Texel& b) {
   return f2u(a.depth) == f2u(b.depth) && f2u(a.max_depth) == f2u(b.max_depth) &&
          f2u(a.radius) == f2u(b.radius) && a.disp == b.disp;
 }
 
 // Generic pull of output (ox, oy) at level l >= 1 (pull.comp:20-126).
 template <int S>
-__device__ __forceinline__ Texel pull_generic(int l, int ox, int oy, int ism_x, int ism_y,
+__host__ __device__ __forceinline__ Texel pull_generic(int l, int ox, int oy, int ism_x, int ism_y,
                                               const uint32_t* base, const Texel* src,
                                               const float* rtab, const float* mtab,
                                               const Decoded& empty, float z_far) {
@@ -238,7 +263,7 @@ __device__ __forceinline__ Texel pull_generic(int l, int ox, int oy, int ism_x, 
   return pull_combine(ds, mds, rs, dx, dy, valid);
 }
 
-__device__ __forceinline__ uint16_t to_r16(float depth) {
+__host__ __device__ __forceinline__ uint16_t to_r16(float depth) {
   const float r = fminf(fmaxf(depth, 0.0f), 1.0f);
   return (uint16_t)rintf(r * 65535.0f);
 }
@@ -247,8 +272,8 @@ __device__ __forceinline__ uint16_t to_r16(float depth) {
 // (fine side so) writes fine pixels 2g - 1 + {0,1}^2 that lie in the ISM.
 // Level 0 reads the originals from the atlas tile and leaves the unorm16
 // result in the same slot; other levels update their texels in place.
-__device__ __forceinline__ void push_quad(int l, int gx, int gy, int so, const Texel* coarse,
-                                          Texel* same, uint32_t* base) {
+__host__ __device__ __forceinline__ void push_quad(int l, int gx, int gy, int so, const Texel* coarse,
+                                          Texel* same, uint32_t* base, const float* rcp) {
   const int sc = so / 2;
   const float scale = __builtin_ldexpf(1.0f, -l);
   // offsets {0,0},{0,1},{1,1},{1,0} for both neighbours and fine pixels
@@ -319,11 +344,12 @@ __device__ __forceinline__ void push_quad(int l, int gx, int gy, int so, const T
     Texel res = orig;
     if (wacc > 0 && (invalid || occluded)) {
       const float fw = (float)wacc;
+      const float y = rcp[wacc];
       float dacc = 0.0f;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         if (w[i] != 0) dacc = dacc + cd[i] * (float)w[i];
-      res.depth = dacc / fw;
+      res.depth = div_by_recip(dacc, fw, y);
       if (l > 0) {
         float macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
 #pragma unroll
@@ -335,9 +361,9 @@ __device__ __forceinline__ void push_quad(int l, int gx, int gy, int so, const T
           xacc = xacc + ddx[i] * fwi;
           yacc = yacc + ddy[i] * fwi;
         }
-        res.max_depth = macc / fw;
-        res.radius = racc / fw;
-        res.disp = pack_half2_hd(xacc / fw, yacc / fw);
+        res.max_depth = div_by_recip(macc, fw, y);
+        res.radius = div_by_recip(racc, fw, y);
+        res.disp = pack_half2_hd(div_by_recip(xacc, fw, y), div_by_recip(yacc, fw, y));
       }
     }
     if (l == 0)
@@ -355,7 +381,7 @@ __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __re
   constexpr int kLog = (S == 8) ? 3 : (S == 16) ? 4 : (S == 32) ? 5 : (S == 64) ? 6 : 7;
   __shared__ uint32_t base[S * S];
   __shared__ Texel lv[level_offset<S>(kLog + 1)];
-  __shared__ float rtab[256], mtab[256];
+  __shared__ float rtab[256], mtab[256], rcp[17];
 
   const int tid = threadIdx.x;
   const int n1d = atlas_side / S;
@@ -369,6 +395,7 @@ __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __re
     rtab[b] = r;
     mtab[b] = r * 2.0f / z_far * 0.6f;
   }
+  if (tid < 17) rcp[tid] = kRcp[tid];
   // ---- load the ISM's atlas tile (S*S u32) with 16-byte loads
   for (int i = tid; i < S * S / 4; i += kThreads) {
     const int row = (i * 4) / S, col = (i * 4) % S;
@@ -411,7 +438,7 @@ __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __re
     const int so = S >> l, nq1 = so / 2 + 1;
     const Texel* coarse = lv + level_offset<S>(l + 1);
     Texel* same = lv + level_offset<S>(l);
-    for (int q = tid; q < nq1 * nq1; q += kThreads) push_quad(l, q % nq1, q / nq1, so, coarse, same, base);
+    for (int q = tid; q < nq1 * nq1; q += kThreads) push_quad(l, q % nq1, q / nq1, so, coarse, same, base, rcp);
     __syncthreads();
   }
 

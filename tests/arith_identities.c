@@ -1,0 +1,69 @@
+/* Exhaustive and randomized checks of the exact arithmetic rewrites used by
+ * the pull-push kernel, against plain IEEE float operations (the form the
+ * FP32 oracle uses). Built and run by tests/test_arith_identities.py. */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+/* x / b from y = RN(1/b): q = RN(x y), r = fma(-q, b, x), RN(q + r y). */
+static float div_by_recip(float x, float b, float y) {
+  volatile float q = x * y;
+  const float r = fmaf(-q, b, x);
+  return fmaf(r, y, q);
+}
+
+/* Mismatches of div_by_recip against x / b for every significand of x in the
+ * binades [2^e0, 2^e1), both signs; y = RN(1/b) moved by y_ulps ulps (0 for
+ * the kernel's form, nonzero to show that the harness catches a bad y). */
+long check_div_by_recip(float b, int y_ulps, int e0, int e1) {
+  volatile float bb = b;
+  const float y = u2f(f2u(1.0f / bb) + (uint32_t)y_ulps);
+  long bad = 0;
+  for (int e = e0; e < e1; ++e)
+    for (uint32_t m = 0; m < (1u << 23); ++m)
+      for (uint32_t s = 0; s < 2; ++s) {
+        const float x = u2f((s << 31) | ((uint32_t)(e + 127) << 23) | m);
+        volatile float ref = x / bb;
+        if (f2u(div_by_recip(x, b, y)) != f2u(ref)) ++bad;
+      }
+  return bad;
+}
+
+/* m^2 with m the midpoint between R and the next float (exact in double);
+ * midpoint = 0 gives the naive bound R^2 instead. */
+static double sqrt_gt_bound(float R, int midpoint) {
+  const float n = u2f(f2u(R) + 1u);
+  const double m = midpoint ? ((double)R + (double)n) * 0.5 : (double)R;
+  return m * m;
+}
+
+static uint64_t splitmix(uint64_t* s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* Mismatches of (double)x > sqrt_gt_bound(R) against sqrtf(x) > R for n
+ * random radii R in [2^-8, 2^5) and, for each, the 65 floats x around
+ * RN(R * R) (where the decisions change) plus one random x. */
+long check_sqrt_bound(long n, uint64_t seed, int midpoint) {
+  long bad = 0;
+  for (long i = 0; i < n; ++i) {
+    const uint32_t e = 119u + (uint32_t)(splitmix(&seed) % 13u);
+    const float R = u2f((e << 23) | (uint32_t)(splitmix(&seed) & 0x7FFFFFu));
+    const double b = sqrt_gt_bound(R, midpoint);
+    const uint32_t c = f2u(R * R);
+    for (int k = -33; k <= 32; ++k) {
+      const float x = k < 32 ? u2f(c + (uint32_t)k) : R * R * (float)(splitmix(&seed) % 1000u) / 500.0f;
+      volatile float sx = sqrtf(x);
+      const int ref = sx > R;
+      const int got = (double)x > b;
+      if (ref != got) ++bad;
+    }
+  }
+  return bad;
+}

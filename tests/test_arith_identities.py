@@ -1,0 +1,52 @@
+"""Pins the exact arithmetic rewrites of the pull-push kernel (CPU only).
+
+The kernel replaces two float operations of the reference's shaders by
+cheaper forms that give the same bits:
+  * x / w for the push and pull weight sums (w = 1..16) and x / 3.14 in the
+    radius decode become RN(q + r y) with y = RN(1/w), q = RN(x y) and the
+    exact FMA remainder r (Markstein's theorem);
+  * sqrt(x) > R in the radius tests becomes (double)x > m^2, m the midpoint
+    between R and the next float.
+Both are checked here against IEEE float division and sqrtf: the division
+over every significand of two binades per divisor and sign (scaling by a
+power of two leaves all three roundings unchanged), the comparison on random
+radii at the 65 floats around R^2 where the decision flips.
+"""
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+SRC = Path(__file__).with_name("arith_identities.c")
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    so = tmp_path_factory.mktemp("arith") / "libarith.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", str(SRC), "-o", str(so), "-lm"],
+                   check=True)
+    L = C.CDLL(str(so))
+    L.check_div_by_recip.argtypes = [C.c_float, C.c_int, C.c_int, C.c_int]
+    L.check_div_by_recip.restype = C.c_long
+    L.check_sqrt_bound.argtypes = [C.c_long, C.c_uint64, C.c_int]
+    L.check_sqrt_bound.restype = C.c_long
+    return L
+
+
+@pytest.mark.parametrize("w", list(range(1, 17)) + [3.14])
+def test_division_by_reciprocal_is_exact(lib, w):
+    assert lib.check_div_by_recip(float(np.float32(w)), 0, -3, -1) == 0
+
+
+def test_sqrt_comparison_without_root(lib):
+    assert lib.check_sqrt_bound(200_000, 12345, 1) == 0
+
+
+def test_planted_mistakes_are_caught(lib):
+    # a reciprocal far enough off that one correction step cannot repair it
+    # (one ulp low already fails for 3.14), or the naive bound R^2, is caught
+    assert lib.check_div_by_recip(3.0, 4096, -1, 0) > 0
+    assert lib.check_div_by_recip(3.14, -1, -1, 0) > 0
+    assert lib.check_sqrt_bound(20_000, 12345, 0) > 0

@@ -42,9 +42,11 @@ def outputs(st):
             st.gi_buffer.numpy())
 
 
-def check_against_oracle(inp, wl, p, rec, atlas, r16, img, tol=1e-4):
+def check_against_oracle(inp, wl, p, rec, atlas, r16, img, tol=1e-4, points=None, bin_offsets=None):
     pos, nrm, col = rec[:, 0:3], rec[:, 4:7], rec[:, 16:19]
-    ref_atlas = oracle.ism_splat(inp.points, inp.bin_offsets, pos, nrm, wl.atlas_side, p.z_far)
+    points = inp.points if points is None else points
+    bin_offsets = inp.bin_offsets if bin_offsets is None else bin_offsets
+    ref_atlas = oracle.ism_splat(points, bin_offsets, pos, nrm, wl.atlas_side, p.z_far)
     assert np.array_equal(atlas, ref_atlas)
     ref_r16 = oracle.pullpush(atlas, wl.ism_side, wl.levels, p.z_far)
     assert np.array_equal(r16, ref_r16)
@@ -99,14 +101,15 @@ def test_variants_bit_identical(ctx):
 def test_empty_point_set(ctx):
     wl = W.WORKLOADS["tiny"]
     inp, p, st, rsm, pts, gb = setup_frame(ctx, wl)
-    off = ctx.upload(np.zeros(wl.n_vpl + 1, np.uint32))
-    empty = mlgi.points_desc(ctx.upload(np.zeros((1, 4), np.float32)), off)
+    zeros = np.zeros(wl.n_vpl + 1, np.uint32)
+    empty = mlgi.points_desc(ctx.upload(np.zeros((1, 4), np.float32)), ctx.upload(zeros))
     empty.count = 0
     st.process(rsm, empty, gb)
     ctx.sync()
     rec, atlas, r16, img = outputs(st)
     assert np.all(atlas == 0xFFFFFFFF) and np.all(r16 == 65535)
-    check_against_oracle(inp, wl, p, rec, atlas, r16, img)
+    check_against_oracle(inp, wl, p, rec, atlas, r16, img, points=np.zeros((0, 4), np.float32),
+                         bin_offsets=zeros)
 
 
 @pytest.mark.parametrize("il", [1, 2, 8])
