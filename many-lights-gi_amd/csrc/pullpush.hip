@@ -14,14 +14,19 @@
 // ISMs are independent and nothing outside the tile is read. Zero-weight
 // terms contribute +-0 to the push sums, so skipping them is exact.
 //
+// Levels are compile-time (one kernel per ISM side and level count), so every
+// level's extents, scale and the fine-to-coarse offsets are constants.
+//
 // Most of an atlas is empty (0xFFFFFFFF), and an empty texel decodes to a
-// fixed, valid far sample, so whole regions of every pull level hold one
-// constant texel E_l (computed on the host by the same code and passed as an
-// argument): a pull output whose four inputs all equal E_{l-1} is E_l.
+// fixed, valid far sample: the first pull level of an all-empty 2x2 block is
+// one constant E_1 (computed on the host by the same code and passed as an
+// argument), taken when a whole wave's blocks are empty.
 //
 // Push runs over the reference's invocations: the (up to) four fine pixels
 // that share the same four coarse neighbours (push.comp:46-67), so loads,
-// half unpacking and per-neighbour constants are formed once per quad.
+// half unpacking and per-neighbour constants are formed once per quad. The
+// coarse-to-fine offset (coarse + 0.5) * 2 - (fine + 0.5) of each pair is one
+// of +-0.5, +-1.5, exact in the reference's global coordinates as well.
 // The radius tests compare a correctly rounded square root with a float,
 // sqrt_rn(x) > R. Because sqrt(x) of a float x can never equal the midpoint m
 // between R and the next float (m has 25 significant bits, m^2 then has more
@@ -41,19 +46,11 @@ struct __align__(16) Texel {
 };
 
 constexpr int kThreads = 256;
-constexpr int kMaxLevels = 7;
 
 struct PullConsts {
-  Texel e[kMaxLevels + 1];  // e[l]: pull level l of an all-empty block (l >= 1)
-  int pull_fast;            // every E_l has zero displacement (value position-free)
+  Texel e1;  // pull level 1 of an all-empty 2x2 block
 };
 
-template <int S>
-constexpr int level_offset(int l) {  // texels of pull levels 1..l-1
-  int o = 0;
-  for (int k = 1; k < l; ++k) o += (S >> k) * (S >> k);
-  return o;
-}
 
 __host__ __device__ inline uint32_t f2u(float f) {
   union {
@@ -212,28 +209,37 @@ __host__ __device__ inline Texel pull_combine(const float* ds, const float* mds,
   return res;
 }
 
-__host__ __device__ __forceinline__ bool same_texel(const Texel& a, const Texel& b) {
-  return f2u(a.depth) == f2u(b.depth) && f2u(a.max_depth) == f2u(b.max_depth) &&
-         f2u(a.radius) == f2u(b.radius) && a.disp == b.disp;
+
+template <int S>
+constexpr int ism_log2() {
+  return (S == 8) ? 3 : (S == 16) ? 4 : (S == 32) ? 5 : 6;
 }
 
-// Generic pull of output (ox, oy) at level l >= 1 (pull.comp:20-126).
-template <int S>
-__host__ __device__ __forceinline__ Texel pull_generic(int l, int ox, int oy, int ism_x, int ism_y,
-                                              const uint32_t* base, const Texel* src,
-                                              const float* rtab, const float* mtab,
-                                              const Decoded& empty, float z_far) {
-  const int so = S >> l, si = so * 2;
-  const float scale = __builtin_ldexpf(1.0f, -l);  // pow(2, -level)
-  const int gox = ism_x * so + ox, goy = ism_y * so + oy;  // global output texel
+__host__ __device__ __forceinline__ uint16_t to_r16(float depth) {
+  const float r = fminf(fmaxf(depth, 0.0f), 1.0f);
+  return (uint16_t)rintf(r * 65535.0f);
+}
+
+// Pull of output (ox, oy) of level l >= 1 (pull.comp:20-126). Level 1 reads
+// the atlas tile and decodes it; higher levels read pull level l - 1.
+template <int S, int l>
+__host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y,
+                                                   const uint32_t* base, const Texel* src,
+                                                   const float* rtab, const float* mtab,
+                                                   const Decoded& empty, float z_far) {
+  constexpr int so = S >> l, si = so * 2;
+  constexpr float scale = 1.0f / (float)(1 << l);  // pow(2, -level)
   float ds[4], mds[4], rs[4], dx[4], dy[4];
   bool valid[4];
+  // global output position and the first input column / row, as floats
+  const float fox = (float)(ism_x * so + ox) + 0.5f, foy = (float)(ism_y * so + oy) + 0.5f;
+  const float fix = (float)(2 * (ism_x * so + ox)) + 0.5f, fiy = (float)(2 * (ism_y * so + oy)) + 0.5f;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     // offsets {0,0},{0,1},{1,0},{1,1}
     const int offx = i >> 1, offy = i & 1;
     const int ix = ox * 2 + offx, iy = oy * 2 + offy;
-    if (l == 1) {
+    if constexpr (l == 1) {
       const uint32_t s = base[iy * S + ix];
       const uint32_t b = s & 0xFFu;
       const Decoded d = (s == 0xFFFFFFFFu) ? empty : decode_texel(s, rtab[b], mtab[b], z_far, S);
@@ -249,9 +255,9 @@ __host__ __device__ __forceinline__ Texel pull_generic(int l, int ox, int oy, in
       const Texel tx = src[iy * si + ix];
       float disp_x, disp_y;
       unpack_half2_hd(tx.disp, disp_x, disp_y);
-      const float gix = (float)(gox * 2 + offx), giy = (float)(goy * 2 + offy);
-      dx[i] = (gix + 0.5f + disp_x) / 2.0f - ((float)gox + 0.5f);
-      dy[i] = (giy + 0.5f + disp_y) / 2.0f - ((float)goy + 0.5f);
+      // (in + 0.5 + disp) / 2 - (out + 0.5), in the reference's global texels
+      dx[i] = ((fix + (float)offx) + disp_x) * 0.5f - fox;
+      dy[i] = ((fiy + (float)offy) + disp_y) * 0.5f - foy;
       ds[i] = tx.depth;
       mds[i] = tx.max_depth;
       rs[i] = tx.radius;
@@ -263,21 +269,19 @@ __host__ __device__ __forceinline__ Texel pull_generic(int l, int ox, int oy, in
   return pull_combine(ds, mds, rs, dx, dy, valid);
 }
 
-__host__ __device__ __forceinline__ uint16_t to_r16(float depth) {
-  const float r = fminf(fmaxf(depth, 0.0f), 1.0f);
-  return (uint16_t)rintf(r * 65535.0f);
-}
-
-// One push invocation (push.comp:44-165): quad g = (gx, gy) of level l
-// (fine side so) writes fine pixels 2g - 1 + {0,1}^2 that lie in the ISM.
-// Level 0 reads the originals from the atlas tile and leaves the unorm16
-// result in the same slot; other levels update their texels in place.
-__host__ __device__ __forceinline__ void push_quad(int l, int gx, int gy, int so, const Texel* coarse,
-                                          Texel* same, uint32_t* base, const float* rcp) {
-  const int sc = so / 2;
-  const float scale = __builtin_ldexpf(1.0f, -l);
+// One push invocation (push.comp:44-165): quad (gx, gy) of level l writes
+// the fine pixels 2g - 1 + {0,1}^2 that lie in the ISM from the four coarse
+// neighbours g - 1 + {0,1}^2 of level l + 1. Level 0 reads the originals from
+// the atlas tile and leaves the unorm16 result in the same slot; other levels
+// update their texels in place. Zero-weight terms add +0 to every sum (all
+// summands are finite), so the sums run over all four neighbours.
+template <int S, int l>
+__host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, Texel* same,
+                                                   uint32_t* base, const float* rcp) {
+  constexpr int so = S >> l, sc = so / 2;
+  constexpr float scale = 1.0f / (float)(1 << l);
   // offsets {0,0},{0,1},{1,1},{1,0} for both neighbours and fine pixels
-  const int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
+  constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
   bool usable[4];
   float cd[4], cmd[4], cr[4], cdx2[4], cdy2[4];
   double bound[4];
@@ -302,71 +306,67 @@ __host__ __device__ __forceinline__ void push_quad(int l, int gx, int gy, int so
   for (int op = 0; op < 4; ++op) {
     const int px = 2 * gx - 1 + ox[op], py = 2 * gy - 1 + oy[op];
     if (px < 0 || py < 0 || px >= so || py >= so) continue;
-    int w[4];
-    float ddx[4], ddy[4];
+    float w[4], ddx[4], ddy[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
-      w[i] = usable[i] ? ((wt == 0) ? 9 : (wt == 2 ? 1 : 3)) : 0;
-      // (coarse + 0.5) * 2 - (fine + 0.5) is exact
-      const float ctx = ((float)(gx - 1 + ox[i]) + 0.5f) * 2.0f;
-      const float cty = ((float)(gy - 1 + oy[i]) + 0.5f) * 2.0f;
-      const float ttx = (float)px + 0.5f, tty = (float)py + 0.5f;
-      ddx[i] = ctx - ttx + cdx2[i];
-      ddy[i] = cty - tty + cdy2[i];
+      w[i] = usable[i] ? ((wt == 0) ? 9.0f : (wt == 2 ? 1.0f : 3.0f)) : 0.0f;
+      // (coarse + 0.5) * 2 - (fine + 0.5) = 2 o_i - o_op - 0.5, exact
+      ddx[i] = (float)(2 * ox[i] - ox[op]) - 0.5f + cdx2[i];
+      ddy[i] = (float)(2 * oy[i] - oy[op]) - 0.5f + cdy2[i];
       const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
-      if ((double)x > bound[i]) w[i] = 0;  // sqrt(x) > radius * 2^-level
+      if ((double)x > bound[i]) w[i] = 0.0f;  // sqrt(x) > radius * 2^-level
     }
     float minimum = 9001.0f;
     float max_depth = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (w[i] == 0) continue;
+      if (w[i] == 0.0f) continue;
       minimum = fminf(cd[i], minimum);
       if (minimum == cd[i]) max_depth = cmd[i];
     }
-    int wacc = 0;
+    float wacc = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (w[i] != 0 && cd[i] > max_depth) w[i] = 0;
-      wacc += w[i];
+      if (cd[i] > max_depth) w[i] = 0.0f;
+      wacc = wacc + w[i];
     }
     const int t = py * so + px;
     Texel orig;
-    if (l == 0)
-      orig = Texel{(float)(base[t] >> 8) / 16777216.0f, 0.0f, 0.0f, 0u};
-    else
+    bool invalid;
+    if constexpr (l == 0) {
+      // (s >> 8) < 2^24: a level-0 original is never the invalid depth 1
+      orig = Texel{(float)(base[t] >> 8) * (1.0f / 16777216.0f), 0.0f, 0.0f, 0u};
+      invalid = false;
+    } else {
       orig = same[t];
-    const bool invalid = orig.depth == 1.0f;
+      invalid = orig.depth == 1.0f;
+    }
     bool occluded = false;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) occluded = occluded || (w[i] > 0 && orig.depth > cmd[i]);
+    for (int i = 0; i < 4; ++i) occluded = occluded || (w[i] > 0.0f && orig.depth > cmd[i]);
     Texel res = orig;
-    if (wacc > 0 && (invalid || occluded)) {
-      const float fw = (float)wacc;
-      const float y = rcp[wacc];
-      float dacc = 0.0f;
+    if (wacc > 0.0f && (invalid || occluded)) {
+      const float y = rcp[(int)wacc];
+      float dacc = cd[0] * w[0];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (w[i] != 0) dacc = dacc + cd[i] * (float)w[i];
-      res.depth = div_by_recip(dacc, fw, y);
-      if (l > 0) {
-        float macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
+      for (int i = 1; i < 4; ++i) dacc = dacc + cd[i] * w[i];
+      res.depth = div_by_recip(dacc, wacc, y);
+      if constexpr (l > 0) {
+        float macc = cmd[0] * w[0], racc = cr[0] * w[0], xacc = ddx[0] * w[0], yacc = ddy[0] * w[0];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (w[i] == 0) continue;
-          const float fwi = (float)w[i];
-          macc = macc + cmd[i] * fwi;
-          racc = racc + cr[i] * fwi;
-          xacc = xacc + ddx[i] * fwi;
-          yacc = yacc + ddy[i] * fwi;
+        for (int i = 1; i < 4; ++i) {
+          macc = macc + cmd[i] * w[i];
+          racc = racc + cr[i] * w[i];
+          xacc = xacc + ddx[i] * w[i];
+          yacc = yacc + ddy[i] * w[i];
         }
-        res.max_depth = div_by_recip(macc, fw, y);
-        res.radius = div_by_recip(racc, fw, y);
-        res.disp = pack_half2_hd(div_by_recip(xacc, fw, y), div_by_recip(yacc, fw, y));
+        res.max_depth = div_by_recip(macc, wacc, y);
+        res.radius = div_by_recip(racc, wacc, y);
+        res.disp = pack_half2_hd(div_by_recip(xacc, wacc, y), div_by_recip(yacc, wacc, y));
       }
     }
-    if (l == 0)
+    if constexpr (l == 0)
       base[t] = to_r16(res.depth);  // the tile slot now holds the result
     else
       same[t] = res;
@@ -374,13 +374,68 @@ __host__ __device__ __forceinline__ void push_quad(int l, int gx, int gy, int so
 }
 
 template <int S>
+constexpr int level_offset(int l) {  // texels of pull levels 1..l-1
+  int o = 0;
+  for (int k = 1; k < l; ++k) o += (S >> k) * (S >> k);
+  return o;
+}
+
+struct Tables {
+  const uint32_t* base;
+  Texel* lv;
+  const float* rtab;
+  const float* mtab;
+  const float* rcp;
+  Decoded empty;
+  float z_far;
+  int ism_x, ism_y;
+};
+
+template <int S, int l, int L>
+__device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& pc) {
+  if constexpr (l <= L) {
+    constexpr int so = S >> l, n = so * so;
+    Texel* dst = tb.lv + level_offset<S>(l);
+    const Texel* src = tb.lv + level_offset<S>(l - 1);
+    for (int t = threadIdx.x; t < n; t += kThreads) {
+      const int ox = t % so, oy = t / so;
+      if constexpr (l == 1) {
+        // a wave whose 2x2 blocks are all empty writes E_1
+        const uint2 a = *reinterpret_cast<const uint2*>(&tb.base[(2 * oy) * S + 2 * ox]);
+        const uint2 b = *reinterpret_cast<const uint2*>(&tb.base[(2 * oy + 1) * S + 2 * ox]);
+        const bool all_empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
+        if (__ballot(!all_empty) == 0ull) {
+          dst[t] = pc.e1;
+          continue;
+        }
+      }
+      dst[t] = pull_one<S, l>(ox, oy, tb.ism_x, tb.ism_y, tb.base, src, tb.rtab, tb.mtab, tb.empty,
+                              tb.z_far);
+    }
+    __syncthreads();
+    pull_levels<S, l + 1, L>(tb, pc);
+  }
+}
+
+template <int S, int l>
+__device__ __forceinline__ void push_levels(const Tables& tb, uint32_t* base) {
+  if constexpr (l >= 0) {
+    constexpr int so = S >> l, nq1 = so / 2 + 1;
+    const Texel* coarse = tb.lv + level_offset<S>(l + 1);
+    Texel* same = tb.lv + level_offset<S>(l);
+    for (int q = threadIdx.x; q < nq1 * nq1; q += kThreads)
+      push_quad<S, l>(q % nq1, q / nq1, coarse, same, base, tb.rcp);
+    __syncthreads();
+    push_levels<S, l - 1>(tb, base);
+  }
+}
+
+template <int S, int L>
 __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __restrict__ atlas,
-                                                            int atlas_side, int levels,
-                                                            float z_far, PullConsts pc,
+                                                            int atlas_side, float z_far, PullConsts pc,
                                                             uint16_t* __restrict__ out) {
-  constexpr int kLog = (S == 8) ? 3 : (S == 16) ? 4 : (S == 32) ? 5 : (S == 64) ? 6 : 7;
   __shared__ uint32_t base[S * S];
-  __shared__ Texel lv[level_offset<S>(kLog + 1)];
+  __shared__ Texel lv[level_offset<S>(L + 1)];
   __shared__ float rtab[256], mtab[256], rcp[17];
 
   const int tid = threadIdx.x;
@@ -403,44 +458,12 @@ __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __re
     *reinterpret_cast<uint4*>(&base[row * S + col]) = v;
   }
   __syncthreads();
+  Tables tb{base, lv, rtab, mtab, rcp, {}, z_far, ism_x, ism_y};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
-  const Decoded empty = decode_texel(0xFFFFFFFFu, rtab[255], mtab[255], z_far, S);
+  tb.empty = decode_texel(0xFFFFFFFFu, rtab[255], mtab[255], z_far, S);
 
-  // ---- pull, level l written from level l-1
-  for (int l = 1; l <= levels; ++l) {
-    const int so = S >> l, n = so * so;
-    Texel* dst = lv + level_offset<S>(l);
-    const Texel* src = lv + level_offset<S>(l - 1);
-    for (int t = tid; t < n; t += kThreads) {
-      const int ox = t % so, oy = t / so;
-      bool all_empty = false;
-      if (!pc.pull_fast) {
-      } else if (l == 1) {
-        const uint2 a = *reinterpret_cast<const uint2*>(&base[(2 * oy) * S + 2 * ox]);
-        const uint2 b = *reinterpret_cast<const uint2*>(&base[(2 * oy + 1) * S + 2 * ox]);
-        all_empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
-      } else {
-        const int si = so * 2;
-        const Texel ep = pc.e[l - 1];
-        all_empty = same_texel(src[(2 * oy) * si + 2 * ox], ep) &&
-                    same_texel(src[(2 * oy) * si + 2 * ox + 1], ep) &&
-                    same_texel(src[(2 * oy + 1) * si + 2 * ox], ep) &&
-                    same_texel(src[(2 * oy + 1) * si + 2 * ox + 1], ep);
-      }
-      dst[t] = all_empty ? pc.e[l]
-                         : pull_generic<S>(l, ox, oy, ism_x, ism_y, base, src, rtab, mtab, empty, z_far);
-    }
-    __syncthreads();
-  }
-
-  // ---- push, level l written from level l+1 in place, one quad per lane
-  for (int l = levels - 1; l >= 0; --l) {
-    const int so = S >> l, nq1 = so / 2 + 1;
-    const Texel* coarse = lv + level_offset<S>(l + 1);
-    Texel* same = lv + level_offset<S>(l);
-    for (int q = tid; q < nq1 * nq1; q += kThreads) push_quad(l, q % nq1, q / nq1, so, coarse, same, base, rcp);
-    __syncthreads();
-  }
+  pull_levels<S, 1, L>(tb, pc);
+  push_levels<S, L - 1>(tb, base);
 
   // ---- write the unorm16 tile (held in the low halves of the tile slots)
   for (int i = tid; i < S * S / 4; i += kThreads) {
@@ -451,76 +474,70 @@ __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __re
   }
 }
 
-// Host-side constants: the pull levels of an all-empty ISM, computed with
+// Host-side constant: pull level 1 of an all-empty 2x2 block, computed with
 // the same functions the kernel uses.
-PullConsts make_consts(int S, int levels, float z_far) {
+PullConsts make_consts(int S, float z_far) {
   PullConsts pc{};
   const float r = (float)255 / 10.0f;
   const float madd = r * 2.0f / z_far * 0.6f;
   const Decoded e0 = decode_texel(0xFFFFFFFFu, r, madd, z_far, S);
   float ds[4], mds[4], rs[4], dx[4], dy[4];
   bool valid[4];
-  for (int l = 1; l <= levels && l <= kMaxLevels; ++l) {
-    const float scale = std::ldexp(1.0f, -l);
-    for (int i = 0; i < 4; ++i) {
-      const int offx = i >> 1, offy = i & 1;
-      float dist;
-      if (l == 1) {
-        ds[i] = e0.depth;
-        mds[i] = e0.max_depth;
-        rs[i] = e0.radius;
-        dx[i] = offx ? 0.25f : -0.25f;
-        dy[i] = offy ? 0.25f : -0.25f;
-        dist = 0.35355339f;
-      } else {
-        const Texel& t = pc.e[l - 1];
-        float disp_x, disp_y;
-        unpack_half2_hd(t.disp, disp_x, disp_y);
-        // output (0, 0): the value is the same at every position while the
-        // displacement is zero, which holds for every E_l
-        dx[i] = ((float)offx + 0.5f + disp_x) / 2.0f - 0.5f;
-        dy[i] = ((float)offy + 0.5f + disp_y) / 2.0f - 0.5f;
-        dist = sqrtf(dx[i] * dx[i] + dy[i] * dy[i]);
-        ds[i] = t.depth;
-        mds[i] = t.max_depth;
-        rs[i] = t.radius;
-      }
-      valid[i] = ds[i] != 1.0f && dist <= rs[i] * scale;
-    }
-    pc.e[l] = pull_combine(ds, mds, rs, dx, dy, valid);
+  for (int i = 0; i < 4; ++i) {
+    ds[i] = e0.depth;
+    mds[i] = e0.max_depth;
+    rs[i] = e0.radius;
+    dx[i] = (i >> 1) ? 0.25f : -0.25f;
+    dy[i] = (i & 1) ? 0.25f : -0.25f;
+    valid[i] = ds[i] != 1.0f && 0.35355339f <= rs[i] * 0.5f;
   }
-  // the all-empty shortcut is position-free only while the displacement is 0
-  pc.pull_fast = 1;
-  for (int l = 1; l <= levels && l <= kMaxLevels; ++l)
-    if (pc.e[l].disp != 0u) pc.pull_fast = 0;
+  pc.e1 = pull_combine(ds, mds, rs, dx, dy, valid);
   return pc;
+}
+
+template <int S, int L>
+hipError_t launch_sl(hipStream_t s, const uint32_t* atlas, int atlas_side, float z_far,
+                     const PullConsts& pc, uint16_t* out) {
+  const int n1d = atlas_side / S;
+  hipLaunchKernelGGL((pullpush_kernel<S, L>), dim3(n1d * n1d), dim3(kThreads), 0, s, atlas, atlas_side,
+                     z_far, pc, out);
+  return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_s(hipStream_t s, const uint32_t* atlas, int atlas_side, int levels, float z_far,
+                    const PullConsts& pc, uint16_t* out) {
+  constexpr int kMax = ism_log2<S>();
+  switch (levels) {
+    case 1: return launch_sl<S, 1>(s, atlas, atlas_side, z_far, pc, out);
+    case 2: return launch_sl<S, 2>(s, atlas, atlas_side, z_far, pc, out);
+    case 3: return launch_sl<S, 3>(s, atlas, atlas_side, z_far, pc, out);
+    case 4:
+      if constexpr (kMax >= 4) return launch_sl<S, 4>(s, atlas, atlas_side, z_far, pc, out);
+      break;
+    case 5:
+      if constexpr (kMax >= 5) return launch_sl<S, 5>(s, atlas, atlas_side, z_far, pc, out);
+      break;
+    case 6:
+      if constexpr (kMax >= 6) return launch_sl<S, 6>(s, atlas, atlas_side, z_far, pc, out);
+      break;
+    default: break;
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 hipError_t launch_pullpush(hipStream_t s, const uint32_t* atlas, int atlas_side, int ism_side,
                            int levels, float z_far, uint16_t* out) {
-  if (levels < 1 || levels > kMaxLevels) return hipErrorInvalidValue;
-  const int n1d = atlas_side / ism_side;
-  const dim3 grid(n1d * n1d), block(kThreads);
-  const PullConsts pc = make_consts(ism_side, levels, z_far);
+  const PullConsts pc = make_consts(ism_side, z_far);
   switch (ism_side) {
-    case 8:
-      hipLaunchKernelGGL(pullpush_kernel<8>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
-      break;
-    case 16:
-      hipLaunchKernelGGL(pullpush_kernel<16>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
-      break;
-    case 32:
-      hipLaunchKernelGGL(pullpush_kernel<32>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
-      break;
-    case 64:
-      hipLaunchKernelGGL(pullpush_kernel<64>, grid, block, 0, s, atlas, atlas_side, levels, z_far, pc, out);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case 8: return launch_s<8>(s, atlas, atlas_side, levels, z_far, pc, out);
+    case 16: return launch_s<16>(s, atlas, atlas_side, levels, z_far, pc, out);
+    case 32: return launch_s<32>(s, atlas, atlas_side, levels, z_far, pc, out);
+    case 64: return launch_s<64>(s, atlas, atlas_side, levels, z_far, pc, out);
+    default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 }  // namespace mlgi
