@@ -1,7 +1,7 @@
 """GI frame benchmark on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1080p]
-                    [--form fp32|mixed|both] [--no-cpu] [--check]
+                    [--form fp32|mixed|both] [--no-cpu]
 
 A step is one GI frame of the many-lights hot path on inputs already resident
 in HBM: VPL extraction from the RSM, ISM atlas clear + point splat, pull-push,
@@ -13,10 +13,13 @@ The default workload is the 1920x1080, 4,096-VPL configuration (40k-triangle
 procedural room, 128^2 RSM, 1M splat points into 4,096 64^2 ISMs of a 4096^2
 atlas, 4 pull-push levels, 8x8 interleave, 16x16 tiles).
 
-roofline: the final gather dominates; its algorithmic work is 84 flops per
-lit pixel-VPL pair (pairs with a positive geometry factor, counted on the
-device), divided by the gather kernel's mean duration from HIP events on the
-context stream, against the MI355X FP32 vector peak.
+roofline: for the kernel with the largest mean duration (HIP events on the
+context stream over the timed frames): pull-push against HBM (6 bytes per
+atlas texel: the u32 atlas read once, the unorm16 ISM written once); the
+final gather against the FP32 vector peak (84 flops per lit pixel-VPL pair,
+pairs counted on the device). Every kernel's figure is under "kernels";
+"traffic" is the HBM bytes per launch measured with rocprofv3 counters
+(profiles/traffic_<workload>.json), null until measured.
 cpu_baseline: the oracle's FP32 instantiation of the same four stages run on
 the host for one full frame (final gathering threaded, the rest on one
 thread), scaled to ms/frame.
@@ -44,7 +47,6 @@ def log(*a):
 
 
 def run_form(ctx, wl, inp, precision: int, steps: int, warmup: int):
-    import numpy as np
     import mlgi
 
     p = wl.params(precision=precision)
@@ -56,20 +58,53 @@ def run_form(ctx, wl, inp, precision: int, steps: int, warmup: int):
     for _ in range(warmup):
         st.process(rsm, pts, gb)
     ctx.sync()
+    # timed region: K frames between device synchronisations; HIP events on
+    # the context stream bracket every stage of every frame
+    ctx.profile(True)
+    ctx.profile_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
         st.process(rsm, pts, gb)
     ctx.sync()
     wall_ms = (time.perf_counter() - t0) * 1e3 / steps
-    # per-stage device times (separate, untimed pass with HIP events)
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(max(3, min(steps, 20))):
-        st.process(rsm, pts, gb)
     stages = ctx.profile_read()
     ctx.profile(False)
     lit = st.count_lit_pairs(gb)
     return {"st": st, "rsm": rsm, "pts": pts, "gb": gb, "wall_ms": wall_ms, "stages": stages, "lit": lit}
+
+
+def kernel_rooflines(wl, stages, lit):
+    """Per-kernel roofline figures from the live stage times.
+
+    pull-push: reads the u32 atlas once and writes the unorm16 ISM once,
+      6 bytes per atlas texel, against HBM.
+    splat: reads every 16-byte point record once (lower bound: the accepted
+      atomicMins add at most 4 x 4 bytes per point), against HBM.
+    final gather: 84 flops per lit pixel-VPL pair, against the FP32 vector
+      peak.
+    """
+    a2 = wl.atlas_side * wl.atlas_side
+    out = {}
+    t = stages["pullpush_ms"] * 1e-3
+    if t > 0:
+        gbs = 6.0 * a2 / t / 1e9
+        out["pullpush"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": 6 * a2,
+                           "kernel_ms": round(stages["pullpush_ms"], 4)}
+    t = stages["ism_splat_ms"] * 1e-3
+    if t > 0:
+        gbs = 16.0 * wl.n_points / t / 1e9
+        out["ism_splat"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": 16 * wl.n_points,
+                            "kernel_ms": round(stages["ism_splat_ms"], 4)}
+    t = stages["final_gather_ms"] * 1e-3
+    if t > 0:
+        tf = FLOPS_PER_LIT_PAIR * lit / t / 1e12
+        out["final_gather"] = {"bound": "valu", "achieved": round(tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(tf / FP32_VECTOR_PEAK_TFLOPS, 4),
+                               "lit_pairs": lit, "flops_per_lit_pair": FLOPS_PER_LIT_PAIR,
+                               "kernel_ms": round(stages["final_gather_ms"], 4)}
+    return out
 
 
 def cpu_baseline(wl, inp, rec, r16, threads: int):
@@ -111,7 +146,9 @@ def main() -> int:
     ap.add_argument("--form", default="both", choices=["fp32", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--check", action="store_true", help="compare the frame with the FP64 oracle")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before the frames, copy this many MiB device-to-device (a plain copy kernel of "
+                         "known size for scaling the HBM traffic counters)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -132,6 +169,14 @@ def main() -> int:
              "both": [mlgi.PRECISION_FP32, mlgi.PRECISION_MIXED]}[a.form]
     res = {}
     with mlgi.Context(0) as ctx:
+        if a.calibrate_copy_mib > 0:
+            n = a.calibrate_copy_mib << 20
+            src, dst = mlgi.DeviceArray(ctx, n, np.uint8), mlgi.DeviceArray(ctx, n, np.uint8)
+            dst.copy_from_device(src)
+            ctx.sync()
+            log(f"[bench] calibration copy of {a.calibrate_copy_mib} MiB done")
+            src.free()
+            dst.free()
         for prec in forms:
             name = "fp32" if prec == mlgi.PRECISION_FP32 else "mixed"
             log(f"[bench] form {name}: warmup {a.warmup}, steps {a.steps}")
@@ -143,9 +188,16 @@ def main() -> int:
             res[name] = r
     head = "mixed" if "mixed" in res else "fp32"
     h = res[head]
-    fg_ms = h["stages"]["final_gather_ms"]
-    flops = FLOPS_PER_LIT_PAIR * h["lit"]
-    achieved = flops / (fg_ms * 1e-3) / 1e12 if fg_ms > 0 else 0.0
+    kernels = kernel_rooflines(wl, h["stages"], h["lit"])
+    dominant = max(kernels, key=lambda k: kernels[k]["kernel_ms"])
+    roof = dict(kernels[dominant])
+    roof["kernel"] = dominant
+    roof["traffic"] = None
+    tpath = ROOT / "profiles" / f"traffic_{wl.name}.json"
+    if tpath.exists():
+        tr = json.loads(tpath.read_text()).get(dominant)
+        if tr:
+            roof["traffic"] = tr["bytes_per_launch"]
     out = {
         "metric": "GI ms/frame 1080p·4096 VPL (mixed vs FP32 HIP)",
         "value": round(h["wall_ms"], 4),
@@ -166,21 +218,9 @@ def main() -> int:
         "forms": {k: {"ms_per_frame": round(v["wall_ms"], 4),
                       "stages_ms": {s: round(x, 4) for s, x in v["stages"].items() if s != "frames"}}
                   for k, v in res.items()},
-        "roofline": {"bound": "valu", "kernel": "final_gather", "achieved": round(achieved, 3),
-                     "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
-                     "lit_pairs": h["lit"], "flops_per_lit_pair": FLOPS_PER_LIT_PAIR,
-                     "kernel_ms": round(fg_ms, 4)},
+        "roofline": roof,
+        "kernels": kernels,
     }
-    if a.check:
-        import oracle
-        for k, v in res.items():
-            rec = v["rec"]
-            p = wl.params()
-            ref, _ = oracle.final_gather(inp.gb_depth, inp.gb_normal, inp.vp_inv, rec[:, 0:3], rec[:, 4:7],
-                                         rec[:, 16:19], v["r16"], wl.atlas_side, p.z_far, p.gi_intensity_factor,
-                                         p.vpl_clamping_value, wl.interleave, threads=a.cpu_threads)
-            out["forms"][k]["rel_l2_vs_fp64"] = float(np.linalg.norm(v["img"] - ref) / np.linalg.norm(ref))
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, inp, h["rec"], h["r16"], a.cpu_threads)

@@ -76,7 +76,7 @@ typedef struct mlgi_params {
   int32_t pull_push_levels;       /* pull levels L (reference: 6) */
   int32_t precision;              /* mlgi_precision */
   int32_t tile_w, tile_h;         /* final-gather tile in interleave-class pixels; 0 = auto */
-  int32_t gather_fetch;           /* final-gather VPL fetch variant 0..2; -1 = auto */
+  int32_t gather_fetch;           /* final-gather VPL fetch variant 0..1; -1 = auto */
   int32_t splat_block;            /* splat workgroup size 64/128/256; 0 = auto */
 } mlgi_params;
 
@@ -172,6 +172,8 @@ int mlgi_malloc(mlgi_context* ctx, size_t bytes, void** dptr);
 int mlgi_free(mlgi_context* ctx, void* dptr);
 int mlgi_memcpy_htod(mlgi_context* ctx, void* dst, const void* src, size_t bytes);
 int mlgi_memcpy_dtoh(mlgi_context* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy, asynchronous on the context stream. */
+int mlgi_memcpy_dtod(mlgi_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* ----------------------------------------------------------------- stages */
 /* VPL extraction: count = n VPLs regularly sampled from the RSM. */
@@ -209,6 +211,111 @@ int mlgi_graph_capture_frame(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_
 /* Enqueue one replay on the context stream (asynchronous). */
 int mlgi_graph_launch(mlgi_context* ctx, mlgi_graph* graph);
 void mlgi_graph_destroy(mlgi_graph* graph);
+
+/* ---------------------------------------------------------------- batches
+ * No counterpart in the reference (it renders one view per frame): many
+ * independent GI frames (views or light poses, each with its own shapes and
+ * parameters) run as one set of persistent launches, one per stage (pull-push
+ * and gather once per ISM shape / precision group). Each frame's results are
+ * bit-identical to running it alone through the batch path. */
+typedef struct mlgi_frame_desc {
+  mlgi_rsm rsm;
+  mlgi_points points;
+  mlgi_gbuffer gbuffer;
+  mlgi_params params;   /* per frame; tile_w selects the gather strip width (8, 16, 32) */
+  mlgi_vpls vpls;       /* caller-owned work and output buffers */
+  mlgi_atlas atlas;
+  mlgi_ism_depth ism;
+  mlgi_image image;
+} mlgi_frame_desc;
+typedef struct mlgi_batch mlgi_batch;
+/* Validate every frame and upload the batch's frame table (allocates). */
+int mlgi_batch_create(mlgi_context* ctx, const mlgi_frame_desc* frames, int32_t count,
+                      mlgi_batch** out);
+/* Enqueue all frames (asynchronous). */
+int mlgi_batch_run(mlgi_context* ctx, mlgi_batch* batch);
+/* Capture one mlgi_batch_run into an instantiated HIP graph. */
+int mlgi_batch_capture(mlgi_context* ctx, mlgi_batch* batch, mlgi_graph** graph);
+void mlgi_batch_destroy(mlgi_batch* batch);
+
+/* ---------------------------------------------------------------- autotuner
+ * No counterpart in the reference. Picks, per frame shape (G-buffer size,
+ * VPL count, precision, shadowing), the interleave and the gather tile:
+ *   interleave: the largest of {8, 4} that leaves at least
+ *     min_vpls_per_pixel VPLs per interleave class (else 4); it decides
+ *     which VPLs a pixel sums, i.e. the image, so it follows a quality rule;
+ *   tile: the fastest gather strip width of {8, 16, 32} (workgroup tiles
+ *     8x8, 16x4 / 16x16, 32x8), timed on the frame itself through the batch
+ *     path; the image does not depend on it.
+ * Writes the choice into frame->params (interleave, use_interleaving,
+ * tile_w, tile_h). Results are cached per shape in the context; a cached
+ * shape is not timed again. Runs the frame (its buffers are overwritten). */
+typedef struct mlgi_tune_result {
+  int32_t interleave;
+  int32_t tile_w, tile_h;
+  float gather_ms[3]; /* timed strip widths 8, 16, 32 (0 when cached) */
+  int32_t cached;
+} mlgi_tune_result;
+int mlgi_autotune(mlgi_context* ctx, mlgi_frame_desc* frame, int32_t min_vpls_per_pixel,
+                  mlgi_tune_result* out);
+
+/* ---------------------------------------------------------------- serving
+ * No counterpart in the reference (an interactive viewer). A GI request
+ * server over resident scenes: each scene's BVH, point clouds (one CSR
+ * binning per VPL count) and per-resolution G-buffers stay on the device;
+ * requests (scene, resolution, VPL count, light pose) arrive over time, are
+ * admitted as in-flight slots and a fixed HBM budget free up (their work
+ * buffers come from one arena allocated up front), and are batched
+ * continuously: every admission round packs all admissible waiting requests
+ * into one batch that is enqueued behind the batches already on the device.
+ * The light pose's RSM is rendered on the device per request. A request's
+ * result depends only on the request (the same bits alone or in any mix). */
+typedef struct mlgi_server mlgi_server;
+typedef struct mlgi_server_config {
+  size_t hbm_budget;          /* arena bytes the admitted requests' buffers come from */
+  int32_t max_inflight;       /* requests on the device at once */
+  int32_t max_batch;          /* requests per batch */
+  int32_t max_queued_batches; /* batches enqueued on the device at once */
+  int32_t rsm_size;           /* RSM side rendered per request */
+  int32_t precision;          /* mlgi_precision */
+  int32_t min_vpls_per_pixel; /* autotuner interleave rule */
+  const int32_t* vpl_counts;  /* VPL counts requests may use (point binnings) */
+  int32_t n_vpl_counts;
+  const int32_t* resolutions; /* (width, height) pairs requests may use */
+  int32_t n_resolutions;
+} mlgi_server_config;
+typedef struct mlgi_server_scene {
+  const struct mlgi_scene* scene; /* host scene (mlgi_scene.h) */
+  int64_t n_points;               /* splat points of the scene */
+  uint64_t point_seed;
+} mlgi_server_scene;
+typedef struct mlgi_request {
+  int32_t scene;          /* index into the server's scenes */
+  int32_t width, height;  /* one of the configured resolutions (the scene's fixed camera) */
+  int32_t n_vpl;          /* one of the configured VPL counts */
+  uint64_t light_seed;    /* spot light pose (mlgi_scene_light_pose) */
+  double arrival_s;       /* arrival, seconds after mlgi_server_run starts */
+  float* host_out;        /* optional: irradiance (h*w*3) copied here on completion */
+} mlgi_request;
+typedef struct mlgi_server_stats {
+  int64_t completed, rejected;
+  double wall_s, req_per_s;
+  double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;
+  int32_t batches, max_inflight_seen;
+  double mean_batch;
+  size_t peak_bytes;      /* peak arena use */
+} mlgi_server_stats;
+/* Uploads the scenes, builds their point binnings and G-buffers, tunes every
+ * (resolution, VPL count) shape. */
+int mlgi_server_create(mlgi_context* ctx, const mlgi_server_config* cfg,
+                       const mlgi_server_scene* scenes, int32_t n_scenes, mlgi_server** out);
+/* Serves the requests (sorted by arrival) in real time and returns when all
+ * are completed or rejected (never fit the budget). latencies_ms (optional,
+ * n entries) receives each request's arrival-to-completion time (-1 if
+ * rejected). */
+int mlgi_server_run(mlgi_server* srv, const mlgi_request* reqs, int32_t n, mlgi_server_stats* stats,
+                    double* latencies_ms);
+void mlgi_server_destroy(mlgi_server* srv);
 
 /* Number of (pixel, VPL-of-its-class) pairs with a positive geometry factor
  * (the lit pairs the gather's algorithmic work is counted on). Synchronous. */

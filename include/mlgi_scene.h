@@ -14,13 +14,16 @@
  *                              the reference's Jenkins-hash random()
  *
  * All outputs are host arrays; the caller uploads them. Matrices are
- * column-major float[16] (GL convention), computed in double.
+ * column-major float[16] (GL convention), computed in double. The same G-buffer
+ * and RSM can also be produced on the device (mlgi_scene_upload below).
  */
 #ifndef MLGI_SCENE_H
 #define MLGI_SCENE_H
 
 #include <stddef.h>
 #include <stdint.h>
+
+#include "mlgi.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -81,6 +84,26 @@ int mlgi_scene_render_rsm(const mlgi_scene* s, const mlgi_camera* light, int32_t
 int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, int32_t n_bins,
                              int32_t ism_count, uint64_t seed, int32_t threads, float* records,
                              uint32_t* bin_offsets);
+
+/* ---------------------------------------------------- device producer
+ * The scene's BVH on the device, for inputs that change per request or per
+ * frame (light poses, views). The kernels run the host renderer's double
+ * precision code, so their texels equal mlgi_scene_render_gbuffer /
+ * mlgi_scene_render_rsm bit for bit. Output pointers are device buffers;
+ * the launches are asynchronous on the context stream. */
+typedef struct mlgi_scene_device mlgi_scene_device;
+int mlgi_scene_upload(mlgi_context* ctx, const mlgi_scene* s, mlgi_scene_device** out);
+void mlgi_scene_device_destroy(mlgi_scene_device* d);
+/* depth (w*h), face_normal (w*h*3); vp_inv16 (host, optional) receives
+ * inverse(P*V) as mlgi_camera_matrices computes it. */
+int mlgi_scene_device_render_gbuffer(mlgi_context* ctx, const mlgi_scene_device* d,
+                                     const mlgi_camera* cam, int32_t width, int32_t height,
+                                     float* depth, float* face_normal, float* vp_inv16);
+/* diffuse (w*h*3), face_normal (w*h*3), depth (w*h); inv_bvp16 (host,
+ * optional) receives inverse(bias * P * V). */
+int mlgi_scene_device_render_rsm(mlgi_context* ctx, const mlgi_scene_device* d,
+                                 const mlgi_camera* light, int32_t width, int32_t height,
+                                 float* diffuse, float* face_normal, float* depth, float* inv_bvp16);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,9 @@ SOURCES = [
     ("ism_splat.hip", HIP_FLAGS + EXACT),
     ("pullpush.hip", HIP_FLAGS + EXACT),
     ("final_gather.hip", HIP_FLAGS),
+    ("raycast.hip", HIP_FLAGS + EXACT),
     ("capi.cpp", HIP_FLAGS),
+    ("server.cpp", HIP_FLAGS),
     ("scene.cpp", ["-ffp-contract=off"]),
 ]
 

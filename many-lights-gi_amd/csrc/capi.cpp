@@ -17,28 +17,14 @@
 #include <vector>
 
 #include "../../include/mlgi.h"
+#include "../../include/mlgi_scene.h"
+#include "capi_internal.hpp"
 #include "kernels.hpp"
+#include "scene_trace.hpp"
 
-struct mlgi_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string error;
-  std::map<int, int32_t*> shuffle;  // VPL count -> device permutation
-  unsigned long long* counter = nullptr;
-  // profiling (PerfCounter analogue): event pairs per stage
-  bool profile = false;
-  struct Span {
-    int stage;
-    hipEvent_t a, b;
-  };
-  std::vector<Span> spans;
-  std::vector<hipEvent_t> pool;
-  int frames = 0;
-};
+using namespace mlgi_capi;
 
-namespace {
-
-enum Stage { kVpl = 0, kClear, kSplat, kPullPush, kGather, kFrame, kStages };
+namespace mlgi_capi {
 
 int fail(mlgi_context* c, int code, const char* fmt, ...) {
   if (c) {
@@ -76,28 +62,6 @@ hipEvent_t take_event(mlgi_context* c) {
   return e;
 }
 
-struct Scope {  // records a timed span on the context stream when profiling
-  mlgi_context* c;
-  int stage;
-  hipEvent_t a = nullptr;
-  Scope(mlgi_context* ctx, int st) : c(ctx), stage(st) {
-    if (c->profile) {
-      a = take_event(c);
-      if (a) (void)hipEventRecord(a, c->stream);
-    }
-  }
-  ~Scope() {
-    if (c->profile && a) {
-      hipEvent_t b = take_event(c);
-      if (b) {
-        (void)hipEventRecord(b, c->stream);
-        c->spans.push_back({stage, a, b});
-      } else {
-        c->pool.push_back(a);
-      }
-    }
-  }
-};
 
 mlgi::Mat4 to_mat(const float* m) {
   mlgi::Mat4 r;
@@ -159,7 +123,7 @@ int atlas_geometry(mlgi_context* c, int side, int n_vpl, int* n1d, int* ism_side
   return MLGI_OK;
 }
 
-}  // namespace
+}  // namespace mlgi_capi
 
 extern "C" {
 
@@ -276,6 +240,13 @@ int mlgi_memcpy_dtoh(mlgi_context* c, void* dst, const void* src, size_t bytes) 
   return hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 }
 
+int mlgi_memcpy_dtod(mlgi_context* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!bytes) return MLGI_OK;
+  return hip_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream),
+                   "hipMemcpyAsync(DtoD)");
+}
+
 int mlgi_shuffle_indices(int32_t n, uint32_t seed, int32_t* out) {
   if (n < 0 || (!out && n)) return MLGI_ERR_INVALID_ARGUMENT;
   std::vector<int> v(n);
@@ -382,7 +353,9 @@ int mlgi_ism_process(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* v
   return mlgi_ism_pullpush(c, atlas, p, out);
 }
 
-namespace {
+}  // extern "C"
+
+namespace mlgi_capi {
 
 int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
                       const mlgi_ism_depth* ism, const mlgi_params* p, mlgi::GatherArgs* a) {
@@ -446,7 +419,9 @@ void pick_tile(const mlgi_params* p, int* tw, int* th) {
   *th = p->tile_h > 0 ? p->tile_h : 8;
 }
 
-}  // namespace
+}  // namespace mlgi_capi
+
+extern "C" {
 
 int mlgi_final_gather(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
                       const mlgi_ism_depth* ism, const mlgi_params* p, mlgi_image* out) {
@@ -463,7 +438,7 @@ int mlgi_final_gather(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
   if (!((tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 8) || (tw == 16 && th == 4)))
     return fail(c, MLGI_ERR_UNSUPPORTED, "tile %dx%d not in {8x8, 16x16, 32x8, 16x4}", tw, th);
   const int fetch = p->gather_fetch >= 0 ? p->gather_fetch : 0;
-  if (fetch > 2) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "gather_fetch %d not in 0..2", fetch);
+  if (fetch > 1) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "gather_fetch %d not in 0..1", fetch);
   a.out = out->rgb;
   Scope sc(c, kGather);
   return hip_check(c, mlgi::launch_final_gather(c->stream, a, p->precision, tw, th, fetch), "final_gather");
@@ -613,6 +588,427 @@ int mlgi_profile_read(mlgi_context* c, mlgi_stage_times* out) {
   out->frame_ms = avg(kFrame);
   out->frames = c->frames;
   return MLGI_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+struct mlgi_batch {
+  mlgi_context* ctx = nullptr;
+  mlgi::BatchFrame* dev = nullptr;  // device copy of the frame table
+  mlgi_capi::BatchPlan plan;
+};
+
+namespace mlgi_capi {
+
+// Validates one frame of a batch and fills its table row (first_* unset).
+int batch_frame(mlgi_context* c, const mlgi_frame_desc& d, mlgi::BatchFrame* f) {
+  const mlgi_params* p = &d.params;
+  int rc = check_params(c, p);
+  if (rc || (rc = check_vpls(c, &d.vpls))) return rc;
+  const mlgi_rsm& rsm = d.rsm;
+  if (!rsm.diffuse || !rsm.face_normal || !rsm.depth)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "rsm buffers are NULL");
+  if (rsm.width <= 0 || rsm.height <= 0 || rsm.width > 16384 || rsm.height > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "rsm size %dx%d out of range", rsm.width, rsm.height);
+  const mlgi_points& pts = d.points;
+  if (pts.count < 0 || (pts.count > 0 && (!pts.records || !pts.bin_offsets)))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "points buffers are NULL");
+  if (pts.count > 0 && pts.n_bins != d.vpls.count)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "point bins (%d) must equal the VPL count (%d)", pts.n_bins,
+                d.vpls.count);
+  if (!d.atlas.texels || !d.ism.texels) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas / ism is NULL");
+  int n1d, s;
+  if ((rc = atlas_geometry(c, d.atlas.side, d.vpls.count, &n1d, &s))) return rc;
+  if ((d.atlas.ism_side != 0 && d.atlas.ism_side != s) || d.ism.side != d.atlas.side ||
+      (d.ism.ism_side != 0 && d.ism.ism_side != s))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas / ism geometry does not match %d VPLs in a %d atlas",
+                d.vpls.count, d.atlas.side);
+  if (s != 8 && s != 16 && s != 32 && s != 64)
+    return fail(c, MLGI_ERR_UNSUPPORTED, "ISM side %d not in {8,16,32,64}", s);
+  if (p->pull_push_levels < 1 || p->pull_push_levels > log2i(s))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "pull-push levels %d out of range 1..%d", p->pull_push_levels,
+                log2i(s));
+  mlgi_ism_depth ism = d.ism;
+  ism.ism_side = s;
+  mlgi::GatherArgs a;
+  if ((rc = build_gather_args(c, &d.gbuffer, &d.vpls, &ism, p, &a))) return rc;
+  if (!d.image.rgb || d.image.width != d.gbuffer.width || d.image.height != d.gbuffer.height)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "output image must match the gbuffer size");
+  if (p->precision != MLGI_PRECISION_FP32 && p->precision != MLGI_PRECISION_MIXED)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "unknown precision %d", p->precision);
+  const int tw = p->tile_w > 0 ? p->tile_w : 8;
+  if (tw != 8 && tw != 16 && tw != 32)
+    return fail(c, MLGI_ERR_UNSUPPORTED, "batched gather strip width %d not in {8, 16, 32}", tw);
+  const int32_t* sh = nullptr;
+  if (p->shuffle_lights && (rc = get_shuffle(c, d.vpls.count, &sh))) return rc;
+
+  std::memset(f, 0, sizeof *f);
+  f->rsm_diffuse = rsm.diffuse;
+  f->rsm_normal = rsm.face_normal;
+  f->rsm_depth = rsm.depth;
+  f->rsm_w = rsm.width;
+  f->rsm_h = rsm.height;
+  f->inv_bvp = to_mat(rsm.inv_biased_view_projection);
+  f->light_intensity = p->light_intensity;
+  f->shuffle = sh;
+  uint32_t s1d = 0;
+  while ((uint64_t)s1d * s1d < (uint64_t)d.vpls.count) ++s1d;  // uint(ceil(sqrt(totalVplCount)))
+  f->axis_x = s1d;
+  f->axis_y = (uint32_t)d.vpls.count / s1d;
+  f->points = pts.count > 0 ? pts.records : nullptr;
+  f->bin_offsets = pts.count > 0 ? pts.bin_offsets : nullptr;
+  f->n_points = (uint32_t)pts.count;
+  f->atlas = d.atlas.texels;
+  f->atlas_side = d.atlas.side;
+  f->ism_side = s;
+  f->n1d = n1d;
+  f->levels = p->pull_push_levels;
+  f->z_far = p->z_far;
+  mlgi::pullpush_empty_texel(s, p->z_far, f->pp_e1);
+  a.out = d.image.rgb;
+  a.ism = d.ism.texels;  // the pull-push writes it even when the gather does not shadow
+  f->g = a;
+  f->wave_w_log2 = log2i(tw);
+  const int cw = (a.width + a.interleave - 1) / a.interleave;
+  f->strips_x = (cw + tw - 1) / tw;
+  return MLGI_OK;
+}
+
+int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::vector<mlgi::BatchFrame>* table,
+               BatchPlan* plan) {
+  if (!frames || count <= 0 || count > (1 << 16))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "batch of %d frames", count);
+  std::vector<mlgi::BatchFrame> rows(count);
+  std::vector<int> key(count);
+  for (int i = 0; i < count; ++i) {
+    int rc = batch_frame(c, frames[i], &rows[i]);
+    if (rc) {
+      std::string m = c->error;
+      return fail(c, rc, "frame %d: %s", i, m.c_str());
+    }
+    key[i] = (frames[i].params.precision << 16) | (rows[i].ism_side << 8) | rows[i].levels;
+  }
+  // group by (precision, ISM side, levels): contiguous runs per launch
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+  *plan = BatchPlan{};
+  plan->count = count;
+  table->assign(count, mlgi::BatchFrame{});
+  uint64_t tv = 0, tc = 0, tb = 0;
+  for (int r = 0; r < count; ++r) {
+    mlgi::BatchFrame f = rows[order[r]];
+    const mlgi::BatchCounts n = mlgi::batch_counts(f);
+    f.first_vpl = (uint32_t)tv;
+    f.first_clear = (uint32_t)tc;
+    f.first_bin = (uint32_t)tb;
+    tv += n.vpl;
+    tc += n.clear;
+    tb += n.bins;
+    const int k = key[order[r]];
+    const int prec = k >> 16;
+    if (r == 0 || key[order[r - 1]] != k) plan->pullpush.push_back({r, 0, 0, f.ism_side, f.levels});
+    auto& pp = plan->pullpush.back();
+    f.first_ism = pp.total;
+    pp.total += n.isms;
+    pp.count++;
+    if (r == 0 || (key[order[r - 1]] >> 16) != prec) plan->gather.push_back({r, 0, 0, prec, 0});
+    auto& fg = plan->gather.back();
+    f.first_wave = fg.total;
+    fg.total += n.waves;
+    fg.count++;
+    (*table)[r] = f;
+  }
+  if (tv > 0xFFFFFFFFull || tc > 0xFFFFFFFFull || tb > 0xFFFFFFFFull)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "batch too large");
+  plan->total_vpl = (uint32_t)tv;
+  plan->total_clear = (uint32_t)tc;
+  plan->total_bins = (uint32_t)tb;
+  return MLGI_OK;
+}
+
+int batch_enqueue(mlgi_context* c, const mlgi::BatchFrame* dev, const BatchPlan& plan) {
+  Scope frame(c, kFrame);
+  int rc;
+  {
+    Scope sc(c, kVpl);
+    if ((rc = hip_check(c, mlgi::launch_batch_vpl(c->stream, dev, plan.count, plan.total_vpl), "batch_vpl")))
+      return rc;
+  }
+  {
+    Scope sc(c, kClear);
+    if ((rc = hip_check(c, mlgi::launch_batch_clear(c->stream, dev, plan.count, plan.total_clear), "batch_clear")))
+      return rc;
+  }
+  {
+    Scope sc(c, kSplat);
+    if ((rc = hip_check(c, mlgi::launch_batch_splat(c->stream, dev, plan.count, plan.total_bins), "batch_splat")))
+      return rc;
+  }
+  {
+    Scope sc(c, kPullPush);
+    for (const auto& r : plan.pullpush)
+      if ((rc = hip_check(c, mlgi::launch_batch_pullpush(c->stream, dev + r.begin, r.count, r.total, r.a, r.b),
+                          "batch_pullpush")))
+        return rc;
+  }
+  {
+    Scope sc(c, kGather);
+    for (const auto& r : plan.gather)
+      if ((rc = hip_check(c, mlgi::launch_batch_gather(c->stream, dev + r.begin, r.count, r.total, r.a),
+                          "batch_gather")))
+        return rc;
+  }
+  if (c->profile) c->frames++;
+  return MLGI_OK;
+}
+
+}  // namespace mlgi_capi
+
+mlgi::BatchCounts mlgi::batch_counts(const mlgi::BatchFrame& f) {
+  BatchCounts n;
+  n.vpl = (uint32_t)f.g.n_vpl;
+  n.clear = batch_clear_chunks(f.atlas_side);
+  n.bins = f.n_points > 0 ? (uint32_t)f.g.n_vpl : 0u;
+  n.isms = (uint32_t)(f.n1d * f.n1d);
+  const int th = 64 >> f.wave_w_log2;
+  const int ch = (f.g.height + f.g.interleave - 1) / f.g.interleave;
+  const uint32_t strips_y = (uint32_t)((ch + th - 1) / th);
+  n.waves = (uint32_t)f.strips_x * strips_y * (uint32_t)(f.g.interleave * f.g.interleave);
+  return n;
+}
+
+extern "C" {
+
+int mlgi_batch_create(mlgi_context* c, const mlgi_frame_desc* frames, int32_t count, mlgi_batch** out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!out) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  std::vector<mlgi::BatchFrame> table;
+  auto* b = new mlgi_batch();
+  b->ctx = c;
+  int rc = batch_plan(c, frames, count, &table, &b->plan);
+  if (!rc) rc = hip_check(c, hipMalloc(&b->dev, sizeof(mlgi::BatchFrame) * count), "hipMalloc(batch)");
+  if (!rc)
+    rc = hip_check(c, hipMemcpy(b->dev, table.data(), sizeof(mlgi::BatchFrame) * count, hipMemcpyHostToDevice),
+                   "hipMemcpy(batch)");
+  if (rc) {
+    mlgi_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return MLGI_OK;
+}
+
+int mlgi_batch_run(mlgi_context* c, mlgi_batch* b) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!b || !b->dev || b->ctx != c) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "batch is NULL or foreign");
+  return batch_enqueue(c, b->dev, b->plan);
+}
+
+int mlgi_batch_capture(mlgi_context* c, mlgi_batch* b, mlgi_graph** graph) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!graph) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "graph is NULL");
+  *graph = nullptr;
+  if (!b || !b->dev || b->ctx != c) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "batch is NULL or foreign");
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (rc) return rc;
+  const bool prof = c->profile;
+  c->profile = false;  // no event nodes in the graph
+  rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (rc) {
+    c->profile = prof;
+    return rc;
+  }
+  const int run_rc = mlgi_batch_run(c, b);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(c->stream, &g);
+  c->profile = prof;
+  if (run_rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return run_rc;
+  }
+  if ((rc = hip_check(c, end, "hipStreamEndCapture"))) return rc;
+  auto* og = new mlgi_graph();
+  og->graph = g;
+  if ((rc = hip_check(c, hipGraphInstantiate(&og->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate"))) {
+    (void)hipGraphDestroy(g);
+    delete og;
+    return rc;
+  }
+  *graph = og;
+  return MLGI_OK;
+}
+
+void mlgi_batch_destroy(mlgi_batch* b) {
+  if (!b) return;
+  if (b->dev) {
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(b->dev);
+  }
+  delete b;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- autotuner
+
+extern "C" int mlgi_autotune(mlgi_context* c, mlgi_frame_desc* frame, int32_t min_vpls_per_pixel,
+                             mlgi_tune_result* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!frame || !out) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "frame / out is NULL");
+  std::memset(out, 0, sizeof *out);
+  mlgi_params& p = frame->params;
+  const int n = frame->vpls.count;
+  const int min_vpp = min_vpls_per_pixel > 0 ? min_vpls_per_pixel : 32;
+  int il = 4;
+  if (n % 64 == 0 && n / 64 >= min_vpp) il = 8;
+  p.interleave = il;
+  p.use_interleaving = 1;
+  const std::vector<int> key = {frame->gbuffer.width, frame->gbuffer.height, n, p.precision,
+                                p.enable_shadowing, il};
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) {
+    p.tile_w = it->second[0];
+    p.tile_h = it->second[1];
+    out->interleave = il;
+    out->tile_w = p.tile_w;
+    out->tile_h = p.tile_h;
+    out->cached = 1;
+    return MLGI_OK;
+  }
+  static const int kStrip[3] = {8, 16, 32}, kTileH[3] = {8, 4, 8};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = hip_check(c, hipEventCreate(&e0), "hipEventCreate");
+  if (!rc) rc = hip_check(c, hipEventCreate(&e1), "hipEventCreate");
+  int best = -1;
+  float best_ms = 0.0f;
+  const bool prof = c->profile;
+  c->profile = false;
+  for (int k = 0; k < 3 && !rc; ++k) {
+    mlgi_frame_desc d = *frame;
+    d.params.tile_w = kStrip[k];
+    d.params.tile_h = kTileH[k];
+    mlgi_batch* b = nullptr;
+    if ((rc = mlgi_batch_create(c, &d, 1, &b))) break;
+    float ms = 0.0f;
+    // one untimed run (fills VPLs and ISM), then the best of three
+    rc = mlgi_batch_run(c, b);
+    for (int rep = 0; rep < 3 && !rc; ++rep) {
+      if ((rc = hip_check(c, hipEventRecord(e0, c->stream), "hipEventRecord"))) break;
+      if ((rc = hip_check(c, mlgi::launch_batch_gather(c->stream, b->dev + b->plan.gather[0].begin, b->plan.gather[0].count,
+                                                       b->plan.gather[0].total, b->plan.gather[0].a),
+                          "batch_gather")))
+        break;
+      if ((rc = hip_check(c, hipEventRecord(e1, c->stream), "hipEventRecord"))) break;
+      if ((rc = hip_check(c, hipEventSynchronize(e1), "hipEventSynchronize"))) break;
+      float t = 0.0f;
+      if ((rc = hip_check(c, hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))) break;
+      ms = rep == 0 ? t : std::min(ms, t);
+    }
+    mlgi_batch_destroy(b);
+    if (rc) break;
+    out->gather_ms[k] = ms;
+    if (best < 0 || ms < best_ms) {
+      best = k;
+      best_ms = ms;
+    }
+  }
+  c->profile = prof;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  p.tile_w = kStrip[best];
+  p.tile_h = kTileH[best];
+  c->tuned[key] = {p.tile_w, p.tile_h};
+  out->interleave = il;
+  out->tile_w = p.tile_w;
+  out->tile_h = p.tile_h;
+  return MLGI_OK;
+}
+
+// ------------------------------------------------------- device producer
+
+struct mlgi_scene_device {
+  mlgi_context* ctx = nullptr;
+  mlgi_scene_impl::Node* nodes = nullptr;
+  mlgi_scene_impl::Tri* tris = nullptr;
+  int* order = nullptr;
+  int n_nodes = 0, n_tris = 0;
+  mlgi_scene_impl::SceneView view() const {
+    return mlgi_scene_impl::SceneView{nodes, tris, order, n_nodes, n_tris};
+  }
+};
+
+extern "C" {
+
+void mlgi_scene_device_destroy(mlgi_scene_device* d) {
+  if (!d) return;
+  if (d->ctx) (void)hipStreamSynchronize(d->ctx->stream);
+  (void)hipFree(d->nodes);
+  (void)hipFree(d->tris);
+  (void)hipFree(d->order);
+  delete d;
+}
+
+int mlgi_scene_upload(mlgi_context* c, const mlgi_scene* s, mlgi_scene_device** out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!s || !out) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "scene / out is NULL");
+  *out = nullptr;
+  const mlgi_scene_impl::SceneView v = mlgi_scene_view(s);
+  if (v.n_nodes <= 0 || v.n_tris <= 0) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "empty scene");
+  auto* d = new mlgi_scene_device();
+  d->ctx = c;
+  d->n_nodes = v.n_nodes;
+  d->n_tris = v.n_tris;
+  int rc = hip_check(c, hipMalloc(&d->nodes, sizeof(mlgi_scene_impl::Node) * v.n_nodes), "hipMalloc(bvh)");
+  if (!rc) rc = hip_check(c, hipMalloc(&d->tris, sizeof(mlgi_scene_impl::Tri) * v.n_tris), "hipMalloc(tris)");
+  if (!rc) rc = hip_check(c, hipMalloc(&d->order, sizeof(int) * v.n_tris), "hipMalloc(order)");
+  if (!rc)
+    rc = hip_check(c, hipMemcpy(d->nodes, v.nodes, sizeof(mlgi_scene_impl::Node) * v.n_nodes, hipMemcpyHostToDevice),
+                   "hipMemcpy(bvh)");
+  if (!rc)
+    rc = hip_check(c, hipMemcpy(d->tris, v.tris, sizeof(mlgi_scene_impl::Tri) * v.n_tris, hipMemcpyHostToDevice),
+                   "hipMemcpy(tris)");
+  if (!rc)
+    rc = hip_check(c, hipMemcpy(d->order, v.order, sizeof(int) * v.n_tris, hipMemcpyHostToDevice), "hipMemcpy(order)");
+  if (rc) {
+    mlgi_scene_device_destroy(d);
+    return rc;
+  }
+  *out = d;
+  return MLGI_OK;
+}
+
+int mlgi_scene_device_render_gbuffer(mlgi_context* c, const mlgi_scene_device* d, const mlgi_camera* cam,
+                                     int32_t w, int32_t h, float* depth, float* normal, float* vp_inv16) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!d || !cam || !depth || !normal) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (w <= 0 || h <= 0 || w > 16384 || h > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "size %dx%d out of range", w, h);
+  mlgi_scene_impl::RayCam rc;
+  if (!mlgi_scene_ray_cam(cam, &rc)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "bad camera");
+  if (vp_inv16 && mlgi_camera_matrices(cam, nullptr, nullptr, vp_inv16))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "singular camera");
+  return hip_check(c, mlgi::launch_raycast(c->stream, d->view(), rc, w, h, 0, depth, normal, nullptr),
+                   "raycast(gbuffer)");
+}
+
+int mlgi_scene_device_render_rsm(mlgi_context* c, const mlgi_scene_device* d, const mlgi_camera* light,
+                                 int32_t w, int32_t h, float* diffuse, float* normal, float* depth,
+                                 float* inv_bvp16) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!d || !light || !diffuse || !normal || !depth) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (w <= 0 || h <= 0 || w > 16384 || h > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "size %dx%d out of range", w, h);
+  mlgi_scene_impl::RayCam rc;
+  if (!mlgi_scene_ray_cam(light, &rc)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "bad light");
+  if (inv_bvp16 && !mlgi_scene_inv_biased_vp(light, inv_bvp16))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "singular light");
+  return hip_check(c, mlgi::launch_raycast(c->stream, d->view(), rc, w, h, 1, diffuse, normal, depth),
+                   "raycast(rsm)");
 }
 
 }  // extern "C"

@@ -1,0 +1,98 @@
+// Internals of the C ABI shared by its translation units (capi.cpp, server.cpp):
+// the context, error reporting, stage timing, descriptor validation and the
+// batch planner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mlgi.h"
+#include "kernels.hpp"
+
+struct mlgi_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  std::map<int, int32_t*> shuffle;  // VPL count -> device permutation
+  unsigned long long* counter = nullptr;
+  // profiling (PerfCounter analogue): event pairs per stage
+  bool profile = false;
+  struct Span {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Span> spans;
+  std::vector<hipEvent_t> pool;
+  int frames = 0;
+  // autotuner cache: shape key -> (tile_w, tile_h)
+  std::map<std::vector<int>, std::vector<int>> tuned;
+};
+
+namespace mlgi_capi {
+
+enum Stage { kVpl = 0, kClear, kSplat, kPullPush, kGather, kFrame, kStages };
+
+int fail(mlgi_context* c, int code, const char* fmt, ...);
+int hip_check(mlgi_context* c, hipError_t e, const char* what);
+bool is_pow2(int v);
+int log2i(int v);
+hipEvent_t take_event(mlgi_context* c);
+mlgi::Mat4 to_mat(const float* m);
+int get_shuffle(mlgi_context* c, int n, const int32_t** out);
+int check_params(mlgi_context* c, const mlgi_params* p);
+int interleave_of(const mlgi_params* p);
+int check_vpls(mlgi_context* c, const mlgi_vpls* v);
+int atlas_geometry(mlgi_context* c, int side, int n_vpl, int* n1d, int* ism_side);
+int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
+                      const mlgi_ism_depth* ism, const mlgi_params* p, mlgi::GatherArgs* a);
+void pick_tile(const mlgi_params* p, int* tw, int* th);
+
+struct Scope {  // records a timed span on the context stream when profiling
+  mlgi_context* c;
+  int stage;
+  hipEvent_t a = nullptr;
+  Scope(mlgi_context* ctx, int st) : c(ctx), stage(st) {
+    if (c->profile) {
+      a = take_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~Scope() {
+    if (c->profile && a) {
+      hipEvent_t b = take_event(c);
+      if (b) {
+        (void)hipEventRecord(b, c->stream);
+        c->spans.push_back({stage, a, b});
+      } else {
+        c->pool.push_back(a);
+      }
+    }
+  }
+};
+
+// A batch's launch plan: per-stage item totals and the contiguous runs of
+// frames that share one pull-push kernel (ISM side a, levels b) or one gather
+// kernel (precision a).
+struct BatchPlan {
+  int count = 0;
+  uint32_t total_vpl = 0, total_clear = 0, total_bins = 0;
+  struct Run {
+    int begin, count;
+    uint32_t total;
+    int a, b;
+  };
+  std::vector<Run> pullpush, gather;
+};
+
+// Validates one frame and fills its table row (first_* unset).
+int batch_frame(mlgi_context* c, const mlgi_frame_desc& d, mlgi::BatchFrame* f);
+// Validates and orders frames into a device-ready table and its plan.
+int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::vector<mlgi::BatchFrame>* table,
+               BatchPlan* plan);
+// Enqueues a planned batch whose table is at `dev` (device memory).
+int batch_enqueue(mlgi_context* c, const mlgi::BatchFrame* dev, const BatchPlan& plan);
+
+}  // namespace mlgi_capi

@@ -32,6 +32,7 @@
 // (round robin), so the linear work index is remapped such that the I^2
 // classes of one screen tile, which read the same G-buffer cache lines, land
 // on one XCD and its L2.
+#include "batch_device.hpp"
 #include "kernels.hpp"
 
 namespace mlgi {
@@ -140,6 +141,31 @@ struct WaveBounds {
   float scale;                         // coordinate magnitude for the margins
 };
 
+// Bounds of the wave's valid pixels (positions, normals, plane offsets);
+// false when the wave has no pixel on screen. Wave-uniform.
+__device__ __forceinline__ bool wave_bounds(const Pixel& p, WaveBounds& wb) {
+  const float inf = __builtin_inff();
+  const float mnx = wave_min(p.valid ? p.fx : inf), mxx = wave_max(p.valid ? p.fx : -inf);
+  const float mny = wave_min(p.valid ? p.fy : inf), mxy = wave_max(p.valid ? p.fy : -inf);
+  const float mnz = wave_min(p.valid ? p.fz : inf), mxz = wave_max(p.valid ? p.fz : -inf);
+  wb.cx = 0.5f * (mnx + mxx);
+  wb.cy = 0.5f * (mny + mxy);
+  wb.cz = 0.5f * (mnz + mxz);
+  wb.hx = 0.5f * (mxx - mnx) * 1.0001f;
+  wb.hy = 0.5f * (mxy - mny) * 1.0001f;
+  wb.hz = 0.5f * (mxz - mnz) * 1.0001f;
+  wb.nlx = wave_min(p.valid ? p.nx : inf);
+  wb.nhx = wave_max(p.valid ? p.nx : -inf);
+  wb.nly = wave_min(p.valid ? p.ny : inf);
+  wb.nhy = wave_max(p.valid ? p.ny : -inf);
+  wb.nlz = wave_min(p.valid ? p.nz : inf);
+  wb.nhz = wave_max(p.valid ? p.nz : -inf);
+  wb.plane_lo = wave_min(p.valid ? p.plane : inf);
+  wb.scale = 4.0f * fmaxf(fmaxf(fmaxf(fabsf(mnx), fabsf(mxx)), fmaxf(fabsf(mny), fabsf(mxy))),
+                          fmaxf(fabsf(mnz), fabsf(mxz)));
+  return mnx <= mxx;
+}
+
 // Survivor mask of the `count` staged VPLs of the current chunk.
 __device__ __forceinline__ unsigned long long cull_chunk(const VplLds* tile, int count,
                                                          const WaveBounds& w) {
@@ -161,25 +187,6 @@ __device__ __forceinline__ unsigned long long cull_chunk(const VplLds* tile, int
     const float tol2 = 1e-6f * (w.scale + fabsf(v.px) + fabsf(v.py) + fabsf(v.pz));
     const bool behind_pixels = w.plane_lo - mx > tol2;
     keep = !(behind_vpl || behind_pixels);
-  }
-  return __ballot(keep);
-}
-
-// Same test on a record held by the lane itself.
-__device__ __forceinline__ unsigned long long cull_one(const VplLds& v, bool present,
-                                                       const WaveBounds& w) {
-  bool keep = false;
-  if (present) {
-    const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
-    const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
-    const float spread = ax * w.hx + ay * w.hy + az * w.hz;
-    const float support = v.nx * dx + v.ny * dy + v.nz * dz + spread;
-    const float mag = ax * fabsf(dx) + ay * fabsf(dy) + az * fabsf(dz) + spread;
-    const float tol = 1e-4f * mag + 1e-6f * (w.scale + fabsf(v.plane));
-    const float mx = fmaxf(w.nlx * v.px, w.nhx * v.px) + fmaxf(w.nly * v.py, w.nhy * v.py) +
-                     fmaxf(w.nlz * v.pz, w.nhz * v.pz);
-    const float tol2 = 1e-6f * (w.scale + fabsf(v.px) + fabsf(v.py) + fabsf(v.pz));
-    keep = !(support < -tol || w.plane_lo - mx > tol2);
   }
   return __ballot(keep);
 }
@@ -222,19 +229,124 @@ __device__ __noinline__ int refine_texel(Mat4 m4, int width, int height, int ism
   return ((ty & 0xFFFF) << 16) | (tx & 0xFFFF);
 }
 
-// Record fetch variants (picked per shape by the autotuner):
-//   kFetchLds     chunk staged in LDS, full record read per iteration
-//   kFetchLdsTest chunk staged in LDS, the hemisphere-test fields read per
-//                 iteration, the rest only when a lane is lit
-//   kFetchScalar  records read from global with wave-uniform (scalar) loads,
-//                 next record prefetched into SGPRs
-enum FetchVariant { kFetchLds = 0, kFetchLdsTest = 1, kFetchScalar = 2 };
-
 struct VplTest {
   float px, py, pz, plane, nx, ny, nz;
 };
 
-template <int TW, int TH, bool kShadow, int kFetch>
+struct LitConsts {
+  float hs;   // ((p + 1) / 2) * S = p * S/2 + S/2
+  int amask;  // atlas side - 1
+  int alog;   // log2(atlas side)
+  float k1;   // smoothstep slope / zFar
+  float k2;   // smoothstep slope / 65535
+};
+
+__device__ __forceinline__ LitConsts lit_consts(const GatherArgs& a) {
+  return LitConsts{0.5f * (float)a.ism_side, a.atlas_side - 1, 31 - __builtin_clz(a.atlas_side),
+                   a.shadow_k1, a.shadow_k2};
+}
+
+// Contribution of one lit pair (n_v . d > 0, n_p . d < 0) to the pixel's sums
+// (final_gathering.comp:100-131): geometry term, clamp, ISM visibility.
+template <bool kMixed>
+__device__ __forceinline__ void accumulate_pair(const GatherArgs& a, const Pixel& p, const VplTest& vt,
+                                                const VplLds& v, int vi, float ndot, float pdot,
+                                                bool shadow, const LitConsts& lc, float& ar, float& ag,
+                                                float& ab) {
+  const float dx = p.fx - vt.px, dy = p.fy - vt.py, dz = p.fz - vt.pz;
+  const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+  const float inv = __builtin_amdgcn_rsqf(d2);
+  const float inv2 = inv * inv;
+  // cos_v cos_p / d^4 = -(ndot pdot) / d^6
+  float g = fminf(-(ndot * pdot) * inv2 * (inv2 * inv2), a.clamp_value);
+  if (shadow) {
+    const float dist = d2 * inv;
+    const float rh = lc.hs * __builtin_amdgcn_rcpf(dist + ndot);
+    const float sd = __builtin_fmaf(v.sx, dx, __builtin_fmaf(v.sy, dy, v.sz * dz));
+    const float ud = __builtin_fmaf(v.ux, dx, __builtin_fmaf(v.uy, dy, v.uz * dz));
+    const float cx = __builtin_fmaf(sd, rh, lc.hs);
+    const float cy = __builtin_fmaf(ud, rh, lc.hs);
+    int tx, ty;
+    if (kMixed) {
+      tx = (int)floorf(cx);
+      ty = (int)floorf(cy);
+    } else {
+      const float fx = __builtin_amdgcn_fractf(cx), fy = __builtin_amdgcn_fractf(cy);
+      tx = (int)(cx - fx);
+      ty = (int)(cy - fy);
+      if (fmaxf(fabsf(fx - 0.5f), fabsf(fy - 0.5f)) > 0.5f - kTie) {
+        const int t2 = refine_texel(a.vp_inv, a.width, a.height, a.ism_side, p.x, p.y, p.depth, vt.px,
+                                    vt.py, vt.pz, vt.nx, vt.ny, vt.nz);
+        tx = (int)(int16_t)(t2 & 0xFFFF);
+        ty = t2 >> 16;
+      }
+    }
+    const int tile_y = vi >> a.n1d_log2;
+    const int tile_xi = vi & (a.n1d - 1);
+    const int gx = (tile_xi * a.ism_side + tx) & lc.amask;  // GL_REPEAT wrap
+    const int gy = (tile_y * a.ism_side + ty) & lc.amask;
+    const float occ = (float)a.ism[(uint32_t)((gy << lc.alog) | gx)];
+    // smoothstep(1 - bias, 1, 1 - (dist/zFar - occ)): t = 1 + (occ - dist/zFar) / bias
+    float t = __builtin_fmaf(occ, lc.k2, __builtin_fmaf(-dist, lc.k1, 1.0f));
+    t = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
+    g *= t * t * __builtin_fmaf(-2.0f, t, 3.0f);
+  }
+  if (kMixed) {
+    // flux as packed halves: (r, g) in cr, (b, 0) in cg
+    const uint32_t rg = __builtin_bit_cast(uint32_t, v.cr);
+    const uint32_t bb = __builtin_bit_cast(uint32_t, v.cg);
+    const float fr = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg & 0xFFFFu));
+    const float fg = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg >> 16));
+    const float fb = (float)__builtin_bit_cast(_Float16, (uint16_t)(bb & 0xFFFFu));
+    ar = __builtin_fmaf(fr, g, ar);
+    ag = __builtin_fmaf(fg, g, ag);
+    ab = __builtin_fmaf(fb, g, ab);
+  } else {
+    ar = __builtin_fmaf(v.cr, g, ar);
+    ag = __builtin_fmaf(v.cg, g, ag);
+    ab = __builtin_fmaf(v.cb, g, ab);
+  }
+}
+
+// A staged VPL record in the form the precision uses (MIXED: flux packed to
+// halves, round to nearest even).
+template <bool kMixed>
+__device__ __forceinline__ VplLds stage_vpl(const float* rec) {
+  VplLds v = load_vpl(rec);
+  if (kMixed) {
+    const uint32_t hr = __builtin_bit_cast(uint16_t, (_Float16)v.cr);
+    const uint32_t hg = __builtin_bit_cast(uint16_t, (_Float16)v.cg);
+    const uint32_t hb = __builtin_bit_cast(uint16_t, (_Float16)v.cb);
+    v.cr = __builtin_bit_cast(float, hr | (hg << 16));
+    v.cg = __builtin_bit_cast(float, hb);
+  }
+  return v;
+}
+
+// Kahan step: total += x.
+__device__ __forceinline__ void kahan_add(float& total, float& comp, float x) {
+  const float y = x - comp;
+  const float t = total + y;
+  comp = (t - total) - y;
+  total = t;
+}
+
+// Record fetch variants (picked per shape by the autotuner):
+//   kFetchLds     full record read from LDS at the top of each iteration
+//   kFetchLdsTest hemisphere-test fields read per iteration, the rest only
+//                 when a lane is lit
+enum FetchVariant { kFetchLds = 0, kFetchLdsTest = 1 };
+
+// Precision forms.
+//   FP32  the reference's arithmetic in FP32, NEAREST texel decisions that
+//         FP32 cannot resolve re-decided in FP64 (rel-L2 <= 1e-4).
+//   MIXED flux staged and multiplied as packed FP16 (f16 x f32 -> f32 mixed
+//         FMAs), NEAREST decided in FP32 alone, per-pixel sums in FP32 per
+//         64-VPL chunk, chunks combined with Kahan compensation
+//         (rel-L2 <= 2e-3). Positions, normals, the projection basis and the
+//         distance terms stay FP32: an FP16 basis moves texel decisions by up
+//         to 0.03 texel, which alone costs ~6e-3 rel-L2.
+template <int TW, int TH, bool kShadow, int kFetch, bool kMixed>
 __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int tiles_x,
                                                               int num_work, int per_xcd) {
   const int b = blockIdx.x;
@@ -249,88 +361,36 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   const Pixel p = load_pixel(a, tile_x * TW + lx, tile_y * TH + ly, kx, ky);
 
   // ---- wave bounds for culling
-  const float inf = __builtin_inff();
   WaveBounds wb;
-  const float mnx = wave_min(p.valid ? p.fx : inf), mxx = wave_max(p.valid ? p.fx : -inf);
-  const bool wave_live = mnx <= mxx;  // some pixel of this wave is on screen (wave-uniform)
-  {
-    const float mny = wave_min(p.valid ? p.fy : inf), mxy = wave_max(p.valid ? p.fy : -inf);
-    const float mnz = wave_min(p.valid ? p.fz : inf), mxz = wave_max(p.valid ? p.fz : -inf);
-    wb.cx = 0.5f * (mnx + mxx);
-    wb.cy = 0.5f * (mny + mxy);
-    wb.cz = 0.5f * (mnz + mxz);
-    wb.hx = 0.5f * (mxx - mnx) * 1.0001f;
-    wb.hy = 0.5f * (mxy - mny) * 1.0001f;
-    wb.hz = 0.5f * (mxz - mnz) * 1.0001f;
-    wb.nlx = wave_min(p.valid ? p.nx : inf);
-    wb.nhx = wave_max(p.valid ? p.nx : -inf);
-    wb.nly = wave_min(p.valid ? p.ny : inf);
-    wb.nhy = wave_max(p.valid ? p.ny : -inf);
-    wb.nlz = wave_min(p.valid ? p.nz : inf);
-    wb.nhz = wave_max(p.valid ? p.nz : -inf);
-    wb.plane_lo = wave_min(p.valid ? p.plane : inf);
-    wb.scale = 4.0f * fmaxf(fmaxf(fmaxf(fabsf(mnx), fabsf(mxx)), fmaxf(fabsf(mny), fabsf(mxy))),
-                            fmaxf(fabsf(mnz), fabsf(mxz)));
-  }
+  const bool wave_live = wave_bounds(p, wb);  // some pixel of this wave is on screen (wave-uniform)
 
   __shared__ VplLds vtile[64];
   const int first = k * a.per_class;
-  const float hs = 0.5f * (float)a.ism_side;  // ((p + 1) / 2) * S = p * S/2 + S/2
-  const int amask = a.atlas_side - 1;
-  const int alog = 31 - __builtin_clz(a.atlas_side);
-  const float k1 = a.shadow_k1;  // smoothstep slope / zFar
-  const float k2 = a.shadow_k2;  // smoothstep slope / 65535
-  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+  const LitConsts lc = lit_consts(a);
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f;     // FP32 form: the sums; MIXED: this chunk's sums
+  float tr = 0.0f, tg = 0.0f, tb = 0.0f;     // MIXED: Kahan totals
+  float er = 0.0f, eg = 0.0f, eb = 0.0f;     // MIXED: Kahan compensations
 
   for (int chunk = 0; chunk < a.per_class; chunk += 64) {
     const int count = min(64, a.per_class - chunk);
     const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-    if (kFetch != kFetchScalar) {
-      // ---- stage the chunk's VPLs in LDS (one record per lane)
-      if (chunk > 0) __syncthreads();
-      if ((int)threadIdx.x < count) vtile[threadIdx.x] = load_vpl(crec + (size_t)threadIdx.x * kVplFloats);
-      __syncthreads();
-    }
+    // ---- stage the chunk's VPLs in LDS (one record per lane)
+    if (chunk > 0) __syncthreads();
+    if ((int)threadIdx.x < count) vtile[threadIdx.x] = stage_vpl<kMixed>(crec + (size_t)threadIdx.x * kVplFloats);
+    __syncthreads();
     if (!wave_live) continue;
-    unsigned long long mask;
-    if (kFetch != kFetchScalar) {
-      mask = cull_chunk(vtile, count, wb);
-    } else {
-      const int lane = threadIdx.x & 63;
-      VplLds mine{};
-      if (lane < count) mine = load_vpl(crec + (size_t)lane * kVplFloats);
-      mask = cull_one(mine, lane < count, wb);
-    }
-    if (!mask) continue;
-    auto fetch_test = [&](int jj) -> VplTest {
-      if (kFetch == kFetchScalar) {
-        const float4* r = reinterpret_cast<const float4*>(crec + (size_t)jj * kVplFloats);
-        const float4 a0 = r[0], b0 = r[1];
-        return VplTest{a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, b0.z};
-      }
-      const VplLds& t = vtile[jj];
-      return VplTest{t.px, t.py, t.pz, t.plane, t.nx, t.ny, t.nz};
-    };
-    // LDS variants read the record at the top of each iteration (other waves
-    // hide the latency; a one-ahead copy would cost a register move per
-    // field and iteration); the scalar variant keeps the next record in SGPRs.
-    int j = __builtin_ctzll(mask);
-    VplTest vt = fetch_test(j);
-    VplLds vf = (kFetch == kFetchScalar) ? load_vpl(crec + (size_t)j * kVplFloats) : VplLds{};
-    while (true) {
+    unsigned long long mask = cull_chunk(vtile, count, wb);
+    while (mask) {
+      const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
-      const int jn = mask ? __builtin_ctzll(mask) : j;
-      VplTest vtn{};
-      VplLds vfn{};
-      if (kFetch == kFetchScalar) {
-        vtn = fetch_test(jn);
-        vfn = load_vpl(crec + (size_t)jn * kVplFloats);
-      }
+      VplTest vt;
+      VplLds vf;
       if (kFetch == kFetchLds) {
         vf = vtile[j];
         vt = VplTest{vf.px, vf.py, vf.pz, vf.plane, vf.nx, vf.ny, vf.nz};
-      } else if (kFetch == kFetchLdsTest) {
-        vt = fetch_test(j);
+      } else {
+        const VplLds& t = vtile[j];
+        vt = VplTest{t.px, t.py, t.pz, t.plane, t.nx, t.ny, t.nz};
       }
       const int vi = first + chunk + j;
       // n_vpl . (p - p_vpl) and n_p . (p - p_vpl) from the plane offsets
@@ -338,54 +398,106 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
       const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
       if (ndot > 0.0f && pdot < 0.0f) {
         const VplLds v = (kFetch == kFetchLdsTest) ? vtile[j] : vf;
-        const float dx = p.fx - vt.px, dy = p.fy - vt.py, dz = p.fz - vt.pz;
-        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        const float inv = __builtin_amdgcn_rsqf(d2);
-        const float inv2 = inv * inv;
-        // cos_v cos_p / d^4 = -(ndot pdot) / d^6
-        float g = fminf(-(ndot * pdot) * inv2 * (inv2 * inv2), a.clamp_value);
-        if (kShadow) {
-          const float dist = d2 * inv;
-          const float rh = hs * __builtin_amdgcn_rcpf(dist + ndot);
-          const float sd = __builtin_fmaf(v.sx, dx, __builtin_fmaf(v.sy, dy, v.sz * dz));
-          const float ud = __builtin_fmaf(v.ux, dx, __builtin_fmaf(v.uy, dy, v.uz * dz));
-          const float cx = __builtin_fmaf(sd, rh, hs);
-          const float cy = __builtin_fmaf(ud, rh, hs);
-          const float fx = __builtin_amdgcn_fractf(cx), fy = __builtin_amdgcn_fractf(cy);
-          int tx = (int)(cx - fx), ty = (int)(cy - fy);
-          if (fmaxf(fabsf(fx - 0.5f), fabsf(fy - 0.5f)) > 0.5f - kTie) {
-            const int t2 = refine_texel(a.vp_inv, a.width, a.height, a.ism_side, p.x, p.y, p.depth,
-                                        vt.px, vt.py, vt.pz, vt.nx, vt.ny, vt.nz);
-            tx = (int)(int16_t)(t2 & 0xFFFF);
-            ty = t2 >> 16;
-          }
-          const int tile_y = vi >> a.n1d_log2;
-          const int tile_xi = vi & (a.n1d - 1);
-          const int gx = (tile_xi * a.ism_side + tx) & amask;  // GL_REPEAT wrap
-          const int gy = (tile_y * a.ism_side + ty) & amask;
-          const float occ = (float)a.ism[(uint32_t)((gy << alog) | gx)];
-          // smoothstep(1 - bias, 1, 1 - (dist/zFar - occ)): t = 1 + (occ - dist/zFar) / bias
-          float t = __builtin_fmaf(occ, k2, __builtin_fmaf(-dist, k1, 1.0f));
-          t = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
-          g *= t * t * __builtin_fmaf(-2.0f, t, 3.0f);
-        }
-        ar = __builtin_fmaf(v.cr, g, ar);
-        ag = __builtin_fmaf(v.cg, g, ag);
-        ab = __builtin_fmaf(v.cb, g, ab);
+        accumulate_pair<kMixed>(a, p, vt, v, vi, ndot, pdot, kShadow, lc, ar, ag, ab);
       }
-      if (!mask) break;
-      if (kFetch == kFetchScalar) {
-        vt = vtn;
-        vf = vfn;
-      }
-      j = jn;
     }
+    if (kMixed) {
+      kahan_add(tr, er, ar);
+      kahan_add(tg, eg, ag);
+      kahan_add(tb, eb, ab);
+      ar = ag = ab = 0.0f;
+    }
+  }
+  if (kMixed) {
+    ar = tr;
+    ag = tg;
+    ab = tb;
   }
   if (p.valid) {
     float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
     o[0] = ar * a.out_scale;
     o[1] = ag * a.out_scale;
     o[2] = ab * a.out_scale;
+  }
+}
+
+// Batched gather: persistent wave64s over the (frame, class, strip) items of
+// all frames of one precision. A strip is 2^w x 64/2^w pixels of one
+// interleave class (w per frame, picked by the autotuner). Each wave stages
+// its own 64-record VPL chunks in LDS and synchronises only within itself, so
+// frames of different shapes share workgroups freely. The per-pixel
+// arithmetic and summation order are the single-frame kernel's, so a frame
+// gives the same bits alone or in any batch.
+template <bool kMixed>
+__global__ __launch_bounds__(256) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
+                                                           uint32_t total) {
+  __shared__ VplLds vtiles[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  VplLds* vtile = vtiles[wid];
+  const uint32_t slots = (gridDim.x / kBatchXcds) * 4;
+  const uint32_t slot = (blockIdx.x / kBatchXcds) * 4 + wid;
+  const XcdRange xr = xcd_range(total, slot, slots);
+  for (uint32_t item = xr.start; item < xr.end; item += xr.stride) {
+    const BatchFrame& f = frames[batch_find<&BatchFrame::first_wave>(frames, n, item)];
+    const GatherArgs& a = f.g;
+    const uint32_t local = item - f.first_wave;
+    const int classes_log2 = 2 * a.interleave_log2;
+    const int k = local & ((1u << classes_log2) - 1);
+    const int strip = local >> classes_log2;
+    const int kx = k & (a.interleave - 1), ky = k >> a.interleave_log2;
+    const int sx = strip % f.strips_x, sy = strip / f.strips_x;
+    const int wlog = f.wave_w_log2;
+    const int lx = lane & ((1 << wlog) - 1), ly = lane >> wlog;
+    const Pixel p = load_pixel(a, (sx << wlog) + lx, sy * (64 >> wlog) + ly, kx, ky);
+    WaveBounds wb;
+    const bool live = wave_bounds(p, wb);
+    if (!live) continue;  // wave-uniform: nothing of this strip is on screen
+    const int first = k * a.per_class;
+    const LitConsts lc = lit_consts(a);
+    const bool shadow = a.shadowing != 0;
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    float tr = 0.0f, tg = 0.0f, tb = 0.0f, er = 0.0f, eg = 0.0f, eb = 0.0f;
+    for (int chunk = 0; chunk < a.per_class; chunk += 64) {
+      const int count = min(64, a.per_class - chunk);
+      const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
+      // wave-local staging: the previous chunk's reads are done before the
+      // overwrite, and the new records are written before any lane reads them
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane < count) vtile[lane] = stage_vpl<kMixed>(crec + (size_t)lane * kVplFloats);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      unsigned long long mask = cull_chunk(vtile, count, wb);
+      while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1ull;
+        const VplLds v = vtile[j];
+        const VplTest vt{v.px, v.py, v.pz, v.plane, v.nx, v.ny, v.nz};
+        const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
+        const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
+        if (ndot > 0.0f && pdot < 0.0f)
+          accumulate_pair<kMixed>(a, p, vt, v, first + chunk + j, ndot, pdot, shadow, lc, ar, ag, ab);
+      }
+      if (kMixed) {
+        kahan_add(tr, er, ar);
+        kahan_add(tg, eg, ag);
+        kahan_add(tb, eb, ab);
+        ar = ag = ab = 0.0f;
+      }
+    }
+    if (kMixed) {
+      ar = tr;
+      ag = tg;
+      ab = tb;
+    }
+    if (p.valid) {
+      float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
+      o[0] = ar * a.out_scale;
+      o[1] = ag * a.out_scale;
+      o[2] = ab * a.out_scale;
+    }
   }
 }
 
@@ -417,7 +529,7 @@ __global__ __launch_bounds__(64) void count_lit_kernel(GatherArgs a, int tiles_x
     __hip_atomic_fetch_add(counter, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int TW, int TH, int kFetch>
+template <int TW, int TH, int kFetch, bool kMixed>
 hipError_t launch_tiled(hipStream_t s, const GatherArgs& a) {
   const int cw = (a.width + a.interleave - 1) / a.interleave;
   const int ch = (a.height + a.interleave - 1) / a.interleave;
@@ -426,20 +538,21 @@ hipError_t launch_tiled(hipStream_t s, const GatherArgs& a) {
   const int per_xcd = (num_work + kXcds - 1) / kXcds;
   const dim3 grid(per_xcd * kXcds), block(TW * TH);
   if (a.shadowing)
-    hipLaunchKernelGGL((final_gather_kernel<TW, TH, true, kFetch>), grid, block, 0, s, a, tiles_x,
-                       num_work, per_xcd);
+    hipLaunchKernelGGL((final_gather_kernel<TW, TH, true, kFetch, kMixed>), grid, block, 0, s, a,
+                       tiles_x, num_work, per_xcd);
   else
-    hipLaunchKernelGGL((final_gather_kernel<TW, TH, false, kFetch>), grid, block, 0, s, a, tiles_x,
-                       num_work, per_xcd);
+    hipLaunchKernelGGL((final_gather_kernel<TW, TH, false, kFetch, kMixed>), grid, block, 0, s, a,
+                       tiles_x, num_work, per_xcd);
   return hipGetLastError();
 }
 
 template <int TW, int TH>
-hipError_t launch_fetch(hipStream_t s, const GatherArgs& a, int fetch) {
-  switch (fetch) {
-    case kFetchLds: return launch_tiled<TW, TH, kFetchLds>(s, a);
-    case kFetchLdsTest: return launch_tiled<TW, TH, kFetchLdsTest>(s, a);
-    case kFetchScalar: return launch_tiled<TW, TH, kFetchScalar>(s, a);
+hipError_t launch_variant(hipStream_t s, const GatherArgs& a, int fetch, bool mixed) {
+  switch (fetch * 2 + (mixed ? 1 : 0)) {
+    case kFetchLds * 2: return launch_tiled<TW, TH, kFetchLds, false>(s, a);
+    case kFetchLds * 2 + 1: return launch_tiled<TW, TH, kFetchLds, true>(s, a);
+    case kFetchLdsTest * 2: return launch_tiled<TW, TH, kFetchLdsTest, false>(s, a);
+    case kFetchLdsTest * 2 + 1: return launch_tiled<TW, TH, kFetchLdsTest, true>(s, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -448,12 +561,23 @@ hipError_t launch_fetch(hipStream_t s, const GatherArgs& a, int fetch) {
 
 hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision, int tile_w,
                                int tile_h, int fetch) {
-  (void)precision;
-  if (tile_w == 8 && tile_h == 8) return launch_fetch<8, 8>(s, a, fetch);
-  if (tile_w == 16 && tile_h == 16) return launch_fetch<16, 16>(s, a, fetch);
-  if (tile_w == 32 && tile_h == 8) return launch_fetch<32, 8>(s, a, fetch);
-  if (tile_w == 16 && tile_h == 4) return launch_fetch<16, 4>(s, a, fetch);
+  const bool mixed = precision == 1;
+  if (tile_w == 8 && tile_h == 8) return launch_variant<8, 8>(s, a, fetch, mixed);
+  if (tile_w == 16 && tile_h == 16) return launch_variant<16, 16>(s, a, fetch, mixed);
+  if (tile_w == 32 && tile_h == 8) return launch_variant<32, 8>(s, a, fetch, mixed);
+  if (tile_w == 16 && tile_h == 4) return launch_variant<16, 4>(s, a, fetch, mixed);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_gather(hipStream_t s, const BatchFrame* frames, int n, uint32_t total,
+                               int precision) {
+  if (total == 0) return hipSuccess;
+  const unsigned grid = persistent_grid((total + 3) / 4, 8);
+  if (precision == 1)
+    hipLaunchKernelGGL(batch_gather_kernel<true>, dim3(grid), dim3(256), 0, s, frames, n, total);
+  else
+    hipLaunchKernelGGL(batch_gather_kernel<false>, dim3(grid), dim3(256), 0, s, frames, n, total);
+  return hipGetLastError();
 }
 
 hipError_t launch_count_lit_pairs(hipStream_t s, const GatherArgs& a,

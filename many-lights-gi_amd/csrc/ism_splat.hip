@@ -12,6 +12,7 @@
 // hemisphere tests as a bit mask, and projects only those. The projected
 // texel receives a no-return, relaxed, agent-scope atomic umin, which the L2
 // executes without a round trip to the wave.
+#include "batch_device.hpp"
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
 
@@ -40,14 +41,21 @@ struct Candidate {
   int pad;
 };
 
+// Bin b of one frame: stage the 16 candidates in LDS, then every lane splats
+// one point per pass. The first point record is requested before the staging
+// barrier and each next one while the current one is processed.
 template <int kThreads>
-__global__ __launch_bounds__(kThreads) void ism_splat_kernel(
-    const float4* __restrict__ points, const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
-    const float* __restrict__ vpl, int n_vpl, int n1d, int atlas_side, float z_far,
-    uint32_t* __restrict__ atlas) {
-  __shared__ Candidate cand[kMaxTest];
-  const int b = blockIdx.x;
+__device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restrict__ points,
+                                          const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
+                                          const float* __restrict__ vpl, int n_vpl, int n1d,
+                                          int atlas_side, float z_far, uint32_t* __restrict__ atlas,
+                                          int b) {
   const int lane = threadIdx.x;
+  // clamp to the point count so malformed offsets cannot index past the buffer
+  const uint32_t end = min(bin_offsets[b + 1], n_points);
+  const uint32_t begin = min(bin_offsets[b], end);
+  uint32_t p = begin + lane;
+  float4 rec = p < end ? points[p] : make_float4(0.f, 0.f, 0.f, 0.f);
   if (lane < kMaxTest) {
     const int idx = (b + lane) % n_vpl;  // index %= totalVplCount
     const float4* r = reinterpret_cast<const float4*>(vpl + (size_t)idx * kVplFloats);
@@ -66,13 +74,11 @@ __global__ __launch_bounds__(kThreads) void ism_splat_kernel(
   }
   __syncthreads();
 
-  // clamp to the point count so malformed offsets cannot index past the buffer
-  const uint32_t end = min(bin_offsets[b + 1], n_points);
-  const uint32_t begin = min(bin_offsets[b], end);
   const float inv_n1d = 1.0f / (float)n1d;  // exact: n1d is a power of two
   const float side = (float)atlas_side;
-  for (uint32_t p = begin + lane; p < end; p += kThreads) {
-    const float4 rec = points[p];
+  while (p < end) {
+    const uint32_t pn = p + kThreads;
+    const float4 next = pn < end ? points[pn] : rec;
     const f3 pos = dev::make(rec.x, rec.y, rec.z);
     const uint32_t bits = __float_as_uint(rec.w);
     // unpackUnorm4x8: c / 255
@@ -80,7 +86,7 @@ __global__ __launch_bounds__(kThreads) void ism_splat_kernel(
     const float uy = (float)((bits >> 8) & 0xFFu) / 255.0f;
     const float uz = (float)((bits >> 16) & 0xFFu) / 255.0f;
     const float uw = (float)(bits >> 24) / 255.0f;
-    const f3 pn = dev::make(ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, uz * 2.0f - 1.0f);
+    const f3 pn3 = dev::make(ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, uz * 2.0f - 1.0f);
     const float radius = uw * 25.0f;
     const uint32_t radius_byte = dev::to_uint(radius * 10.0f / 2.0f);  // sqrt(4) == 2
 
@@ -92,7 +98,7 @@ __global__ __launch_bounds__(kThreads) void ism_splat_kernel(
       const Candidate& c = cand[i];
       const f3 rel = dev::sub(pos, dev::make(c.px, c.py, c.pz));
       const f3 nrel = dev::make(-rel.x, -rel.y, -rel.z);
-      const bool cull = dev::dot(dev::make(c.nx, c.ny, c.nz), rel) < 0.0f || dev::dot(pn, nrel) < 0.0f;
+      const bool cull = dev::dot(dev::make(c.nx, c.ny, c.nz), rel) < 0.0f || dev::dot(pn3, nrel) < 0.0f;
       if (!cull && found < kMaxCollect) {
         keep |= 1u << i;
         ++found;
@@ -132,6 +138,45 @@ __global__ __launch_bounds__(kThreads) void ism_splat_kernel(
       const size_t texel = (size_t)(uint32_t)fy * (uint32_t)atlas_side + (uint32_t)fx;
       __hip_atomic_fetch_min(atlas + texel, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    rec = next;
+    p = pn;
+  }
+}
+
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void ism_splat_kernel(
+    const float4* __restrict__ points, const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
+    const float* __restrict__ vpl, int n_vpl, int n1d, int atlas_side, float z_far,
+    uint32_t* __restrict__ atlas) {
+  __shared__ Candidate cand[kMaxTest];
+  splat_bin<kThreads>(cand, points, bin_offsets, n_points, vpl, n_vpl, n1d, atlas_side, z_far, atlas,
+                      blockIdx.x);
+}
+
+// Batched splat: persistent workgroups over the (frame, bin) items.
+__global__ __launch_bounds__(256) void batch_splat_kernel(const BatchFrame* __restrict__ frames, int n,
+                                                          uint32_t total) {
+  __shared__ Candidate cand[kMaxTest];
+  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+    const BatchFrame& f = frames[batch_find<&BatchFrame::first_bin>(frames, n, item)];
+    splat_bin<256>(cand, reinterpret_cast<const float4*>(f.points), f.bin_offsets, f.n_points, f.g.vpl,
+                   f.g.n_vpl, f.n1d, f.atlas_side, f.z_far, f.atlas, (int)(item - f.first_bin));
+    __syncthreads();  // the next item restages the candidates
+  }
+}
+
+// Batched clear: one 64 KiB chunk (4096 x 16 bytes) of one frame's atlas per item.
+constexpr uint32_t kClearChunk = 4096;
+__global__ __launch_bounds__(256) void batch_clear_kernel(const BatchFrame* __restrict__ frames, int n,
+                                                          uint32_t total) {
+  const uint4 ones = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+    const BatchFrame& f = frames[batch_find<&BatchFrame::first_clear>(frames, n, item)];
+    const size_t n4 = (size_t)f.atlas_side * f.atlas_side / 4;
+    const size_t c0 = (size_t)(item - f.first_clear) * kClearChunk;
+    const size_t c1 = min(n4, c0 + kClearChunk);
+    uint4* a = reinterpret_cast<uint4*>(f.atlas);
+    for (size_t i = c0 + threadIdx.x; i < c1; i += 256) a[i] = ones;
   }
 }
 
@@ -162,6 +207,23 @@ hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* 
     hipLaunchKernelGGL(ism_splat_kernel<256>, dim3(n_bins), dim3(256), 0, s, pts, bin_offsets,
                        (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas);
   return hipGetLastError();
+}
+
+hipError_t launch_batch_clear(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_clear_kernel, dim3(persistent_grid(total, 8)), dim3(256), 0, s, frames, n, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_splat(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_splat_kernel, dim3(persistent_grid(total, 8)), dim3(256), 0, s, frames, n, total);
+  return hipGetLastError();
+}
+
+uint32_t batch_clear_chunks(int atlas_side) {
+  const uint64_t n4 = (uint64_t)atlas_side * atlas_side / 4;
+  return (uint32_t)((n4 + kClearChunk - 1) / kClearChunk);
 }
 
 }  // namespace mlgi

@@ -65,4 +65,65 @@ hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision
 hipError_t launch_count_lit_pairs(hipStream_t s, const GatherArgs& a,
                                   unsigned long long* counter);
 
+// raycast.hip: G-buffer (rsm = 0: a = depth, b = normal) or RSM (rsm = 1:
+// a = diffuse, b = normal, c = depth) of a scene uploaded to the device
+}  // namespace mlgi
+namespace mlgi_scene_impl {
+struct SceneView;
+struct RayCam;
+}  // namespace mlgi_scene_impl
+namespace mlgi {
+hipError_t launch_raycast(hipStream_t st, const mlgi_scene_impl::SceneView& s,
+                          const mlgi_scene_impl::RayCam& rc, int w, int h, int rsm, float* a, float* b,
+                          float* c);
+
+// ------------------------------------------------------------------ batches
+// One frame of a batch as the persistent batched kernels see it. The table
+// lives in device memory; every stage's work items of all frames form one
+// index space, and first_* is the frame's first item in it (frames of a
+// pull-push or gather group are contiguous and counted within their group).
+struct BatchFrame {
+  // VPL extraction
+  const float* rsm_diffuse;
+  const float* rsm_normal;
+  const float* rsm_depth;
+  int rsm_w, rsm_h;
+  Mat4 inv_bvp;
+  float light_intensity;
+  const int32_t* shuffle;  // or null
+  uint32_t axis_x, axis_y;
+  // splat
+  const float* points;
+  const uint32_t* bin_offsets;
+  uint32_t n_points;
+  uint32_t* atlas;
+  int atlas_side, ism_side, n1d, levels;
+  float z_far;
+  uint32_t pp_e1[4];  // pull level 1 of an all-empty block (bits of depth, maxDepth, radius, disp)
+  // final gather (g.vpl = VPL records, g.ism = unorm16 ISM atlas, g.out)
+  GatherArgs g;
+  int wave_w_log2;  // gather wave strip: 2^wave_w_log2 x 64 / 2^wave_w_log2 class pixels
+  int strips_x;     // strips per class row
+  // first work item per stage
+  uint32_t first_vpl, first_clear, first_bin, first_ism, first_wave;
+};
+
+// Work-item counts of a frame, in each stage's unit.
+struct BatchCounts {
+  uint32_t vpl, clear, bins, isms, waves;
+};
+BatchCounts batch_counts(const BatchFrame& f);
+
+hipError_t launch_batch_vpl(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
+hipError_t launch_batch_clear(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
+uint32_t batch_clear_chunks(int atlas_side);  // clear items of one atlas
+hipError_t launch_batch_splat(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
+void pullpush_empty_texel(int ism_side, float z_far, uint32_t out[4]);
+// frames[0..n) all have the given ISM side and level count
+hipError_t launch_batch_pullpush(hipStream_t s, const BatchFrame* frames, int n, uint32_t total,
+                                 int ism_side, int levels);
+// frames[0..n) all use the given precision
+hipError_t launch_batch_gather(hipStream_t s, const BatchFrame* frames, int n, uint32_t total,
+                               int precision);
+
 }  // namespace mlgi

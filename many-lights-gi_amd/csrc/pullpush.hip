@@ -32,6 +32,7 @@
 // between R and the next float (m has 25 significant bits, m^2 then has more
 // than 48, x has 24), that is exactly x > m^2, evaluated in FP64 where m and
 // m^2 are exact: the same decision as the FP32 twin for a fraction of the cost.
+#include "batch_device.hpp"
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
 
@@ -430,47 +431,78 @@ __device__ __forceinline__ void push_levels(const Tables& tb, uint32_t* base) {
   }
 }
 
+// LDS of one ISM's pull-push.
 template <int S, int L>
-__global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __restrict__ atlas,
-                                                            int atlas_side, float z_far, PullConsts pc,
-                                                            uint16_t* __restrict__ out) {
-  __shared__ uint32_t base[S * S];
-  __shared__ Texel lv[level_offset<S>(L + 1)];
-  __shared__ float rtab[256], mtab[256], rcp[17];
+struct TileLds {
+  uint32_t base[S * S];
+  Texel lv[level_offset<S>(L + 1)];
+  float rtab[256], mtab[256], rcp[17];
+};
 
+// Pull-push of ISM `ism` of an atlas, by the whole workgroup; ends with the
+// unorm16 tile written (the caller synchronises before reusing the LDS).
+template <int S, int L>
+__device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t* __restrict__ atlas,
+                                              int atlas_side, float z_far, const PullConsts& pc,
+                                              uint16_t* __restrict__ out, int ism) {
   const int tid = threadIdx.x;
   const int n1d = atlas_side / S;
-  const int ism_x = blockIdx.x % n1d;
-  const int ism_y = blockIdx.x / n1d;
+  const int ism_x = ism % n1d;
+  const int ism_y = ism / n1d;
   const size_t a0 = (size_t)ism_y * S * atlas_side + (size_t)ism_x * S;
 
   // ---- radius-byte tables (same expressions as the per-texel decode)
   for (int b = tid; b < 256; b += kThreads) {
     const float r = (float)b / 10.0f;
-    rtab[b] = r;
-    mtab[b] = r * 2.0f / z_far * 0.6f;
+    sm.rtab[b] = r;
+    sm.mtab[b] = r * 2.0f / z_far * 0.6f;
   }
-  if (tid < 17) rcp[tid] = kRcp[tid];
+  if (tid < 17) sm.rcp[tid] = kRcp[tid];
   // ---- load the ISM's atlas tile (S*S u32) with 16-byte loads
   for (int i = tid; i < S * S / 4; i += kThreads) {
     const int row = (i * 4) / S, col = (i * 4) % S;
     const uint4 v = *reinterpret_cast<const uint4*>(atlas + a0 + (size_t)row * atlas_side + col);
-    *reinterpret_cast<uint4*>(&base[row * S + col]) = v;
+    *reinterpret_cast<uint4*>(&sm.base[row * S + col]) = v;
   }
   __syncthreads();
-  Tables tb{base, lv, rtab, mtab, rcp, {}, z_far, ism_x, ism_y};
+  Tables tb{sm.base, sm.lv, sm.rtab, sm.mtab, sm.rcp, {}, z_far, ism_x, ism_y};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
-  tb.empty = decode_texel(0xFFFFFFFFu, rtab[255], mtab[255], z_far, S);
+  tb.empty = decode_texel(0xFFFFFFFFu, sm.rtab[255], sm.mtab[255], z_far, S);
 
   pull_levels<S, 1, L>(tb, pc);
-  push_levels<S, L - 1>(tb, base);
+  push_levels<S, L - 1>(tb, sm.base);
 
   // ---- write the unorm16 tile (held in the low halves of the tile slots)
   for (int i = tid; i < S * S / 4; i += kThreads) {
     const int row = (i * 4) / S, col = (i * 4) % S;
-    const uint4 v = *reinterpret_cast<const uint4*>(&base[row * S + col]);
+    const uint4 v = *reinterpret_cast<const uint4*>(&sm.base[row * S + col]);
     *reinterpret_cast<uint2*>(out + a0 + (size_t)row * atlas_side + col) =
         make_uint2(v.x | (v.y << 16), v.z | (v.w << 16));
+  }
+}
+
+template <int S, int L>
+__global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __restrict__ atlas,
+                                                            int atlas_side, float z_far, PullConsts pc,
+                                                            uint16_t* __restrict__ out) {
+  __shared__ TileLds<S, L> sm;
+  pullpush_tile<S, L>(sm, atlas, atlas_side, z_far, pc, out, blockIdx.x);
+}
+
+// Batched: persistent workgroups over the (frame, ISM) items of frames that
+// share the ISM side S and the level count L.
+template <int S, int L>
+__global__ __launch_bounds__(kThreads) void batch_pullpush_kernel(const BatchFrame* __restrict__ frames,
+                                                                  int n, uint32_t total) {
+  __shared__ TileLds<S, L> sm;
+  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+    const BatchFrame& f = frames[batch_find<&BatchFrame::first_ism>(frames, n, item)];
+    PullConsts pc;
+    pc.e1 = Texel{__uint_as_float(f.pp_e1[0]), __uint_as_float(f.pp_e1[1]), __uint_as_float(f.pp_e1[2]),
+                  f.pp_e1[3]};
+    pullpush_tile<S, L>(sm, f.atlas, f.atlas_side, f.z_far, pc, const_cast<uint16_t*>(f.g.ism),
+                        (int)(item - f.first_ism));
+    __syncthreads();
   }
 }
 
@@ -526,7 +558,55 @@ hipError_t launch_s(hipStream_t s, const uint32_t* atlas, int atlas_side, int le
   return hipErrorInvalidValue;
 }
 
+template <int S, int L>
+hipError_t launch_batch_sl(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+  hipLaunchKernelGGL((batch_pullpush_kernel<S, L>), dim3(persistent_grid(total, 4)), dim3(kThreads), 0, s,
+                     frames, n, total);
+  return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_batch_s(hipStream_t s, const BatchFrame* frames, int n, uint32_t total, int levels) {
+  constexpr int kMax = ism_log2<S>();
+  switch (levels) {
+    case 1: return launch_batch_sl<S, 1>(s, frames, n, total);
+    case 2: return launch_batch_sl<S, 2>(s, frames, n, total);
+    case 3: return launch_batch_sl<S, 3>(s, frames, n, total);
+    case 4:
+      if constexpr (kMax >= 4) return launch_batch_sl<S, 4>(s, frames, n, total);
+      break;
+    case 5:
+      if constexpr (kMax >= 5) return launch_batch_sl<S, 5>(s, frames, n, total);
+      break;
+    case 6:
+      if constexpr (kMax >= 6) return launch_batch_sl<S, 6>(s, frames, n, total);
+      break;
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
 }  // namespace
+
+void pullpush_empty_texel(int ism_side, float z_far, uint32_t out[4]) {
+  const PullConsts pc = make_consts(ism_side, z_far);
+  out[0] = f2u(pc.e1.depth);
+  out[1] = f2u(pc.e1.max_depth);
+  out[2] = f2u(pc.e1.radius);
+  out[3] = pc.e1.disp;
+}
+
+hipError_t launch_batch_pullpush(hipStream_t s, const BatchFrame* frames, int n, uint32_t total,
+                                 int ism_side, int levels) {
+  if (total == 0) return hipSuccess;
+  switch (ism_side) {
+    case 8: return launch_batch_s<8>(s, frames, n, total, levels);
+    case 16: return launch_batch_s<16>(s, frames, n, total, levels);
+    case 32: return launch_batch_s<32>(s, frames, n, total, levels);
+    case 64: return launch_batch_s<64>(s, frames, n, total, levels);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t launch_pullpush(hipStream_t s, const uint32_t* atlas, int atlas_side, int ism_side,
                            int levels, float z_far, uint16_t* out) {

@@ -1,0 +1,36 @@
+// Device G-buffer / RSM producer: the scene's BVH ray cast on the GPU, one
+// lane per texel, with the host renderer's double-precision code
+// (scene_trace.hpp), so both write the same texels. Replaces the
+// reference's rasterisation of the G-buffer and RSM (RasterizationStage.cpp
+// and model.frag) for inputs that change per request or per frame.
+// Compiled with -ffp-contract=off like the host renderer.
+#include "kernels.hpp"
+#include "scene_trace.hpp"
+
+namespace mlgi {
+namespace {
+
+using namespace mlgi_scene_impl;
+
+__global__ __launch_bounds__(256) void raycast_kernel(SceneView s, RayCam rc, int w, int h, int rsm,
+                                                      float* __restrict__ a, float* __restrict__ b,
+                                                      float* __restrict__ c) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w || y >= h) return;
+  if (rsm)
+    shade_rsm(s, rc, x, y, w, h, a, b, c);  // diffuse, normal, depth
+  else
+    shade_gbuffer(s, rc, x, y, w, h, a, b);  // depth, normal
+}
+
+}  // namespace
+
+hipError_t launch_raycast(hipStream_t st, const SceneView& s, const RayCam& rc, int w, int h, int rsm,
+                          float* a, float* b, float* c) {
+  const dim3 grid((w + 15) / 16, (h + 15) / 16);
+  hipLaunchKernelGGL(raycast_kernel, grid, dim3(256), 0, st, s, rc, w, h, rsm, a, b, c);
+  return hipGetLastError();
+}
+
+}  // namespace mlgi

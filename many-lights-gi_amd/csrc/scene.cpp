@@ -13,20 +13,11 @@
 
 #include "../../include/mlgi.h"
 #include "../../include/mlgi_scene.h"
+#include "scene_trace.hpp"
 
 namespace {
 
-struct D3 {
-  double x, y, z;
-};
-inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
-inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-inline double len(D3 a) { return std::sqrt(dot(a, a)); }
-inline D3 norm(D3 a) { return a * (1.0 / len(a)); }
-inline double comp(D3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+using namespace mlgi_scene_impl;
 
 // splitmix64: sequential generator and counter-based hash
 inline uint64_t mix64(uint64_t z) {
@@ -46,21 +37,6 @@ inline double counter_uniform(uint64_t seed, uint64_t i, uint64_t k) {
   uint64_t z = mix64(seed ^ mix64(i * 4 + k + 0x632BE59BD9B4E019ull));
   return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
-
-struct Tri {
-  D3 v0, v1, v2, n;
-  float albedo[3];
-};
-
-struct Node {
-  double lo[3], hi[3];
-  int left, count;  // leaf: count > 0, left = first index; inner: left child, right = left+1
-};
-
-struct Hit {
-  double t;
-  int tri;
-};
 
 }  // namespace
 
@@ -136,70 +112,7 @@ void build_bvh(mlgi_scene* s) {
   }
 }
 
-inline bool box_hit(const Node& nd, D3 o, D3 inv, double tmax, double* tnear) {
-  double t0 = 0.0, t1 = tmax;
-  for (int k = 0; k < 3; ++k) {
-    double a = (nd.lo[k] - comp(o, k)) * comp(inv, k);
-    double b = (nd.hi[k] - comp(o, k)) * comp(inv, k);
-    if (a > b) std::swap(a, b);
-    t0 = std::max(t0, a);
-    t1 = std::min(t1, b);
-    if (t0 > t1) return false;
-  }
-  *tnear = t0;
-  return true;
-}
-
-// Moller-Trumbore, two-sided
-inline bool tri_hit(const Tri& t, D3 o, D3 d, double tmax, double* th) {
-  D3 e1 = t.v1 - t.v0, e2 = t.v2 - t.v0;
-  D3 p = cross(d, e2);
-  double det = dot(e1, p);
-  if (std::fabs(det) < 1e-14) return false;
-  double inv = 1.0 / det;
-  D3 s = o - t.v0;
-  double u = dot(s, p) * inv;
-  if (u < 0.0 || u > 1.0) return false;
-  D3 q = cross(s, e1);
-  double v = dot(d, q) * inv;
-  if (v < 0.0 || u + v > 1.0) return false;
-  double tt = dot(e2, q) * inv;
-  if (tt <= 0.0 || tt >= tmax) return false;
-  *th = tt;
-  return true;
-}
-
-Hit trace(const mlgi_scene* s, D3 o, D3 d, double tmax) {
-  Hit h{tmax, -1};
-  D3 inv = {1.0 / (d.x != 0 ? d.x : 1e-300), 1.0 / (d.y != 0 ? d.y : 1e-300), 1.0 / (d.z != 0 ? d.z : 1e-300)};
-  int stack[128];
-  int sp = 0;
-  stack[sp++] = 0;
-  while (sp) {
-    const Node& nd = s->nodes[stack[--sp]];
-    double tn;
-    if (!box_hit(nd, o, inv, h.t, &tn)) continue;
-    if (nd.count > 0) {
-      for (int i = nd.left; i < nd.left + nd.count; ++i) {
-        double th;
-        int ti = s->order[i];
-        if (tri_hit(s->tris[ti], o, d, h.t, &th) && (th < h.t || (th == h.t && ti < h.tri))) {
-          h.t = th;
-          h.tri = ti;
-        }
-      }
-    } else {
-      stack[sp++] = nd.left;
-      stack[sp++] = nd.left + 1;
-    }
-  }
-  return h;
-}
-
 // 4x4 column-major helpers in double
-struct M4 {
-  double m[16];
-};
 M4 mul(const M4& a, const M4& b) {
   M4 r{};
   for (int c = 0; c < 4; ++c)
@@ -235,14 +148,6 @@ bool inverse(const M4& a, M4* out) {
     for (int c = 0; c < 4; ++c) out->m[c * 4 + r] = m[r][c + 4];
   return true;
 }
-void xform(const M4& a, const double v[4], double o[4]) {
-  for (int r = 0; r < 4; ++r) {
-    double acc = 0;
-    for (int c = 0; c < 4; ++c) acc += a.m[c * 4 + r] * v[c];
-    o[r] = acc;
-  }
-}
-
 void camera_matrices(const mlgi_camera* cam, M4* view, M4* proj) {
   D3 eye = {cam->eye[0], cam->eye[1], cam->eye[2]};
   D3 ctr = {cam->center[0], cam->center[1], cam->center[2]};
@@ -289,11 +194,6 @@ void parallel_rows(int rows, int threads, F f) {
   for (auto& th : pool) th.join();
 }
 
-struct RayCam {
-  M4 vp, inv_vp;
-  D3 eye, axis;
-};
-
 RayCam ray_cam(const mlgi_camera* cam) {
   M4 v, p;
   camera_matrices(cam, &v, &p);
@@ -302,26 +202,11 @@ RayCam ray_cam(const mlgi_camera* cam) {
   inverse(rc.vp, &rc.inv_vp);
   rc.eye = {cam->eye[0], cam->eye[1], cam->eye[2]};
   rc.axis = norm(D3{cam->center[0], cam->center[1], cam->center[2]} - rc.eye);
+  rc.cos_half = std::cos((double)cam->fovy_deg * M_PI / 180.0 / 2.0);
   return rc;
 }
 
-// Ray through the pixel centre from the near plane to the far plane.
-void pixel_ray(const RayCam& rc, int x, int y, int w, int h, D3* o, D3* d) {
-  double nx = ((x + 0.5) / w) * 2.0 - 1.0, ny = ((y + 0.5) / h) * 2.0 - 1.0;
-  double a[4] = {nx, ny, -1.0, 1.0}, b[4] = {nx, ny, 1.0, 1.0}, pa[4], pb[4];
-  xform(rc.inv_vp, a, pa);
-  xform(rc.inv_vp, b, pb);
-  D3 p0 = {pa[0] / pa[3], pa[1] / pa[3], pa[2] / pa[3]};
-  D3 p1 = {pb[0] / pb[3], pb[1] / pb[3], pb[2] / pb[3]};
-  *o = p0;
-  *d = p1 - p0;
-}
-
-double window_depth(const RayCam& rc, D3 p) {
-  double v[4] = {p.x, p.y, p.z, 1.0}, c[4];
-  xform(rc.vp, v, c);
-  return 0.5 * (c[2] / c[3]) + 0.5;
-}
+SceneView view_of(const mlgi_scene* s);
 
 }  // namespace
 
@@ -457,24 +342,9 @@ int mlgi_scene_render_gbuffer(const mlgi_scene* s, const mlgi_camera* cam, int32
                               int32_t threads, float* depth, float* normal) {
   if (!s || !cam || w <= 0 || h <= 0 || !depth || !normal) return MLGI_ERR_INVALID_ARGUMENT;
   const RayCam rc = ray_cam(cam);
+  const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    for (int x = 0; x < w; ++x) {
-      D3 o, d;
-      pixel_ray(rc, x, y, w, h, &o, &d);
-      Hit hit = trace(s, o, d, 1.0);
-      size_t p = (size_t)y * w + x;
-      if (hit.tri < 0) {
-        depth[p] = 1.0f;
-        normal[p * 3] = normal[p * 3 + 1] = normal[p * 3 + 2] = 0.5f;
-        continue;
-      }
-      D3 n = s->tris[hit.tri].n;
-      if (dot(n, d) > 0) n = n * -1.0;
-      depth[p] = (float)window_depth(rc, o + d * hit.t);
-      normal[p * 3 + 0] = (float)n.x * 0.5f + 0.5f;
-      normal[p * 3 + 1] = (float)n.y * 0.5f + 0.5f;
-      normal[p * 3 + 2] = (float)n.z * 0.5f + 0.5f;
-    }
+    for (int x = 0; x < w; ++x) shade_gbuffer(sv, rc, x, y, w, h, depth, normal);
   });
   return MLGI_OK;
 }
@@ -485,40 +355,11 @@ int mlgi_scene_render_rsm(const mlgi_scene* s, const mlgi_camera* light, int32_t
   if (!s || !light || w <= 0 || h <= 0 || !diffuse || !normal || !depth)
     return MLGI_ERR_INVALID_ARGUMENT;
   const RayCam rc = ray_cam(light);
-  const double cos_half = std::cos((double)light->fovy_deg * M_PI / 180.0 / 2.0);
+  const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    for (int x = 0; x < w; ++x) {
-      D3 o, d;
-      pixel_ray(rc, x, y, w, h, &o, &d);
-      Hit hit = trace(s, o, d, 1.0);
-      size_t p = (size_t)y * w + x;
-      if (hit.tri < 0) {
-        depth[p] = 1.0f;
-        for (int k = 0; k < 3; ++k) {
-          normal[p * 3 + k] = 0.5f;
-          diffuse[p * 3 + k] = 0.0f;
-        }
-        continue;
-      }
-      const Tri& t = s->tris[hit.tri];
-      D3 n = t.n;
-      if (dot(n, d) > 0) n = n * -1.0;
-      const bool in_cone = dot(norm(d), rc.axis) >= cos_half;
-      depth[p] = (float)window_depth(rc, o + d * hit.t);
-      normal[p * 3 + 0] = (float)n.x * 0.5f + 0.5f;
-      normal[p * 3 + 1] = (float)n.y * 0.5f + 0.5f;
-      normal[p * 3 + 2] = (float)n.z * 0.5f + 0.5f;
-      for (int k = 0; k < 3; ++k) diffuse[p * 3 + k] = in_cone ? t.albedo[k] : 0.0f;
-    }
+    for (int x = 0; x < w; ++x) shade_rsm(sv, rc, x, y, w, h, diffuse, normal, depth);
   });
-  if (inv_bvp16) {
-    M4 bias{};
-    bias.m[0] = 0.5; bias.m[5] = 0.5; bias.m[10] = 0.5;
-    bias.m[12] = 0.5; bias.m[13] = 0.5; bias.m[14] = 0.5; bias.m[15] = 1.0;
-    M4 inv;
-    if (!inverse(mul(bias, rc.vp), &inv)) return MLGI_ERR_INVALID_ARGUMENT;
-    to_float(inv, inv_bvp16);
-  }
+  if (inv_bvp16 && !mlgi_scene_inv_biased_vp(light, inv_bvp16)) return MLGI_ERR_INVALID_ARGUMENT;
   return MLGI_OK;
 }
 
@@ -617,4 +458,29 @@ extern "C" int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, i
     std::memcpy(records + (size_t)dst * 4, &tmp[i * 4], 16);
   }
   return MLGI_OK;
+}
+
+namespace {
+SceneView view_of(const mlgi_scene* s) {
+  return SceneView{s->nodes.data(), s->tris.data(), s->order.data(), (int)s->nodes.size(), (int)s->tris.size()};
+}
+}  // namespace
+
+SceneView mlgi_scene_view(const mlgi_scene* s) { return view_of(s); }
+
+bool mlgi_scene_ray_cam(const mlgi_camera* cam, RayCam* out) {
+  if (!cam || !out) return false;
+  *out = ray_cam(cam);
+  return true;
+}
+
+bool mlgi_scene_inv_biased_vp(const mlgi_camera* light, float* out16) {
+  const RayCam rc = ray_cam(light);
+  M4 bias{};
+  bias.m[0] = 0.5; bias.m[5] = 0.5; bias.m[10] = 0.5;
+  bias.m[12] = 0.5; bias.m[13] = 0.5; bias.m[14] = 0.5; bias.m[15] = 1.0;
+  M4 inv;
+  if (!inverse(mul(bias, rc.vp), &inv)) return false;
+  to_float(inv, out16);
+  return true;
 }

@@ -6,6 +6,7 @@
 // normal, depth), unprojects with inverse(bias*P*V) using an explicit fused
 // multiply-add chain, quantises the normal through the 10:10:10 packing, and
 // also stores the lookAtRH basis so the splat and gather never recompute it.
+#include "batch_device.hpp"
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
 
@@ -14,13 +15,15 @@ namespace {
 
 using dev::f3;
 
-__global__ __launch_bounds__(64) void vpl_extract_kernel(
-    const float* __restrict__ diffuse, const float* __restrict__ normal,
-    const float* __restrict__ depth, int w, int h, Mat4 m, float light_intensity,
-    const int32_t* __restrict__ shuffle, int n, uint32_t axis_x, uint32_t axis_y,
-    float* __restrict__ records) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (uint32_t)n) return;
+// VPL i of a frame: RSM texel fetch, unprojection, normal quantisation,
+// basis, flux; record written at shuffle[i].
+__device__ __forceinline__ void vpl_extract_one(const float* __restrict__ diffuse,
+                                                const float* __restrict__ normal,
+                                                const float* __restrict__ depth, int w, int h,
+                                                const Mat4& m, float light_intensity,
+                                                const int32_t* __restrict__ shuffle, uint32_t i,
+                                                uint32_t axis_x, uint32_t axis_y,
+                                                float* __restrict__ records) {
   // uvec2(resultIndex % x, resultIndex / x) * samplerSize / lightsOnAxis
   const uint32_t tx = (i % axis_x) * (uint32_t)w / axis_x;
   const uint32_t ty = (i / axis_x) * (uint32_t)h / axis_y;
@@ -69,6 +72,28 @@ __global__ __launch_bounds__(64) void vpl_extract_kernel(
   out[7] = zero;
 }
 
+__global__ __launch_bounds__(64) void vpl_extract_kernel(
+    const float* __restrict__ diffuse, const float* __restrict__ normal,
+    const float* __restrict__ depth, int w, int h, Mat4 m, float light_intensity,
+    const int32_t* __restrict__ shuffle, int n, uint32_t axis_x, uint32_t axis_y,
+    float* __restrict__ records) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint32_t)n) return;
+  vpl_extract_one(diffuse, normal, depth, w, h, m, light_intensity, shuffle, i, axis_x, axis_y, records);
+}
+
+// Batched: one lane per VPL of any frame.
+__global__ __launch_bounds__(256) void batch_vpl_kernel(const BatchFrame* __restrict__ frames, int n,
+                                                        uint32_t total) {
+  for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < total;
+       item += gridDim.x * blockDim.x) {
+    const BatchFrame& f = frames[batch_find<&BatchFrame::first_vpl>(frames, n, item)];
+    vpl_extract_one(f.rsm_diffuse, f.rsm_normal, f.rsm_depth, f.rsm_w, f.rsm_h, f.inv_bvp,
+                    f.light_intensity, f.shuffle, item - f.first_vpl, f.axis_x, f.axis_y,
+                    const_cast<float*>(f.g.vpl));
+  }
+}
+
 }  // namespace
 
 hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* normal,
@@ -81,6 +106,13 @@ hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* 
   const int blocks = (n + 63) / 64;
   hipLaunchKernelGGL(vpl_extract_kernel, dim3(blocks), dim3(64), 0, s, diffuse, normal, depth, w,
                      h, inv_bvp, light_intensity, shuffle, n, s1d, axis_y, records);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_vpl(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+  if (total == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  hipLaunchKernelGGL(batch_vpl_kernel, dim3(blocks), dim3(256), 0, s, frames, n, total);
   return hipGetLastError();
 }
 

@@ -155,6 +155,13 @@ class DeviceArray:
         self.ctx.check(lib.mlgi_memcpy_htod(self.ctx.h, C.c_void_p(self.ptr), a.ctypes.data, a.nbytes),
                        "mlgi_memcpy_htod")
 
+    def copy_from_device(self, src: "DeviceArray") -> None:
+        """Stream-ordered device-to-device copy of src (same byte size)."""
+        if src.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch {src.nbytes} != {self.nbytes}")
+        self.ctx.check(lib.mlgi_memcpy_dtod(self.ctx.h, C.c_void_p(self.ptr), C.c_void_p(src.ptr), self.nbytes),
+                       "mlgi_memcpy_dtod")
+
     def numpy(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         self.ctx.check(lib.mlgi_memcpy_dtoh(self.ctx.h, out.ctypes.data, C.c_void_p(self.ptr), self.nbytes),
@@ -267,6 +274,10 @@ class Scene:
             raise MlgiError(rc, "mlgi_scene_render_rsm")
         return diffuse, normal, depth, inv
 
+    def upload(self, ctx: "Context") -> "DeviceScene":
+        """The scene's BVH on the device (mlgi_scene_upload)."""
+        return DeviceScene(ctx, self)
+
     def sample_points(self, n_points: int, n_bins: int, ism_count: int, seed: int, threads: int = 0):
         rec = np.empty((n_points, 4), np.float32)
         off = np.empty(n_bins + 1, np.uint32)
@@ -275,6 +286,43 @@ class Scene:
         if rc:
             raise MlgiError(rc, "mlgi_scene_sample_points")
         return rec, off
+
+
+class DeviceScene:
+    """Device G-buffer / RSM producer: the host renderer's ray caster on the GPU."""
+
+    def __init__(self, ctx: "Context", scene: Scene):
+        self.ctx, self.scene = ctx, scene
+        h = C.c_void_p()
+        ctx.check(lib.mlgi_scene_upload(ctx.h, scene.h, C.byref(h)), "mlgi_scene_upload")
+        self.h = h
+
+    def render_gbuffer(self, cam: Camera, depth: "DeviceArray", normal: "DeviceArray"):
+        """Into device arrays (h, w) and (h, w, 3); returns inverse(P*V) (host)."""
+        height, width = depth.shape
+        inv = np.empty(16, np.float32)
+        self.ctx.check(lib.mlgi_scene_device_render_gbuffer(self.ctx.h, self.h, C.byref(cam.to_c()), width, height,
+                                                            C.c_void_p(depth.ptr), C.c_void_p(normal.ptr),
+                                                            inv.ctypes.data), "mlgi_scene_device_render_gbuffer")
+        return inv
+
+    def render_rsm(self, light: Camera, diffuse: "DeviceArray", normal: "DeviceArray", depth: "DeviceArray"):
+        """Into device arrays; returns inverse(bias*P*V) (host)."""
+        height, width = depth.shape
+        inv = np.empty(16, np.float32)
+        self.ctx.check(lib.mlgi_scene_device_render_rsm(self.ctx.h, self.h, C.byref(light.to_c()), width, height,
+                                                        C.c_void_p(diffuse.ptr), C.c_void_p(normal.ptr),
+                                                        C.c_void_p(depth.ptr), inv.ctypes.data),
+                       "mlgi_scene_device_render_rsm")
+        return inv
+
+    def close(self) -> None:
+        if self.h:
+            lib.mlgi_scene_device_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 # ------------------------------------------------------------ descriptors
@@ -415,8 +463,109 @@ class GIStage:
         return int(out.value)
 
 
+class Batch:
+    """Many GI frames as one set of persistent batched launches.
+
+    `frames` is a list of (GIStage, rsm, points, gbuffer): each stage owns its
+    parameters and its VPL / atlas / ISM / image buffers.
+    """
+
+    def __init__(self, ctx: Context, frames):
+        self.ctx = ctx
+        self.frames = list(frames)
+        arr = (N.FrameDesc * len(self.frames))()
+        for i, (st, rsm, pts, gb) in enumerate(self.frames):
+            arr[i] = frame_desc(st, rsm, pts, gb)
+        self._descs = arr
+        h = C.c_void_p()
+        ctx.check(lib.mlgi_batch_create(ctx.h, arr, len(self.frames), C.byref(h)), "mlgi_batch_create")
+        self.h = h
+
+    def run(self) -> None:
+        self.ctx.check(lib.mlgi_batch_run(self.ctx.h, self.h), "mlgi_batch_run")
+
+    def capture(self) -> "FrameGraph":
+        g = C.c_void_p()
+        self.ctx.check(lib.mlgi_batch_capture(self.ctx.h, self.h, C.byref(g)), "mlgi_batch_capture")
+        return FrameGraph(self.ctx, g)
+
+    def close(self) -> None:
+        if self.h:
+            lib.mlgi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def frame_desc(stage: "GIStage", rsm: N.Rsm, points: N.Points, gbuffer: N.GBuffer) -> N.FrameDesc:
+    d = N.FrameDesc()
+    d.rsm, d.points, d.gbuffer, d.params = rsm, points, gbuffer, stage.params
+    d.vpls, d.atlas, d.ism, d.image = (stage.vpl_processor.desc(), stage.ism.atlas_desc(), stage.ism.depth_desc(),
+                                       stage.image_desc())
+    return d
+
+
+def autotune(ctx: Context, stage: "GIStage", rsm: N.Rsm, points: N.Points, gbuffer: N.GBuffer,
+             min_vpls_per_pixel: int = 32) -> dict:
+    """Pick interleave and gather tile for the stage's shape (mlgi_autotune);
+    updates stage.params and returns the choice. Runs the frame."""
+    d = frame_desc(stage, rsm, points, gbuffer)
+    r = N.TuneResult()
+    ctx.check(lib.mlgi_autotune(ctx.h, C.byref(d), min_vpls_per_pixel, C.byref(r)), "mlgi_autotune")
+    stage.params.interleave, stage.params.use_interleaving = d.params.interleave, d.params.use_interleaving
+    stage.params.tile_w, stage.params.tile_h = d.params.tile_w, d.params.tile_h
+    return {"interleave": r.interleave, "tile": (r.tile_w, r.tile_h), "gather_ms": list(r.gather_ms),
+            "cached": bool(r.cached)}
+
+
+class Server:
+    """GI request server (mlgi_server_*): resident scenes, HBM-budget
+    admission, continuous batching on the device."""
+
+    def __init__(self, ctx: Context, scenes, vpl_counts, resolutions, hbm_budget: int, max_inflight: int = 64,
+                 max_batch: int = 32, max_queued_batches: int = 3, rsm_size: int = 128,
+                 precision: int = PRECISION_MIXED, min_vpls_per_pixel: int = 32):
+        """scenes: list of (Scene, n_points, point_seed); resolutions: list of (w, h)."""
+        self.ctx = ctx
+        self._scenes = [sc for sc, _, _ in scenes]  # keep the host scenes alive
+        arr = (N.ServerScene * len(scenes))()
+        for i, (sc, npts, seed) in enumerate(scenes):
+            arr[i].scene, arr[i].n_points, arr[i].point_seed = sc.h, npts, seed
+        vc = (C.c_int32 * len(vpl_counts))(*vpl_counts)
+        rs = (C.c_int32 * (2 * len(resolutions)))(*[v for wh in resolutions for v in wh])
+        cfg = N.ServerConfig(hbm_budget, max_inflight, max_batch, max_queued_batches, rsm_size, precision,
+                             min_vpls_per_pixel, vc, len(vpl_counts), rs, len(resolutions))
+        h = C.c_void_p()
+        ctx.check(lib.mlgi_server_create(ctx.h, C.byref(cfg), arr, len(scenes), C.byref(h)), "mlgi_server_create")
+        self.h = h
+
+    def run(self, requests, outputs=None):
+        """requests: list of (scene, w, h, n_vpl, light_seed, arrival_s), sorted by arrival.
+        outputs: optional list of float32 (h, w, 3) arrays (or None) receiving the results.
+        Returns (stats dict, latencies ms array)."""
+        arr = (N.Request * len(requests))()
+        for i, (sc, w, h, nv, seed, t) in enumerate(requests):
+            out = outputs[i] if outputs is not None else None
+            arr[i] = N.Request(sc, w, h, nv, seed, t, out.ctypes.data if out is not None else None)
+        st = N.ServerStats()
+        lat = np.empty(len(requests), np.float64)
+        self.ctx.check(lib.mlgi_server_run(self.h, arr, len(requests), C.byref(st),
+                                           lat.ctypes.data_as(C.POINTER(C.c_double))), "mlgi_server_run")
+        stats = {k: getattr(st, k) for k, _ in N.ServerStats._fields_}
+        return stats, lat
+
+    def close(self) -> None:
+        if self.h:
+            lib.mlgi_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class FrameGraph:
-    """An instantiated HIP graph of one GI frame."""
+    """An instantiated HIP graph of one GI frame (or one batch)."""
 
     def __init__(self, ctx: Context, h):
         self.ctx, self.h = ctx, h

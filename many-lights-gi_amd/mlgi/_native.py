@@ -97,6 +97,39 @@ class Image(C.Structure):
     _fields_ = [("rgb", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32)]
 
 
+class FrameDesc(C.Structure):
+    _fields_ = [("rsm", Rsm), ("points", Points), ("gbuffer", GBuffer), ("params", Params),
+                ("vpls", Vpls), ("atlas", Atlas), ("ism", IsmDepth), ("image", Image)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("interleave", C.c_int32), ("tile_w", C.c_int32), ("tile_h", C.c_int32),
+                ("gather_ms", C.c_float * 3), ("cached", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int32), ("max_batch", C.c_int32),
+                ("max_queued_batches", C.c_int32), ("rsm_size", C.c_int32), ("precision", C.c_int32),
+                ("min_vpls_per_pixel", C.c_int32), ("vpl_counts", C.POINTER(C.c_int32)),
+                ("n_vpl_counts", C.c_int32), ("resolutions", C.POINTER(C.c_int32)), ("n_resolutions", C.c_int32)]
+
+
+class ServerScene(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("n_points", C.c_int64), ("point_seed", C.c_uint64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("n_vpl", C.c_int32),
+                ("light_seed", C.c_uint64), ("arrival_s", C.c_double), ("host_out", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("rejected", C.c_int64), ("wall_s", C.c_double),
+                ("req_per_s", C.c_double), ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
+                ("latency_mean_ms", C.c_double), ("latency_max_ms", C.c_double), ("batches", C.c_int32),
+                ("max_inflight_seen", C.c_int32), ("mean_batch", C.c_double), ("peak_bytes", C.c_size_t)]
+
+
 class StageTimes(C.Structure):
     _fields_ = [
         ("vpl_ms", C.c_float),
@@ -139,6 +172,7 @@ SIGNATURES = {
     "mlgi_free": (C.c_int, [vp, vp]),
     "mlgi_memcpy_htod": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "mlgi_memcpy_dtoh": (C.c_int, [vp, vp, vp, C.c_size_t]),
+    "mlgi_memcpy_dtod": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "mlgi_vpl_extract": (C.c_int, [vp, P(Rsm), P(Params), P(Vpls)]),
     "mlgi_ism_splat": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas)]),
     "mlgi_ism_pullpush": (C.c_int, [vp, P(Atlas), P(Params), P(IsmDepth)]),
@@ -150,6 +184,14 @@ SIGNATURES = {
     "mlgi_graph_capture_frame": (C.c_int, [vp, P(Rsm), P(Points), P(GBuffer), P(Params), P(Vpls),
                                            P(Atlas), P(IsmDepth), P(Image), P(vp)]),
     "mlgi_graph_launch": (C.c_int, [vp, vp]),
+    "mlgi_batch_create": (C.c_int, [vp, P(FrameDesc), C.c_int32, P(vp)]),
+    "mlgi_batch_run": (C.c_int, [vp, vp]),
+    "mlgi_batch_capture": (C.c_int, [vp, vp, P(vp)]),
+    "mlgi_batch_destroy": (None, [vp]),
+    "mlgi_autotune": (C.c_int, [vp, P(FrameDesc), C.c_int32, P(TuneResult)]),
+    "mlgi_server_create": (C.c_int, [vp, P(ServerConfig), P(ServerScene), C.c_int32, P(vp)]),
+    "mlgi_server_run": (C.c_int, [vp, P(Request), C.c_int32, P(ServerStats), P(C.c_double)]),
+    "mlgi_server_destroy": (None, [vp]),
     "mlgi_graph_destroy": (None, [vp]),
     "mlgi_profile_enable": (C.c_int, [vp, i32]),
     "mlgi_profile_reset": (C.c_int, [vp]),
@@ -167,6 +209,10 @@ SIGNATURES = {
     "mlgi_camera_matrices": (C.c_int, [P(Camera), vp, vp, vp]),
     "mlgi_scene_render_gbuffer": (C.c_int, [vp, P(Camera), i32, i32, i32, vp, vp]),
     "mlgi_scene_render_rsm": (C.c_int, [vp, P(Camera), i32, i32, i32, vp, vp, vp, vp]),
+    "mlgi_scene_upload": (C.c_int, [vp, vp, P(vp)]),
+    "mlgi_scene_device_destroy": (None, [vp]),
+    "mlgi_scene_device_render_gbuffer": (C.c_int, [vp, vp, P(Camera), i32, i32, vp, vp, vp]),
+    "mlgi_scene_device_render_rsm": (C.c_int, [vp, vp, P(Camera), i32, i32, vp, vp, vp, vp]),
     "mlgi_scene_sample_points": (C.c_int, [vp, C.c_int64, i32, i32, C.c_uint64, i32, vp, vp]),
 }
 

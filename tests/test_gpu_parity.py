@@ -5,7 +5,9 @@ Bars (stage-wise: each oracle stage consumes the GPU's previous stage):
     normals, flux, lookAtRH basis); positions within 1e-5 of FP64.
   * ISM atlas: bit-exact vs the FP32 twin (integer atomicMin is order-free).
   * Pull-push R16: bit-exact vs the FP32 twin.
-  * Irradiance: rel-L2 <= 1e-4 vs the FP64 transcription (plain FP32 form).
+  * Irradiance: rel-L2 <= 1e-4 vs the FP64 transcription (plain FP32 form),
+    <= 2e-3 (mixed form), with the mixed form's error printed per output
+    channel and as per-pixel quantiles.
 """
 import numpy as np
 import pytest
@@ -17,6 +19,7 @@ from mlgi import workloads as W
 pytestmark = pytest.mark.gpu
 
 REL_L2_FP32 = 1e-4
+REL_L2_MIXED = 2e-3
 
 
 def rel_l2(a, b):
@@ -105,6 +108,31 @@ def test_final_gather_fp32_vs_fp64(frame):
     err = rel_l2(frame.img, ref)
     print(f"[fg] {frame.wl.name} rel-L2 {err:.3e} lit pairs {lit}")
     assert err <= REL_L2_FP32
+
+
+def test_final_gather_mixed_vs_fp64(frame, ctx):
+    pos, nrm, col = frame.vpl
+    p = frame.params
+    st = frame.stage
+    st.params.precision = mlgi.PRECISION_MIXED
+    try:
+        st.compute_final_gathering(frame.gb)
+        ctx.sync()
+        img = st.gi_buffer.numpy()
+    finally:
+        st.params.precision = mlgi.PRECISION_FP32
+    ref, _ = oracle.final_gather(frame.inp.gb_depth, frame.inp.gb_normal, frame.inp.vp_inv, pos, nrm, col,
+                                 frame.r16, frame.wl.atlas_side, p.z_far, p.gi_intensity_factor,
+                                 p.vpl_clamping_value, frame.wl.interleave)
+    err = rel_l2(img, ref)
+    per_channel = [rel_l2(img[..., c], ref[..., c]) for c in range(3)]
+    pix = np.linalg.norm(img - ref, axis=-1) / np.maximum(np.linalg.norm(ref, axis=-1), 1e-30)
+    q = np.quantile(pix[np.linalg.norm(ref, axis=-1) > 0], [0.5, 0.99, 0.999])
+    print(f"[fg mixed] {frame.wl.name} rel-L2 {err:.3e} per channel {['%.3e' % e for e in per_channel]} "
+          f"per-pixel rel err p50/p99/p99.9 {q[0]:.2e}/{q[1]:.2e}/{q[2]:.2e}")
+    assert err <= REL_L2_MIXED
+    assert max(per_channel) <= REL_L2_MIXED
+    assert not np.array_equal(img, frame.img)  # the mixed kernel did run
 
 
 def test_frame_matches_stages(frame, ctx):
