@@ -1,28 +1,44 @@
-"""GI frame benchmark on one MI355X.
+"""GI benchmarks on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1080p]
                     [--form fp32|mixed|both] [--no-cpu]
 
-A step is one GI frame of the many-lights hot path on inputs already resident
-in HBM: VPL extraction from the RSM, ISM atlas clear + point splat, pull-push,
-and interleaved final gathering (mlgi_frame). The timed region is exactly K
-frames bracketed by a device synchronize on both sides; ms_per_step is wall
-time / K. Rank 0 prints one JSON line.
+Default (workload 1080p): a step is one GI frame of the many-lights hot path
+on inputs already resident in HBM -- VPL extraction from the RSM, ISM atlas
+clear + point splat, pull-push, interleaved final gathering -- at 1920x1080,
+4,096 VPLs (128^2 RSM), 1M splat points into 4,096 64^2 ISMs of a 4096^2
+atlas, 4 pull-push levels, 8x8 interleave, 40k-triangle procedural room.
+Two forms run on the same inputs:
+  fp32   the plain FP32 form through the single-frame entry point (mlgi_frame)
+  mixed  the mixed-precision form through the persistent batched kernels
+         (mlgi_batch_run of the one frame)
+The timed region is exactly K frames of a form bracketed by a device
+synchronize on both sides; ms_per_step is wall time / K; HIP events on the
+context stream bracket every stage of every timed frame. `value` is the
+mixed form's ms/frame when it runs. The mixed batch is also replayed from a
+captured HIP graph (reported as graph_ms_per_frame).
 
-The default workload is the 1920x1080, 4,096-VPL configuration (40k-triangle
-procedural room, 128^2 RSM, 1M splat points into 4,096 64^2 ISMs of a 4096^2
-atlas, 4 pull-push levels, 8x8 interleave, 16x16 tiles).
+roofline: for the kernel with the largest mean duration in the timed frames:
+pull-push against HBM (6 bytes per atlas texel: the u32 atlas read once, the
+unorm16 ISM written once), the final gather against the FP32 vector peak (84
+flops per lit pixel-VPL pair, pairs counted on the device), the splat against
+HBM (16 bytes per point record read). Every kernel's figure is under
+"kernels"; "traffic" is the HBM bytes per launch measured with rocprofv3
+counters (profiles/traffic_<workload>.json), null until measured.
+cpu_baseline: the oracle's FP32 instantiation of the same four stages on the
+host for one full frame (final gathering on --cpu-threads threads).
 
-roofline: for the kernel with the largest mean duration (HIP events on the
-context stream over the timed frames): pull-push against HBM (6 bytes per
-atlas texel: the u32 atlas read once, the unorm16 ISM written once); the
-final gather against the FP32 vector peak (84 flops per lit pixel-VPL pair,
-pairs counted on the device). Every kernel's figure is under "kernels";
-"traffic" is the HBM bytes per launch measured with rocprofv3 counters
-(profiles/traffic_<workload>.json), null until measured.
-cpu_baseline: the oracle's FP32 instantiation of the same four stages run on
-the host for one full frame (final gathering threaded, the rest on one
-thread), scaled to ms/frame.
+Other workloads (one JSON line each, not the default):
+  small     512x512, 1,024 VPLs, 256k points, 2048^2 atlas, 4x4 interleave
+  4k        3840x2160, 16,384 VPLs, 8M points, 16,384 32^2 ISMs, 5 levels,
+            160k triangles, mixed batched form, light pose changed every
+            frame (device RSM) and the frame replayed from a HIP graph
+  batch64   64 independent frames of the 40k scene (resolutions 128^2..1024^2,
+            256..4,096 VPLs, random light poses), autotuned, one persistent
+            batch replayed from a HIP graph; ms per batch and per frame
+  serve     the request server over 3 resident scenes (4k/40k/160k triangles),
+            Poisson arrivals at --rate req/s for --seconds, HBM-budget
+            admission; req/s and p50/p99 latency
 """
 from __future__ import annotations
 
@@ -40,49 +56,62 @@ sys.path.insert(0, str(ROOT / "many-lights-gi_amd"))
 FLOPS_PER_LIT_PAIR = 84
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+DATA = "synthetic: seeded procedural room, ray-cast G-buffer/RSM, area-proportional points"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(ctx, wl, inp, precision: int, steps: int, warmup: int):
+def upload_frame(ctx, inp):
     import mlgi
 
-    p = wl.params(precision=precision)
-    st = mlgi.GIStage(ctx, p, wl.n_vpl, wl.width, wl.height)
     rsm = mlgi.rsm_desc(ctx.upload(inp.rsm_diffuse), ctx.upload(inp.rsm_normal), ctx.upload(inp.rsm_depth),
                         inp.inv_bvp)
     pts = mlgi.points_desc(ctx.upload(inp.points), ctx.upload(inp.bin_offsets))
     gb = mlgi.gbuffer_desc(ctx.upload(inp.gb_depth), ctx.upload(inp.gb_normal), inp.vp_inv)
+    return rsm, pts, gb
+
+
+def run_form(ctx, wl, inp, form: str, steps: int, warmup: int):
+    """fp32: plain single-frame path; mixed: batched path (batch of one)."""
+    import mlgi
+
+    prec = mlgi.PRECISION_FP32 if form == "fp32" else mlgi.PRECISION_MIXED
+    st = mlgi.GIStage(ctx, wl.params(precision=prec), wl.n_vpl, wl.width, wl.height)
+    rsm, pts, gb = upload_frame(ctx, inp)
+    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb)]) if form == "mixed" else None
+    step = batch.run if batch else (lambda: st.process(rsm, pts, gb))
     for _ in range(warmup):
-        st.process(rsm, pts, gb)
+        step()
     ctx.sync()
-    # timed region: K frames between device synchronisations; HIP events on
-    # the context stream bracket every stage of every frame
     ctx.profile(True)
     ctx.profile_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
-        st.process(rsm, pts, gb)
+        step()
     ctx.sync()
     wall_ms = (time.perf_counter() - t0) * 1e3 / steps
     stages = ctx.profile_read()
     ctx.profile(False)
-    lit = st.count_lit_pairs(gb)
-    return {"st": st, "rsm": rsm, "pts": pts, "gb": gb, "wall_ms": wall_ms, "stages": stages, "lit": lit}
+    out = {"st": st, "wall_ms": wall_ms, "stages": stages, "lit": st.count_lit_pairs(gb),
+           "path": "batched (mlgi_batch_run)" if batch else "single frame (mlgi_frame)"}
+    if batch:
+        g = batch.capture()
+        for _ in range(warmup):
+            g.launch()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.launch()
+        ctx.sync()
+        out["graph_ms"] = (time.perf_counter() - t0) * 1e3 / steps
+        g.close()
+        batch.close()
+    return out
 
 
 def kernel_rooflines(wl, stages, lit):
-    """Per-kernel roofline figures from the live stage times.
-
-    pull-push: reads the u32 atlas once and writes the unorm16 ISM once,
-      6 bytes per atlas texel, against HBM.
-    splat: reads every 16-byte point record once (lower bound: the accepted
-      atomicMins add at most 4 x 4 bytes per point), against HBM.
-    final gather: 84 flops per lit pixel-VPL pair, against the FP32 vector
-      peak.
-    """
     a2 = wl.atlas_side * wl.atlas_side
     out = {}
     t = stages["pullpush_ms"] * 1e-3
@@ -107,7 +136,7 @@ def kernel_rooflines(wl, stages, lit):
     return out
 
 
-def cpu_baseline(wl, inp, rec, r16, threads: int):
+def cpu_baseline(wl, inp, threads: int):
     import numpy as np
     import mlgi
     import oracle
@@ -123,10 +152,10 @@ def cpu_baseline(wl, inp, rec, r16, threads: int):
     atlas = oracle.ism_splat(inp.points, inp.bin_offsets, pos, nrm, wl.atlas_side, p.z_far)
     t["splat"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    r16c = oracle.pullpush(atlas, wl.ism_side, wl.levels, p.z_far)
+    r16 = oracle.pullpush(atlas, wl.ism_side, wl.levels, p.z_far)
     t["pullpush"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    oracle.final_gather(inp.gb_depth, inp.gb_normal, inp.vp_inv, pos, nrm, col, r16c, wl.atlas_side, p.z_far,
+    oracle.final_gather(inp.gb_depth, inp.gb_normal, inp.vp_inv, pos, nrm, col, r16, wl.atlas_side, p.z_far,
                         p.gi_intensity_factor, p.vpl_clamping_value, wl.interleave, threads=threads,
                         dtype=np.float32)
     t["gather"] = time.perf_counter() - t0
@@ -137,15 +166,222 @@ def cpu_baseline(wl, inp, rec, r16, threads: int):
             "stages_s": {k: round(v, 4) for k, v in t.items()}}
 
 
+def config_of(wl, form):
+    return {"workload": wl.name, "width": wl.width, "height": wl.height, "vpls": wl.n_vpl,
+            "interleave": wl.interleave, "points": wl.n_points, "atlas": wl.atlas_side, "ism": wl.ism_side,
+            "pull_push_levels": wl.levels, "tile": f"{wl.tile[0]}x{wl.tile[1]}", "form": form,
+            "triangles": wl.n_random + 12}
+
+
+def frame_bench(a, ctx, wl):
+    from mlgi import workloads as W
+
+    log(f"[bench] building inputs for {wl.name} ...")
+    t0 = time.perf_counter()
+    inp = W.build_inputs(wl)
+    log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s")
+    forms = {"fp32": ["fp32"], "mixed": ["mixed"], "both": ["fp32", "mixed"]}[a.form]
+    res = {}
+    for form in forms:
+        log(f"[bench] form {form}: warmup {a.warmup}, steps {a.steps}")
+        res[form] = run_form(ctx, wl, inp, form, a.steps, a.warmup)
+        log(f"[bench] {form}: {res[form]['wall_ms']:.4f} ms/frame, stages {res[form]['stages']}")
+    head = "mixed" if "mixed" in res else "fp32"
+    h = res[head]
+    kernels = kernel_rooflines(wl, h["stages"], h["lit"])
+    dominant = max(kernels, key=lambda k: kernels[k]["kernel_ms"])
+    roof = dict(kernels[dominant])
+    roof["kernel"] = dominant
+    roof["traffic"] = None
+    tpath = ROOT / "profiles" / f"traffic_{wl.name}.json"
+    if tpath.exists():
+        tr = json.loads(tpath.read_text()).get(dominant)
+        if tr:
+            roof["traffic"] = tr["bytes_per_launch"]
+    forms_out = {}
+    for k, v in res.items():
+        forms_out[k] = {"ms_per_frame": round(v["wall_ms"], 4), "path": v["path"],
+                        "stages_ms": {s: round(x, 4) for s, x in v["stages"].items() if s != "frames"}}
+        if "graph_ms" in v:
+            forms_out[k]["graph_ms_per_frame"] = round(v["graph_ms"], 4)
+    out = {
+        "metric": "GI ms/frame 1080p·4096 VPL (mixed vs FP32 HIP)",
+        "value": round(h["wall_ms"], 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(h["wall_ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16x2+fp32" if head == "mixed" else "fp32",
+        "data": DATA,
+        "config": config_of(wl, head),
+        "forms": forms_out,
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    if "fp32" in res and "mixed" in res:
+        out["mixed_speedup_vs_fp32"] = round(res["fp32"]["wall_ms"] / res["mixed"]["wall_ms"], 4)
+    if not a.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(wl, inp, a.cpu_threads)
+    return out
+
+
+def stress_bench(a, ctx, wl):
+    """4K stress: the light pose changes every frame; the frame's RSM is
+    rendered on the device and the mixed batched frame replays from a graph."""
+    import numpy as np
+    import mlgi
+    from mlgi import workloads as W
+
+    log(f"[bench] building inputs for {wl.name} ...")
+    inp = W.build_inputs(wl)
+    scene = W._scene(wl.seed, wl.n_random)
+    dev = scene.upload(ctx)
+    st = mlgi.GIStage(ctx, wl.params(precision=mlgi.PRECISION_MIXED), wl.n_vpl, wl.width, wl.height)
+    _, pts, gb = upload_frame(ctx, inp)
+    r = wl.rsm
+    dif, nrm, dep = ctx.empty((r, r, 3), np.float32), ctx.empty((r, r, 3), np.float32), ctx.empty((r, r), np.float32)
+    rsm = mlgi.rsm_desc(dif, nrm, dep, inp.inv_bvp)
+    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb)])
+    g = batch.capture()
+
+    def frame(i):
+        inv = dev.render_rsm(mlgi.Camera.light(1000 + i), dif, nrm, dep)
+        batch.update_light(0, inv)
+        g.launch()
+
+    for i in range(a.warmup):
+        frame(i)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        frame(a.warmup + i)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    g.close()
+    batch.close()
+    dev.close()
+    return {"metric": "GI ms/frame 4K·16384 VPL, light pose per frame (mixed batched, graph replay)",
+            "value": round(ms, 4), "unit": "ms/frame", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp16x2+fp32", "data": DATA,
+            "config": dict(config_of(wl, "mixed"), light="new pose per frame, device RSM")}
+
+
+def batch_bench(a, ctx):
+    """64 frames of the 40k scene with random shapes and light poses, autotuned,
+    one persistent batch replayed from a HIP graph."""
+    import numpy as np
+    import mlgi
+    from mlgi import workloads as W
+
+    rng = np.random.default_rng(2024)
+    scene = W._scene(1, 40_000)
+    dev = scene.upload(ctx)
+    sizes = [128, 256, 512, 1024]
+    vpls = [256, 1024, 2048, 4096]
+    points, gbufs, frames = {}, {}, []
+    for i in range(64):
+        s = int(rng.choice(sizes))
+        n = int(rng.choice(vpls))
+        ism = 32 if n <= 1024 else 64
+        side = mlgi.ism_indices_1d(n) * ism
+        if n not in points:
+            rec, off = scene.sample_points(1_000_000, n, n, 11)
+            points[n] = mlgi.points_desc(ctx.upload(rec), ctx.upload(off))
+        if s not in gbufs:
+            d, nm = ctx.empty((s, s), np.float32), ctx.empty((s, s, 3), np.float32)
+            inv = dev.render_gbuffer(mlgi.Camera.default(1.0), d, nm)
+            gbufs[s] = (d, nm, mlgi.gbuffer_desc(d, nm, inv))
+        p = mlgi.default_params(atlas_side=side, pull_push_levels=3 if ism == 32 else 4, vpl_end_index=n,
+                                z_far=50.0, precision=mlgi.PRECISION_MIXED)
+        st = mlgi.GIStage(ctx, p, n, s, s)
+        dif, nrm, dep = (ctx.empty((128, 128, 3), np.float32), ctx.empty((128, 128, 3), np.float32),
+                         ctx.empty((128, 128), np.float32))
+        inv = dev.render_rsm(mlgi.Camera.light(5000 + i), dif, nrm, dep)
+        rsm = mlgi.rsm_desc(dif, nrm, dep, inv)
+        st.rsm_buffers = (dif, nrm, dep)
+        tune = mlgi.autotune(ctx, st, rsm, points[n], gbufs[s][2])
+        frames.append((st, rsm, points[n], gbufs[s][2], s, n, tune))
+    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb) for st, rsm, pts, gb, _, _, _ in frames])
+    g = batch.capture()
+    for _ in range(a.warmup):
+        g.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g.launch()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    tuned = {}
+    for _, _, _, _, s, n, tune in frames:
+        tuned[f"{s}x{s}/{n}"] = {"interleave": tune["interleave"], "tile": "%dx%d" % tuple(tune["tile"])}
+    g.close()
+    batch.close()
+    dev.close()
+    return {"metric": "GI ms per batch of 64 mixed-shape frames (mixed, persistent batched, graph replay)",
+            "value": round(ms, 4), "unit": "ms/batch", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": round(ms, 4), "ms_per_frame": round(ms / 64, 4), "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp16x2+fp32", "data": DATA,
+            "config": {"workload": "batch64", "frames": 64, "resolutions": sizes, "vpls": vpls,
+                       "triangles": 40_012, "points": 1_000_000, "autotuned": tuned}}
+
+
+def serve_bench(a, ctx):
+    import numpy as np
+    import mlgi
+    from mlgi import workloads as W
+
+    scenes = [(W._scene(1, 4_000), 256_000, 21), (W._scene(1, 40_000), 1_000_000, 22),
+              (W._scene(1, 160_000), 4_000_000, 23)]
+    sizes = [128, 256, 512, 1024]
+    vpls = [256, 1024, 2048, 4096]
+    log("[bench] server create (resident scenes, point binnings, G-buffers, tuning) ...")
+    t0 = time.perf_counter()
+    srv = mlgi.Server(ctx, scenes, vpls, [(s, s) for s in sizes], hbm_budget=a.budget_gib << 30,
+                      max_inflight=64, max_batch=32, max_queued_batches=3)
+    log(f"[bench] server ready in {time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(7)
+    t, reqs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / a.rate)
+        if t >= a.seconds:
+            break
+        s = int(rng.choice(sizes))
+        reqs.append((int(rng.integers(3)), s, s, int(rng.choice(vpls)), int(rng.integers(1 << 40)), t))
+    log(f"[bench] serving {len(reqs)} requests over {a.seconds:.0f} s ...")
+    st, lat = srv.run(reqs)
+    srv.close()
+    return {"metric": "served GI req/s + p99 latency (3 resident scenes, Poisson arrivals, mixed form)",
+            "value": round(st["req_per_s"], 3), "unit": "req/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
+            "ms_per_step": round(st["latency_mean_ms"], 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp16x2+fp32", "data": DATA,
+            "latency_ms": {"p50": round(st["latency_p50_ms"], 3), "p99": round(st["latency_p99_ms"], 3),
+                           "mean": round(st["latency_mean_ms"], 3), "max": round(st["latency_max_ms"], 3)},
+            "server": {k: st[k] for k in ("completed", "rejected", "batches", "mean_batch", "max_inflight_seen",
+                                          "peak_bytes", "wall_s")},
+            "config": {"workload": "serve", "arrival_rate_per_s": a.rate, "seconds": a.seconds,
+                       "scenes_triangles": [4_012, 40_012, 160_012],
+                       "scene_points": [256_000, 1_000_000, 4_000_000], "resolutions": sizes, "vpls": vpls,
+                       "hbm_budget_gib": a.budget_gib, "rsm": 128}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="1080p")
+    ap.add_argument("--workload", default="1080p", choices=["1080p", "small", "4k", "batch64", "serve"])
     ap.add_argument("--form", default="both", choices=["fp32", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate (req/s)")
+    ap.add_argument("--seconds", type=float, default=60.0, help="serve: length of the arrival trace")
+    ap.add_argument("--budget-gib", type=int, default=16, help="serve: HBM budget of admitted requests")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the frames, copy this many MiB device-to-device (a plain copy kernel of "
                          "known size for scaling the HBM traffic counters)")
@@ -160,14 +396,6 @@ def main() -> int:
     import mlgi
     from mlgi import workloads as W
 
-    wl = W.WORKLOADS[a.workload]
-    log(f"[bench] building inputs for {wl.name} ...")
-    t0 = time.perf_counter()
-    inp = W.build_inputs(wl)
-    log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s")
-    forms = {"fp32": [mlgi.PRECISION_FP32], "mixed": [mlgi.PRECISION_MIXED],
-             "both": [mlgi.PRECISION_FP32, mlgi.PRECISION_MIXED]}[a.form]
-    res = {}
     with mlgi.Context(0) as ctx:
         if a.calibrate_copy_mib > 0:
             n = a.calibrate_copy_mib << 20
@@ -177,53 +405,14 @@ def main() -> int:
             log(f"[bench] calibration copy of {a.calibrate_copy_mib} MiB done")
             src.free()
             dst.free()
-        for prec in forms:
-            name = "fp32" if prec == mlgi.PRECISION_FP32 else "mixed"
-            log(f"[bench] form {name}: warmup {a.warmup}, steps {a.steps}")
-            r = run_form(ctx, wl, inp, prec, a.steps, a.warmup)
-            r["rec"] = r["st"].vpl_processor.host_records()
-            r["r16"] = r["st"].ism.pushpull_result.numpy()
-            r["img"] = r["st"].gi_buffer.numpy()
-            log(f"[bench] {name}: {r['wall_ms']:.4f} ms/frame, stages {r['stages']}")
-            res[name] = r
-    head = "mixed" if "mixed" in res else "fp32"
-    h = res[head]
-    kernels = kernel_rooflines(wl, h["stages"], h["lit"])
-    dominant = max(kernels, key=lambda k: kernels[k]["kernel_ms"])
-    roof = dict(kernels[dominant])
-    roof["kernel"] = dominant
-    roof["traffic"] = None
-    tpath = ROOT / "profiles" / f"traffic_{wl.name}.json"
-    if tpath.exists():
-        tr = json.loads(tpath.read_text()).get(dominant)
-        if tr:
-            roof["traffic"] = tr["bytes_per_launch"]
-    out = {
-        "metric": "GI ms/frame 1080p·4096 VPL (mixed vs FP32 HIP)",
-        "value": round(h["wall_ms"], 4),
-        "unit": "ms/frame",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(h["wall_ms"], 4),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp16x2+fp32" if head == "mixed" else "fp32",
-        "data": "synthetic: seeded procedural room, ray-cast G-buffer/RSM, area-proportional points",
-        "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "vpls": wl.n_vpl,
-                   "interleave": wl.interleave, "points": wl.n_points, "atlas": wl.atlas_side,
-                   "ism": wl.ism_side, "pull_push_levels": wl.levels, "tile": f"{wl.tile[0]}x{wl.tile[1]}",
-                   "form": head, "triangles": wl.n_random + 12},
-        "forms": {k: {"ms_per_frame": round(v["wall_ms"], 4),
-                      "stages_ms": {s: round(x, 4) for s, x in v["stages"].items() if s != "frames"}}
-                  for k, v in res.items()},
-        "roofline": roof,
-        "kernels": kernels,
-    }
-    if not a.no_cpu:
-        log("[bench] cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(wl, inp, h["rec"], h["r16"], a.cpu_threads)
+        if a.workload == "batch64":
+            out = batch_bench(a, ctx)
+        elif a.workload == "serve":
+            out = serve_bench(a, ctx)
+        elif a.workload == "4k":
+            out = stress_bench(a, ctx, W.WORKLOADS["4k"])
+        else:
+            out = frame_bench(a, ctx, W.WORKLOADS[a.workload])
     print(json.dumps(out), flush=True)
     return 0
 

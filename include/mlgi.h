@@ -234,6 +234,11 @@ int mlgi_batch_create(mlgi_context* ctx, const mlgi_frame_desc* frames, int32_t 
                       mlgi_batch** out);
 /* Enqueue all frames (asynchronous). */
 int mlgi_batch_run(mlgi_context* ctx, mlgi_batch* batch);
+/* Replace frame i's inverse(bias*P*V) in the device frame table
+ * (stream-ordered), e.g. after rendering a new light pose into the frame's
+ * RSM buffers; runs and graph replays after this call use the new matrix. */
+int mlgi_batch_update_light(mlgi_context* ctx, mlgi_batch* batch, int32_t frame,
+                            const float* inv_bvp16);
 /* Capture one mlgi_batch_run into an instantiated HIP graph. */
 int mlgi_batch_capture(mlgi_context* ctx, mlgi_batch* batch, mlgi_graph** graph);
 void mlgi_batch_destroy(mlgi_batch* batch);

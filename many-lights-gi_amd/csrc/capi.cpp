@@ -696,6 +696,8 @@ int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::v
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
   *plan = BatchPlan{};
   plan->count = count;
+  plan->row_of.assign(count, 0);
+  for (int r = 0; r < count; ++r) plan->row_of[order[r]] = r;
   table->assign(count, mlgi::BatchFrame{});
   uint64_t tv = 0, tc = 0, tb = 0;
   for (int r = 0; r < count; ++r) {
@@ -806,6 +808,15 @@ int mlgi_batch_run(mlgi_context* c, mlgi_batch* b) {
   if (!c) return MLGI_ERR_INVALID_ARGUMENT;
   if (!b || !b->dev || b->ctx != c) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "batch is NULL or foreign");
   return batch_enqueue(c, b->dev, b->plan);
+}
+
+int mlgi_batch_update_light(mlgi_context* c, mlgi_batch* b, int32_t frame, const float* inv_bvp16) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!b || !b->dev || b->ctx != c || !inv_bvp16) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (frame < 0 || frame >= b->plan.count) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "frame %d out of range", frame);
+  mlgi::BatchFrame* row = b->dev + b->plan.row_of[frame];
+  return hip_check(c, hipMemcpyAsync(&row->inv_bvp, inv_bvp16, sizeof(mlgi::Mat4), hipMemcpyHostToDevice, c->stream),
+                   "hipMemcpyAsync(light)");
 }
 
 int mlgi_batch_capture(mlgi_context* c, mlgi_batch* b, mlgi_graph** graph) {

@@ -78,6 +78,7 @@ struct Scope {  // records a timed span on the context stream when profiling
 // kernel (precision a).
 struct BatchPlan {
   int count = 0;
+  std::vector<int> row_of;  // caller's frame index -> table row
   uint32_t total_vpl = 0, total_clear = 0, total_bins = 0;
   struct Run {
     int begin, count;

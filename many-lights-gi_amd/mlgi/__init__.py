@@ -484,6 +484,13 @@ class Batch:
     def run(self) -> None:
         self.ctx.check(lib.mlgi_batch_run(self.ctx.h, self.h), "mlgi_batch_run")
 
+    def update_light(self, frame: int, inv_bvp) -> None:
+        """New inverse(bias*P*V) for frame `frame` (stream-ordered)."""
+        m = np.ascontiguousarray(inv_bvp, np.float32)
+        self._light_staging = m  # the copy is asynchronous: keep the source alive
+        self.ctx.check(lib.mlgi_batch_update_light(self.ctx.h, self.h, frame, m.ctypes.data),
+                       "mlgi_batch_update_light")
+
     def capture(self) -> "FrameGraph":
         g = C.c_void_p()
         self.ctx.check(lib.mlgi_batch_capture(self.ctx.h, self.h, C.byref(g)), "mlgi_batch_capture")

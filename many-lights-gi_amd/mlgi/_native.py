@@ -188,6 +188,7 @@ SIGNATURES = {
     "mlgi_batch_run": (C.c_int, [vp, vp]),
     "mlgi_batch_capture": (C.c_int, [vp, vp, P(vp)]),
     "mlgi_batch_destroy": (None, [vp]),
+    "mlgi_batch_update_light": (C.c_int, [vp, vp, C.c_int32, vp]),
     "mlgi_autotune": (C.c_int, [vp, P(FrameDesc), C.c_int32, P(TuneResult)]),
     "mlgi_server_create": (C.c_int, [vp, P(ServerConfig), P(ServerScene), C.c_int32, P(vp)]),
     "mlgi_server_run": (C.c_int, [vp, P(Request), C.c_int32, P(ServerStats), P(C.c_double)]),
