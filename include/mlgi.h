@@ -151,6 +151,7 @@ typedef struct mlgi_stage_times {
   float final_gather_ms;
   float frame_ms;
   int32_t frames; /* frames accumulated since the last reset */
+  float blur_ms;  /* mlgi_gi_blur */
 } mlgi_stage_times;
 
 /* ---------------------------------------------------------------- context */
@@ -198,6 +199,14 @@ int mlgi_final_gather(mlgi_context* ctx, const mlgi_gbuffer* gb, const mlgi_vpls
 int mlgi_frame(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_points* pts,
                const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls,
                mlgi_atlas* atlas, mlgi_ism_depth* ism, mlgi_image* out);
+
+/* Geometry-aware separable blur of the irradiance (GIStage::blur,
+ * GIStage.cpp:308-344; gi/gi_blur.frag): x pass then y pass with weights
+ * from the G-buffer's face normals and linear depth. projection16 is the
+ * camera's P (column-major; only P[2][2] and P[3][2] are used). gi and out
+ * must be distinct buffers of the G-buffer's size. Asynchronous. */
+int mlgi_gi_blur(mlgi_context* ctx, const mlgi_gbuffer* gb, const float* projection16,
+                 const mlgi_image* gi, mlgi_image* out);
 
 /* ------------------------------------------------------------ HIP graphs */
 typedef struct mlgi_graph mlgi_graph;

@@ -43,6 +43,7 @@ SOURCES = [
     ("pullpush.hip", HIP_FLAGS + EXACT),
     ("final_gather.hip", HIP_FLAGS),
     ("raycast.hip", HIP_FLAGS + EXACT),
+    ("blur.hip", HIP_FLAGS + EXACT),
     ("capi.cpp", HIP_FLAGS),
     ("server.cpp", HIP_FLAGS),
     ("scene.cpp", ["-ffp-contract=off"]),

@@ -444,6 +444,22 @@ int mlgi_final_gather(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
   return hip_check(c, mlgi::launch_final_gather(c->stream, a, p->precision, tw, th, fetch), "final_gather");
 }
 
+int mlgi_gi_blur(mlgi_context* c, const mlgi_gbuffer* gb, const float* projection16, const mlgi_image* gi,
+                 mlgi_image* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!gb || !gb->depth || !gb->face_normal || !projection16 || !gi || !gi->rgb || !out || !out->rgb)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (gb->width <= 0 || gb->height <= 0 || gb->width > 16384 || gb->height > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "gbuffer size %dx%d out of range", gb->width, gb->height);
+  if (gi->width != gb->width || gi->height != gb->height || out->width != gb->width || out->height != gb->height)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "images must match the gbuffer size");
+  if (gi->rgb == out->rgb) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "the blur cannot run in place");
+  Scope sc(c, kBlur);
+  return hip_check(c, mlgi::launch_gi_blur(c->stream, gi->rgb, gb->face_normal, gb->depth, gb->width, gb->height,
+                                           projection16, out->rgb),
+                   "gi_blur");
+}
+
 int mlgi_frame(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_points* pts,
                const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls, mlgi_atlas* atlas,
                mlgi_ism_depth* ism, mlgi_image* out) {
@@ -585,6 +601,7 @@ int mlgi_profile_read(mlgi_context* c, mlgi_stage_times* out) {
   out->ism_splat_ms = avg(kSplat);
   out->pullpush_ms = avg(kPullPush);
   out->final_gather_ms = avg(kGather);
+  out->blur_ms = avg(kBlur);
   out->frame_ms = avg(kFrame);
   out->frames = c->frames;
   return MLGI_OK;

@@ -33,7 +33,7 @@ struct mlgi_context {
 
 namespace mlgi_capi {
 
-enum Stage { kVpl = 0, kClear, kSplat, kPullPush, kGather, kFrame, kStages };
+enum Stage { kVpl = 0, kClear, kSplat, kPullPush, kGather, kFrame, kBlur, kStages };
 
 int fail(mlgi_context* c, int code, const char* fmt, ...);
 int hip_check(mlgi_context* c, hipError_t e, const char* what);

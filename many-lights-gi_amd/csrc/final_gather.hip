@@ -246,14 +246,14 @@ __device__ __forceinline__ LitConsts lit_consts(const GatherArgs& a) {
                    a.shadow_k1, a.shadow_k2};
 }
 
-// Contribution of one lit pair (n_v . d > 0, n_p . d < 0) to the pixel's sums
-// (final_gathering.comp:100-131): geometry term, clamp, ISM visibility.
+// Weight of one lit pair (n_v . d > 0, n_p . d < 0): geometry term, clamp
+// and ISM visibility (final_gathering.comp:100-131). (px, py, pz) is the
+// pixel's position; x, y, depth identify the pixel for the FP64 tie check.
 template <bool kMixed>
-__device__ __forceinline__ void accumulate_pair(const GatherArgs& a, const Pixel& p, const VplTest& vt,
-                                                const VplLds& v, int vi, float ndot, float pdot,
-                                                bool shadow, const LitConsts& lc, float& ar, float& ag,
-                                                float& ab) {
-  const float dx = p.fx - vt.px, dy = p.fy - vt.py, dz = p.fz - vt.pz;
+__device__ __forceinline__ float pair_weight(const GatherArgs& a, float px, float py, float pz, int x, int y,
+                                             float depth, const VplTest& vt, const VplLds& v, int vi, float ndot,
+                                             float pdot, bool shadow, const LitConsts& lc) {
+  const float dx = px - vt.px, dy = py - vt.py, dz = pz - vt.pz;
   const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
   const float inv = __builtin_amdgcn_rsqf(d2);
   const float inv2 = inv * inv;
@@ -275,8 +275,8 @@ __device__ __forceinline__ void accumulate_pair(const GatherArgs& a, const Pixel
       tx = (int)(cx - fx);
       ty = (int)(cy - fy);
       if (fmaxf(fabsf(fx - 0.5f), fabsf(fy - 0.5f)) > 0.5f - kTie) {
-        const int t2 = refine_texel(a.vp_inv, a.width, a.height, a.ism_side, p.x, p.y, p.depth, vt.px,
-                                    vt.py, vt.pz, vt.nx, vt.ny, vt.nz);
+        const int t2 = refine_texel(a.vp_inv, a.width, a.height, a.ism_side, x, y, depth, vt.px, vt.py, vt.pz,
+                                    vt.nx, vt.ny, vt.nz);
         tx = (int)(int16_t)(t2 & 0xFFFF);
         ty = t2 >> 16;
       }
@@ -291,21 +291,38 @@ __device__ __forceinline__ void accumulate_pair(const GatherArgs& a, const Pixel
     t = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
     g *= t * t * __builtin_fmaf(-2.0f, t, 3.0f);
   }
+  return g;
+}
+
+// The VPL's flux as floats (MIXED: packed halves, (r, g) in cr, (b, 0) in cg).
+template <bool kMixed>
+__device__ __forceinline__ void vpl_flux(const VplLds& v, float& r, float& g, float& b) {
   if (kMixed) {
-    // flux as packed halves: (r, g) in cr, (b, 0) in cg
     const uint32_t rg = __builtin_bit_cast(uint32_t, v.cr);
     const uint32_t bb = __builtin_bit_cast(uint32_t, v.cg);
-    const float fr = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg & 0xFFFFu));
-    const float fg = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg >> 16));
-    const float fb = (float)__builtin_bit_cast(_Float16, (uint16_t)(bb & 0xFFFFu));
-    ar = __builtin_fmaf(fr, g, ar);
-    ag = __builtin_fmaf(fg, g, ag);
-    ab = __builtin_fmaf(fb, g, ab);
+    r = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg & 0xFFFFu));
+    g = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg >> 16));
+    b = (float)__builtin_bit_cast(_Float16, (uint16_t)(bb & 0xFFFFu));
   } else {
-    ar = __builtin_fmaf(v.cr, g, ar);
-    ag = __builtin_fmaf(v.cg, g, ag);
-    ab = __builtin_fmaf(v.cb, g, ab);
+    r = v.cr;
+    g = v.cg;
+    b = v.cb;
   }
+}
+
+// Contribution of one lit pair to the pixel's running sums (one fused
+// multiply-add per channel).
+template <bool kMixed>
+__device__ __forceinline__ void accumulate_pair(const GatherArgs& a, const Pixel& p, const VplTest& vt,
+                                                const VplLds& v, int vi, float ndot, float pdot,
+                                                bool shadow, const LitConsts& lc, float& ar, float& ag,
+                                                float& ab) {
+  const float g = pair_weight<kMixed>(a, p.fx, p.fy, p.fz, p.x, p.y, p.depth, vt, v, vi, ndot, pdot, shadow, lc);
+  float fr, fg, fb;
+  vpl_flux<kMixed>(v, fr, fg, fb);
+  ar = __builtin_fmaf(fr, g, ar);
+  ag = __builtin_fmaf(fg, g, ag);
+  ab = __builtin_fmaf(fb, g, ab);
 }
 
 // A staged VPL record in the form the precision uses (MIXED: flux packed to
@@ -423,18 +440,71 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
 
 // Batched gather: persistent wave64s over the (frame, class, strip) items of
 // all frames of one precision. A strip is 2^w x 64/2^w pixels of one
-// interleave class (w per frame, picked by the autotuner). Each wave stages
-// its own 64-record VPL chunks in LDS and synchronises only within itself, so
-// frames of different shapes share workgroups freely. The per-pixel
-// arithmetic and summation order are the single-frame kernel's, so a frame
-// gives the same bits alone or in any batch.
+// interleave class (w per frame, picked by the autotuner). Each wave works in
+// its own LDS region and synchronises only within itself, so frames of
+// different shapes share workgroups freely.
+//
+// Lit pairs are compacted: for every VPL that survives culling, the lanes
+// whose pixel it lights append (pixel, VPL, n_v.d, n_p.d) to the wave's pair
+// queue (ballot + rank within the ballot); every 64 queued pairs are
+// evaluated by the 64 lanes at once, each adding its weighted flux to its
+// pixel's sums in LDS. The lanes of a wave thus spend the expensive part
+// (projection, ISM fetch, smoothstep) on lit pairs only, instead of waiting
+// on each other's hemisphere tests. Queue order is VPL order, and the LDS
+// adds of one pixel are applied in queue order, so a frame gives the same
+// bits alone or in any batch.
+struct PixelLds {
+  float fx, fy, fz;
+  uint32_t xy;  // x | y << 16
+  float depth;
+  float pad[3];
+};
+struct PairLds {
+  uint32_t pj;  // pixel lane | VPL index in chunk << 8
+  float ndot, pdot, pad;
+};
+struct WaveLds {
+  VplLds vpl[64];
+  PixelLds pix[64];
+  float acc[64][4];
+  PairLds queue[128];
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Evaluates queued pairs [0, count) (count <= 64), one per lane.
+template <bool kMixed>
+__device__ __forceinline__ void drain_pairs(const GatherArgs& a, WaveLds& w, int count, int first_vpl, bool shadow,
+                                            const LitConsts& lc) {
+  const int lane = threadIdx.x & 63;
+  if (lane < count) {
+    const PairLds e = w.queue[lane];
+    const int pl = e.pj & 0xFF, j = e.pj >> 8;
+    const PixelLds px = w.pix[pl];
+    const VplLds v = w.vpl[j];
+    const VplTest vt{v.px, v.py, v.pz, v.plane, v.nx, v.ny, v.nz};
+    const float g = pair_weight<kMixed>(a, px.fx, px.fy, px.fz, (int)(px.xy & 0xFFFF), (int)(px.xy >> 16), px.depth,
+                                        vt, v, first_vpl + j, e.ndot, e.pdot, shadow, lc);
+    float fr, fg, fb;
+    vpl_flux<kMixed>(v, fr, fg, fb);
+    __hip_atomic_fetch_add(&w.acc[pl][0], fr * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_fetch_add(&w.acc[pl][1], fg * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_fetch_add(&w.acc[pl][2], fb * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
+  wave_sync();
+}
+
 template <bool kMixed>
 __global__ __launch_bounds__(256) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
                                                            uint32_t total) {
-  __shared__ VplLds vtiles[4][64];
+  __shared__ WaveLds lds[4];
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  VplLds* vtile = vtiles[wid];
+  WaveLds& w = lds[wid];
   const uint32_t slots = (gridDim.x / kBatchXcds) * 4;
   const uint32_t slot = (blockIdx.x / kBatchXcds) * 4 + wid;
   const XcdRange xr = xcd_range(total, slot, slots);
@@ -456,47 +526,62 @@ __global__ __launch_bounds__(256) void batch_gather_kernel(const BatchFrame* __r
     const int first = k * a.per_class;
     const LitConsts lc = lit_consts(a);
     const bool shadow = a.shadowing != 0;
-    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    w.pix[lane] = PixelLds{p.fx, p.fy, p.fz, (uint32_t)(p.x & 0xFFFF) | ((uint32_t)p.y << 16), p.depth, {}};
     float tr = 0.0f, tg = 0.0f, tb = 0.0f, er = 0.0f, eg = 0.0f, eb = 0.0f;
     for (int chunk = 0; chunk < a.per_class; chunk += 64) {
       const int count = min(64, a.per_class - chunk);
       const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-      // wave-local staging: the previous chunk's reads are done before the
-      // overwrite, and the new records are written before any lane reads them
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      if (lane < count) vtile[lane] = stage_vpl<kMixed>(crec + (size_t)lane * kVplFloats);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      unsigned long long mask = cull_chunk(vtile, count, wb);
+      wave_sync();  // the previous chunk's pairs are done with the VPLs and sums
+      if (lane < count) w.vpl[lane] = stage_vpl<kMixed>(crec + (size_t)lane * kVplFloats);
+      w.acc[lane][0] = w.acc[lane][1] = w.acc[lane][2] = 0.0f;
+      wave_sync();
+      unsigned long long mask = cull_chunk(w.vpl, count, wb);
+      int queued = 0;  // wave-uniform
       while (mask) {
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1ull;
-        const VplLds v = vtile[j];
-        const VplTest vt{v.px, v.py, v.pz, v.plane, v.nx, v.ny, v.nz};
-        const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
-        const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
-        if (ndot > 0.0f && pdot < 0.0f)
-          accumulate_pair<kMixed>(a, p, vt, v, first + chunk + j, ndot, pdot, shadow, lc, ar, ag, ab);
+        const VplLds& v = w.vpl[j];
+        const float ndot = __builtin_fmaf(v.nx, p.fx, __builtin_fmaf(v.ny, p.fy, __builtin_fmaf(v.nz, p.fz, -v.plane)));
+        const float pdot = __builtin_fmaf(-p.nx, v.px, __builtin_fmaf(-p.ny, v.py, __builtin_fmaf(-p.nz, v.pz, p.plane)));
+        const bool lit = ndot > 0.0f && pdot < 0.0f;
+        const unsigned long long b = __ballot(lit);
+        if (!b) continue;
+        if (lit) {
+          const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0));
+          w.queue[queued + rank] = PairLds{(uint32_t)lane | ((uint32_t)j << 8), ndot, pdot, 0.0f};
+        }
+        queued += __builtin_popcountll(b);
+        if (queued >= 64) {
+          wave_sync();
+          drain_pairs<kMixed>(a, w, 64, first + chunk, shadow, lc);
+          queued -= 64;
+          PairLds carry{};
+          if (lane < queued) carry = w.queue[64 + lane];
+          wave_sync();
+          if (lane < queued) w.queue[lane] = carry;
+          wave_sync();
+        }
       }
+      if (queued > 0) {
+        wave_sync();
+        drain_pairs<kMixed>(a, w, queued, first + chunk, shadow, lc);
+      }
+      const float cr = w.acc[lane][0], cg = w.acc[lane][1], cb = w.acc[lane][2];
       if (kMixed) {
-        kahan_add(tr, er, ar);
-        kahan_add(tg, eg, ag);
-        kahan_add(tb, eb, ab);
-        ar = ag = ab = 0.0f;
+        kahan_add(tr, er, cr);
+        kahan_add(tg, eg, cg);
+        kahan_add(tb, eb, cb);
+      } else {
+        tr += cr;
+        tg += cg;
+        tb += cb;
       }
-    }
-    if (kMixed) {
-      ar = tr;
-      ag = tg;
-      ab = tb;
     }
     if (p.valid) {
       float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
-      o[0] = ar * a.out_scale;
-      o[1] = ag * a.out_scale;
-      o[2] = ab * a.out_scale;
+      o[0] = tr * a.out_scale;
+      o[1] = tg * a.out_scale;
+      o[2] = tb * a.out_scale;
     }
   }
 }

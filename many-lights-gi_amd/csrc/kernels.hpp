@@ -65,6 +65,10 @@ hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision
 hipError_t launch_count_lit_pairs(hipStream_t s, const GatherArgs& a,
                                   unsigned long long* counter);
 
+// blur.hip: geometry-aware separable blur (x then y pass, fused)
+hipError_t launch_gi_blur(hipStream_t s, const float* gi, const float* normal, const float* depth, int w, int h,
+                          const float* projection16, float* out);
+
 // raycast.hip: G-buffer (rsm = 0: a = depth, b = normal) or RSM (rsm = 1:
 // a = diffuse, b = normal, c = depth) of a scene uploaded to the device
 }  // namespace mlgi

@@ -456,6 +456,16 @@ class GIStage:
                                                     C.byref(im), C.byref(h)), "mlgi_graph_capture_frame")
         return FrameGraph(self.ctx, h)
 
+    def blur(self, gbuffer: N.GBuffer, projection) -> None:
+        """GIStage::blur: geometry-aware x/y blur of gi_buffer into gi_blur_final."""
+        if getattr(self, "gi_blur_final", None) is None:
+            self.gi_blur_final = self.ctx.empty((self.height, self.width, 3), np.float32)
+        out = N.Image()
+        out.rgb, out.width, out.height = self.gi_blur_final.ptr, self.width, self.height
+        self._projection = np.ascontiguousarray(projection, np.float32)
+        self.ctx.check(lib.mlgi_gi_blur(self.ctx.h, C.byref(gbuffer), self._projection.ctypes.data,
+                                        C.byref(self.image_desc()), C.byref(out)), "mlgi_gi_blur")
+
     def count_lit_pairs(self, gbuffer: N.GBuffer) -> int:
         out = C.c_uint64(0)
         self.ctx.check(lib.mlgi_count_lit_pairs(self.ctx.h, C.byref(gbuffer), C.byref(self.vpl_processor.desc()),

@@ -139,6 +139,7 @@ class StageTimes(C.Structure):
         ("final_gather_ms", C.c_float),
         ("frame_ms", C.c_float),
         ("frames", C.c_int32),
+        ("blur_ms", C.c_float),
     ]
 
 
@@ -184,6 +185,7 @@ SIGNATURES = {
     "mlgi_graph_capture_frame": (C.c_int, [vp, P(Rsm), P(Points), P(GBuffer), P(Params), P(Vpls),
                                            P(Atlas), P(IsmDepth), P(Image), P(vp)]),
     "mlgi_graph_launch": (C.c_int, [vp, vp]),
+    "mlgi_gi_blur": (C.c_int, [vp, P(GBuffer), vp, P(Image), P(Image)]),
     "mlgi_batch_create": (C.c_int, [vp, P(FrameDesc), C.c_int32, P(vp)]),
     "mlgi_batch_run": (C.c_int, [vp, vp]),
     "mlgi_batch_capture": (C.c_int, [vp, vp, P(vp)]),

@@ -65,6 +65,8 @@ _sigs = {
                                        f32, f32, i32, i32, i32, vp, vp]),
     "oracle_final_gather_rows_f32": (None, [vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, i32, i32, i32,
                                             vp, i32, f32, f32, f32, i32, i32, i32, vp, vp]),
+    "oracle_gi_blur_f32": (None, [vp, vp, vp, i32, i32, vp, vp]),
+    "oracle_gi_blur_f64": (None, [vp, vp, vp, i32, i32, vp, vp]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(_lib, _n)
@@ -215,3 +217,13 @@ def final_gather_rows_f32(depth, normal, y0, y1, vp_inv, vpl_pos, vpl_nrm, vpl_c
                                       z_far, gi_intensity, clamp_value, interleave, int(shadowing),
                                       threads, _p(out), C.addressof(lit))
     return out, int(lit.value)
+
+
+def gi_blur(gi, normal, depth, projection, dtype=np.float64):
+    """Geometry-aware separable blur (gi_blur.frag, x pass then y pass)."""
+    g, nm, dp = _c(gi, np.float32), _c(normal, np.float32), _c(depth, np.float32)
+    h, w = dp.shape
+    out = np.empty((h, w, 3), dtype)
+    fn = _lib.oracle_gi_blur_f64 if dtype == np.float64 else _lib.oracle_gi_blur_f32
+    fn(_p(g), _p(nm), _p(dp), w, h, _p(_c(projection, np.float32)), _p(out))
+    return out

@@ -615,6 +615,64 @@ void final_gather_rows(const float* depth, const float* normal, int w, int h, in
 
 }  // namespace
 
+// ------------------------------------------------------ geometry-aware blur
+// gi/gi_blur.frag:24-72 run as GIStage::blur does (GIStage.cpp:308-344):
+// pass x (DIRECTION (1,0)) into a temporary, then pass y (DIRECTION (0,1))
+// of the temporary. Both passes read the same face normals and depth.
+// linearDepth (reprojection.glsl:6-13) uses P[2][2] and P[3][2]
+// (column-major m[10], m[14]). Out-of-range texelFetch reads 0.
+template <class T> T blur_linear_depth(T d, T A, T B) {
+  T zn = T(2.0) * d - T(1.0);
+  return -B / (A + zn);
+}
+
+template <class T>
+void gi_blur_pass(const T* gi, const float* normal, const float* depth, int w, int h, const float* proj,
+                  int dx, int dy, T* out) {
+  const T A = T(proj[10]), B = T(proj[14]);
+  const int radius = 3;
+  for (int y = 0; y < h; ++y) {
+    for (int x = 0; x < w; ++x) {
+      const size_t c = (size_t)y * w + x;
+      const T d = blur_linear_depth<T>(T(depth[c]), A, B);
+      const T nx = T(normal[c * 3]) * T(2.0) - T(1.0), ny = T(normal[c * 3 + 1]) * T(2.0) - T(1.0),
+              nz = T(normal[c * 3 + 2]) * T(2.0) - T(1.0);
+      T acc[3] = {gi[c * 3], gi[c * 3 + 1], gi[c * 3 + 2]};
+      T fsum = T(1.0);
+      for (int side = -1; side <= 1; side += 2) {  // the -DIRECTION loop, then +DIRECTION
+        for (int i = 1; i <= radius; ++i) {
+          const int sx = x + side * dx * i, sy = y + side * dy * i;
+          const bool in = sx >= 0 && sy >= 0 && sx < w && sy < h;
+          const size_t q = (size_t)sy * w + sx;
+          const T g0 = in ? gi[q * 3] : T(0), g1 = in ? gi[q * 3 + 1] : T(0), g2 = in ? gi[q * 3 + 2] : T(0);
+          const T mx = (in ? T(normal[q * 3]) : T(0)) * T(2.0) - T(1.0);
+          const T my = (in ? T(normal[q * 3 + 1]) : T(0)) * T(2.0) - T(1.0);
+          const T mz = (in ? T(normal[q * 3 + 2]) : T(0)) * T(2.0) - T(1.0);
+          const T ds = blur_linear_depth<T>(in ? T(depth[q]) : T(0), A, B);
+          const T nf = T(1.0) - std::max(T(0), nx * mx + ny * my + nz * mz);
+          const T dd = (ds - d) / T(1.0);
+          const T nd = nf * T(5.0);
+          // weight(r, depthDiff, normalDiff): max(0, (radius - r + 1) / 4 - dd^2 - nd^2)
+          const T factor = std::max(T(0), (T(radius) - T(i) + T(1.0)) / T(4.0) - dd * dd - nd * nd);
+          fsum = fsum + factor;
+          acc[0] = acc[0] + g0 * factor;
+          acc[1] = acc[1] + g1 * factor;
+          acc[2] = acc[2] + g2 * factor;
+        }
+      }
+      for (int k = 0; k < 3; ++k) out[c * 3 + k] = acc[k] / fsum;
+    }
+  }
+}
+
+template <class T>
+void gi_blur(const float* gi, const float* normal, const float* depth, int w, int h, const float* proj, T* out) {
+  std::vector<T> in((size_t)w * h * 3), tmp((size_t)w * h * 3);
+  for (size_t i = 0; i < in.size(); ++i) in[i] = T(gi[i]);
+  gi_blur_pass<T>(in.data(), normal, depth, w, h, proj, 1, 0, tmp.data());
+  gi_blur_pass<T>(tmp.data(), normal, depth, w, h, proj, 0, 1, out);
+}
+
 extern "C" {
 
 uint32_t oracle_pack3sn_f32(float x, float y, float z) { return pack3sn<float>(x, y, z); }
@@ -710,6 +768,15 @@ void oracle_final_gather_f32(const float* depth, const float* normal, int w, int
   final_gather_rows<float>(depth, normal, w, h, 0, h, vp_inv, vpos, vnrm, vcol, n_vpl, vs, ve, ism,
                            side, zf, gi, cl, il, sh, th, out, lit);
 }
+void oracle_gi_blur_f32(const float* gi, const float* normal, const float* depth, int w, int h,
+                        const float* proj16, float* out) {
+  gi_blur<float>(gi, normal, depth, w, h, proj16, out);
+}
+void oracle_gi_blur_f64(const float* gi, const float* normal, const float* depth, int w, int h,
+                        const float* proj16, double* out) {
+  gi_blur<double>(gi, normal, depth, w, h, proj16, out);
+}
+
 void oracle_final_gather_rows_f32(const float* depth, const float* normal, int w, int h, int y0,
                                   int y1, const float* vp_inv, const float* vpos, const float* vnrm,
                                   const float* vcol, int n_vpl, int vs, int ve, const uint16_t* ism,

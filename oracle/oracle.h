@@ -120,6 +120,13 @@ void oracle_final_gather_rows_f32(const float* depth, const float* normal, int w
                                   float clamp_value, int interleave, int shadowing,
                                   int threads, float* out, uint64_t* lit_pairs);
 
+/* Geometry-aware separable blur of the irradiance (gi_blur.frag, x pass
+ * then y pass); projection16 = the camera's P (column-major). */
+void oracle_gi_blur_f32(const float* gi, const float* normal, const float* depth, int w, int h,
+                        const float* projection16, float* out);
+void oracle_gi_blur_f64(const float* gi, const float* normal, const float* depth, int w, int h,
+                        const float* projection16, double* out);
+
 #ifdef __cplusplus
 }
 #endif
