@@ -77,7 +77,9 @@ typedef struct mlgi_params {
   int32_t precision;              /* mlgi_precision */
   int32_t tile_w, tile_h;         /* final-gather tile in interleave-class pixels; 0 = auto */
   int32_t gather_fetch;           /* final-gather VPL fetch variant 0..1; -1 = auto */
-  int32_t splat_block;            /* splat workgroup size 64/128/256; 0 = auto */
+  int32_t splat_block;            /* 0 = ISM-major splat (LDS tiles, no clear);
+                                     64/128/256 = point-major splat (global atomics
+                                     into a cleared atlas) with that workgroup size */
 } mlgi_params;
 
 /* RSM inputs (RasterizationStage outputs bound in VPLProcessor.cpp:84-86).
@@ -181,9 +183,17 @@ int mlgi_memcpy_dtod(mlgi_context* ctx, void* dst, const void* src, size_t bytes
 int mlgi_vpl_extract(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_params* p,
                      mlgi_vpls* out);
 /* Clear the atlas to 0xFFFFFFFF and splat every point into <= 4 of the 16
- * candidate ISMs of its bin with 32-bit atomicMin. Bit-exact. */
+ * candidate ISMs of its bin with 32-bit min. Bit-exact. With splat_block 0
+ * (and ISM sides 4..64, a 16-byte aligned atlas) the splat is ISM-major: the
+ * context keeps a work area of 2 bytes per point plus 128 KiB, grown on
+ * demand (growing it drains the context stream). */
 int mlgi_ism_splat(mlgi_context* ctx, const mlgi_points* pts, const mlgi_vpls* vpls,
                    const mlgi_params* p, mlgi_atlas* atlas);
+/* Diagnostics of the context's last ISM-major splat (waits for the stream):
+ * splats that landed outside their ISM's tile, and whether they exceeded the
+ * spill list (then the point-major splat ran over the tiles). No reference
+ * counterpart. */
+int mlgi_splat_spill_stats(mlgi_context* ctx, uint32_t* spills, int32_t* overflowed);
 /* Pull-push hole filling, one workgroup per ISM, R16 result. Bit-exact.
  * atlas->ism_side (ismPixelSize) must be one of 8, 16, 32, 64. */
 int mlgi_ism_pullpush(mlgi_context* ctx, const mlgi_atlas* atlas, const mlgi_params* p,

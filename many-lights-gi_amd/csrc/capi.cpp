@@ -110,6 +110,23 @@ int check_vpls(mlgi_context* c, const mlgi_vpls* v) {
   return MLGI_OK;
 }
 
+// Grows the context's splat work area to at least `bytes`. The stream is
+// drained first: a queued splat may still use the old area.
+int ensure_splat_work(mlgi_context* c, size_t bytes) {
+  if (c->splat_ws_bytes >= bytes) return MLGI_OK;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (rc) return rc;
+  if (c->splat_ws) (void)hipFree(c->splat_ws);
+  c->splat_ws = nullptr;
+  c->splat_ws_bytes = 0;
+  if (hipMalloc(&c->splat_ws, bytes) != hipSuccess) {
+    c->splat_ws = nullptr;
+    return fail(c, MLGI_ERR_OUT_OF_MEMORY, "hipMalloc(splat work, %zu bytes)", bytes);
+  }
+  c->splat_ws_bytes = bytes;
+  return MLGI_OK;
+}
+
 // Atlas geometry: n1d = ism grid side, S = atlas side / n1d.
 int atlas_geometry(mlgi_context* c, int side, int n_vpl, int* n1d, int* ism_side) {
   if (!is_pow2(side) || side > 16384)
@@ -169,6 +186,7 @@ void mlgi_context_destroy(mlgi_context* c) {
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
   (void)hipFree(c->counter);
+  if (c->splat_ws) (void)hipFree(c->splat_ws);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -304,6 +322,18 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas ism_side %d != %d for %d VPLs in a %d atlas",
                 atlas->ism_side, s, vpls->count, atlas->side);
   const size_t texels = (size_t)atlas->side * atlas->side;
+  if (p->splat_block == 0 && mlgi::ism_major_supported(s) && ((uintptr_t)atlas->texels & 15u) == 0) {
+    // ISM-major form: writes every texel, no clear
+    const uint32_t np = pts->count > 0 ? (uint32_t)pts->count : 0u;
+    if ((rc = ensure_splat_work(c, mlgi::splat_work_bytes(np)))) return rc;
+    Scope sc(c, kSplat);
+    return hip_check(c,
+                     mlgi::launch_ism_splat_tiled(c->stream, pts->records, pts->bin_offsets,
+                                                  pts->count > 0 ? pts->n_bins : 0, pts->count, vpls->records,
+                                                  vpls->count, n1d, s, atlas->side, p->z_far, atlas->texels,
+                                                  mlgi::splat_work(c->splat_ws, np)),
+                     "ism_splat_tiled");
+  }
   {
     Scope sc(c, kClear);
     if ((rc = hip_check(c, mlgi::launch_atlas_clear(c->stream, atlas->texels, texels), "atlas_clear")))
@@ -317,6 +347,21 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
                                           p->z_far, atlas->texels,
                                           p->splat_block > 0 ? p->splat_block : 256),
                    "ism_splat");
+}
+
+int mlgi_splat_spill_stats(mlgi_context* c, uint32_t* spills, int32_t* overflowed) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!spills || !overflowed) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  *spills = 0;
+  *overflowed = 0;
+  if (!c->splat_ws) return MLGI_OK;
+  uint32_t st[2] = {0, 0};
+  int rc = hip_check(c, hipMemcpyAsync(st, c->splat_ws, sizeof st, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (rc) return rc;
+  *spills = st[0];
+  *overflowed = st[1] ? 1 : 0;
+  return MLGI_OK;
 }
 
 int mlgi_ism_pullpush(mlgi_context* c, const mlgi_atlas* atlas, const mlgi_params* p,
@@ -614,6 +659,7 @@ int mlgi_profile_read(mlgi_context* c, mlgi_stage_times* out) {
 struct mlgi_batch {
   mlgi_context* ctx = nullptr;
   mlgi::BatchFrame* dev = nullptr;  // device copy of the frame table
+  char* work = nullptr;             // the frames' splat work areas
   mlgi_capi::BatchPlan plan;
 };
 
@@ -716,16 +762,18 @@ int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::v
   plan->row_of.assign(count, 0);
   for (int r = 0; r < count; ++r) plan->row_of[order[r]] = r;
   table->assign(count, mlgi::BatchFrame{});
-  uint64_t tv = 0, tc = 0, tb = 0;
+  uint64_t tv = 0, tt = 0, tb = 0;
+  size_t work = 0;
   for (int r = 0; r < count; ++r) {
     mlgi::BatchFrame f = rows[order[r]];
     const mlgi::BatchCounts n = mlgi::batch_counts(f);
     f.first_vpl = (uint32_t)tv;
-    f.first_clear = (uint32_t)tc;
+    f.first_tile = (uint32_t)tt;
     f.first_bin = (uint32_t)tb;
     tv += n.vpl;
-    tc += n.clear;
+    tt += n.tiles;
     tb += n.bins;
+    work += mlgi::splat_work_bytes(f.n_points);
     const int k = key[order[r]];
     const int prec = k >> 16;
     if (r == 0 || key[order[r - 1]] != k) plan->pullpush.push_back({r, 0, 0, f.ism_side, f.levels});
@@ -740,12 +788,24 @@ int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::v
     fg.count++;
     (*table)[r] = f;
   }
-  if (tv > 0xFFFFFFFFull || tc > 0xFFFFFFFFull || tb > 0xFFFFFFFFull)
+  if (tv > 0xFFFFFFFFull || tt > 0xFFFFFFFFull || tb > 0xFFFFFFFFull)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "batch too large");
   plan->total_vpl = (uint32_t)tv;
-  plan->total_clear = (uint32_t)tc;
+  plan->total_tiles = (uint32_t)tt;
   plan->total_bins = (uint32_t)tb;
+  plan->work_bytes = work;
   return MLGI_OK;
+}
+
+void batch_attach_work(std::vector<mlgi::BatchFrame>* table, char* base) {
+  size_t off = 0;
+  for (auto& f : *table) {
+    const mlgi::SplatWork w = mlgi::splat_work(base + off, f.n_points);
+    f.mask = w.mask;
+    f.spill_state = w.state;
+    f.spill = w.spill;
+    off += mlgi::splat_work_bytes(f.n_points);
+  }
 }
 
 int batch_enqueue(mlgi_context* c, const mlgi::BatchFrame* dev, const BatchPlan& plan) {
@@ -757,13 +817,10 @@ int batch_enqueue(mlgi_context* c, const mlgi::BatchFrame* dev, const BatchPlan&
       return rc;
   }
   {
-    Scope sc(c, kClear);
-    if ((rc = hip_check(c, mlgi::launch_batch_clear(c->stream, dev, plan.count, plan.total_clear), "batch_clear")))
-      return rc;
-  }
-  {
-    Scope sc(c, kSplat);
-    if ((rc = hip_check(c, mlgi::launch_batch_splat(c->stream, dev, plan.count, plan.total_bins), "batch_splat")))
+    Scope sc(c, kSplat);  // ISM-major: mask, tiles (which clear the atlases), spills
+    if ((rc = hip_check(c, mlgi::launch_batch_mask(c->stream, dev, plan.count, plan.total_bins), "batch_mask")) ||
+        (rc = hip_check(c, mlgi::launch_batch_tiles(c->stream, dev, plan.count, plan.total_tiles), "batch_tiles")) ||
+        (rc = hip_check(c, mlgi::launch_batch_spill(c->stream, dev, plan.count, plan.total_bins), "batch_spill")))
       return rc;
   }
   {
@@ -789,7 +846,7 @@ int batch_enqueue(mlgi_context* c, const mlgi::BatchFrame* dev, const BatchPlan&
 mlgi::BatchCounts mlgi::batch_counts(const mlgi::BatchFrame& f) {
   BatchCounts n;
   n.vpl = (uint32_t)f.g.n_vpl;
-  n.clear = batch_clear_chunks(f.atlas_side);
+  n.tiles = (uint32_t)(f.n1d * f.n1d);
   n.bins = f.n_points > 0 ? (uint32_t)f.g.n_vpl : 0u;
   n.isms = (uint32_t)(f.n1d * f.n1d);
   const int th = 64 >> f.wave_w_log2;
@@ -809,6 +866,8 @@ int mlgi_batch_create(mlgi_context* c, const mlgi_frame_desc* frames, int32_t co
   auto* b = new mlgi_batch();
   b->ctx = c;
   int rc = batch_plan(c, frames, count, &table, &b->plan);
+  if (!rc) rc = hip_check(c, hipMalloc(&b->work, b->plan.work_bytes), "hipMalloc(batch work)");
+  if (!rc) batch_attach_work(&table, b->work);
   if (!rc) rc = hip_check(c, hipMalloc(&b->dev, sizeof(mlgi::BatchFrame) * count), "hipMalloc(batch)");
   if (!rc)
     rc = hip_check(c, hipMemcpy(b->dev, table.data(), sizeof(mlgi::BatchFrame) * count, hipMemcpyHostToDevice),
@@ -872,10 +931,9 @@ int mlgi_batch_capture(mlgi_context* c, mlgi_batch* b, mlgi_graph** graph) {
 
 void mlgi_batch_destroy(mlgi_batch* b) {
   if (!b) return;
-  if (b->dev) {
-    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
-    (void)hipFree(b->dev);
-  }
+  if ((b->dev || b->work) && b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->dev) (void)hipFree(b->dev);
+  if (b->work) (void)hipFree(b->work);
   delete b;
 }
 

@@ -29,6 +29,9 @@ struct mlgi_context {
   int frames = 0;
   // autotuner cache: shape key -> (tile_w, tile_h)
   std::map<std::vector<int>, std::vector<int>> tuned;
+  // work area of the single-frame ISM-major splat (grown on demand)
+  void* splat_ws = nullptr;
+  size_t splat_ws_bytes = 0;
 };
 
 namespace mlgi_capi {
@@ -79,7 +82,8 @@ struct Scope {  // records a timed span on the context stream when profiling
 struct BatchPlan {
   int count = 0;
   std::vector<int> row_of;  // caller's frame index -> table row
-  uint32_t total_vpl = 0, total_clear = 0, total_bins = 0;
+  uint32_t total_vpl = 0, total_tiles = 0, total_bins = 0;
+  size_t work_bytes = 0;  // splat work areas of all frames (batch_attach_work)
   struct Run {
     int begin, count;
     uint32_t total;
@@ -93,6 +97,9 @@ int batch_frame(mlgi_context* c, const mlgi_frame_desc& d, mlgi::BatchFrame* f);
 // Validates and orders frames into a device-ready table and its plan.
 int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::vector<mlgi::BatchFrame>* table,
                BatchPlan* plan);
+// Points the table rows' splat work areas into `base` (plan.work_bytes bytes
+// of device memory, one area per frame).
+void batch_attach_work(std::vector<mlgi::BatchFrame>* table, char* base);
 // Enqueues a planned batch whose table is at `dev` (device memory).
 int batch_enqueue(mlgi_context* c, const mlgi::BatchFrame* dev, const BatchPlan& plan);
 

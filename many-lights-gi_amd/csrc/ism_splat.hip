@@ -1,17 +1,36 @@
-// ISM point splatting with 32-bit atomicMin on packed depth.
+// ISM point splatting with 32-bit min on packed depth.
 // Follows ism/ism.comp:54-139 (usePushPull path) and the atlas clear of
-// ImperfectShadowmap.cpp:212. Compiled with -ffp-contract=off: the atlas is
-// bit-exact against the FP32 oracle twin (integer atomicMin is order-free).
+// ImperfectShadowmap.cpp:212. Compiled with -ffp-contract=off: every projected
+// value is bit-exact against the FP32 oracle twin, and the atlas texel is the
+// minimum over all splats that reach it, which no evaluation order changes.
 //
-// One workgroup per point bin b (256 lanes by default, so a bin of a few
-// hundred points is one pass and every SIMD holds enough waves to cover the
-// latency of the point loads and atomics). The 16 candidate VPLs (b+i) mod N
-// are staged in LDS once per workgroup (1 KiB); each lane owns one point per
-// iteration, tests all 16 candidates from broadcast LDS reads (uniform
-// addresses, no bank conflicts), keeps the first four that pass both
-// hemisphere tests as a bit mask, and projects only those. The projected
-// texel receives a no-return, relaxed, agent-scope atomic umin, which the L2
-// executes without a round trip to the wave.
+// Point-major form (the reference's decomposition; splat_block 64/128/256):
+// the atlas is cleared to 0xFFFFFFFF, then one workgroup per point bin b
+// stages the 16 candidate VPLs (b+i) mod N in LDS (1 KiB); each lane owns one
+// point per iteration, tests all 16 candidates from broadcast LDS reads, keeps
+// the first four that pass both hemisphere tests as a bit mask, and projects
+// only those. The projected texel receives a no-return, relaxed, agent-scope
+// atomic umin in L2. Each splat is a scattered atomic on a line of an atlas
+// far larger than L2, so this form is bound by atomic traffic.
+//
+// ISM-major form (default), three kernels:
+//  1. mask: one workgroup per bin, as above, stores each point's 16-bit set of
+//     kept candidates (2 bytes per point) instead of projecting;
+//  2. tiles: one workgroup per ISM v builds its S x S tile in LDS. The points
+//     splatted into v are those of bins (v - i) mod N whose mask holds bit i
+//     (i < 16); each wave scans four of those bins 64 points at a time,
+//     queues the hits in LDS (ballot + mbcnt) and projects 64 queued points at
+//     once, all onto the one, wave-uniform, VPL v, with LDS atomic umin. The
+//     tile is then written with 16-byte stores: that write is the clear.
+//     A projection can land outside v's tile (exactly on the tile's far
+//     border, or one texel across a border by rounding); such splats go to a
+//     spill list of kSpillCap entries;
+//  3. spill: after every tile is written, applies the spill list with global
+//     atomic umin. If the list overflowed, it instead reruns the point-major
+//     splat of the whole frame over the tiles; min is idempotent, so the
+//     result is the same.
+// Tiles are mapped to XCDs in contiguous ranges, so the bins that
+// neighbouring ISMs share are read through one L2.
 #include "batch_device.hpp"
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
@@ -41,9 +60,104 @@ struct Candidate {
   int pad;
 };
 
-// Bin b of one frame: stage the 16 candidates in LDS, then every lane splats
-// one point per pass. The first point record is requested before the staging
-// barrier and each next one while the current one is processed.
+__device__ __forceinline__ Candidate load_candidate(const float* __restrict__ vpl, int idx, int n1d) {
+  const float4* r = reinterpret_cast<const float4*>(vpl + (size_t)idx * kVplFloats);
+  const float4 r0 = r[kVplPos / 4], r1 = r[kVplNormal / 4], r2 = r[kVplS / 4], r3 = r[kVplU / 4];
+  Candidate c;
+  c.px = r0.x; c.py = r0.y; c.pz = r0.z;
+  c.nx = r1.x; c.ny = r1.y; c.nz = r1.z;
+  c.sx = r2.x; c.sy = r2.y; c.sz = r2.z;
+  c.ux = r3.x; c.uy = r3.y; c.uz = r3.z;
+  c.ism = idx;
+  // int(ismIndex / ismIndices1d), exact for idx < 2^24
+  c.tile_y = idx / n1d;
+  c.tile_x = idx - c.tile_y * n1d;
+  c.pad = 0;
+  return c;
+}
+
+// Radius byte of a point record's packed w (unorm4x8 .w * 25 is the radius;
+// the atlas keeps radius * 10 / sqrt(4)).
+__device__ __forceinline__ uint32_t radius_byte_of(float w) {
+  const float uw = (float)(__float_as_uint(w) >> 24) / 255.0f;
+  const float radius = uw * 25.0f;
+  return dev::to_uint(radius * 10.0f / 2.0f);
+}
+
+// Bit i set: candidate i is one of the first kMaxCollect that pass both
+// hemisphere tests (ism.comp:80-103).
+__device__ __forceinline__ uint32_t keep_mask(const Candidate* cand, float4 rec) {
+  const f3 pos = dev::make(rec.x, rec.y, rec.z);
+  const uint32_t bits = __float_as_uint(rec.w);
+  // unpackUnorm4x8: c / 255
+  const float ux = (float)(bits & 0xFFu) / 255.0f;
+  const float uy = (float)((bits >> 8) & 0xFFu) / 255.0f;
+  const float uz = (float)((bits >> 16) & 0xFFu) / 255.0f;
+  const f3 pn3 = dev::make(ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, uz * 2.0f - 1.0f);
+  uint32_t keep = 0;
+  int found = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxTest; ++i) {
+    const Candidate& c = cand[i];
+    const f3 rel = dev::sub(pos, dev::make(c.px, c.py, c.pz));
+    const f3 nrel = dev::make(-rel.x, -rel.y, -rel.z);
+    const bool cull = dev::dot(dev::make(c.nx, c.ny, c.nz), rel) < 0.0f || dev::dot(pn3, nrel) < 0.0f;
+    if (!cull && found < kMaxCollect) {
+      keep |= 1u << i;
+      ++found;
+    }
+  }
+  return keep;
+}
+
+// Projection of a point onto candidate c (paraboloid_project with
+// preserveSign, ism.comp:105-139): the atlas texel (tx, ty) and the packed
+// value. False when the texel lies outside the atlas (imageAtomicMin outside
+// the image is a no-op).
+__device__ __forceinline__ bool project(const Candidate& c, f3 pos, uint32_t radius_byte, float inv_n1d,
+                                        float side, float z_far, uint32_t& tx, uint32_t& ty,
+                                        uint32_t& value) {
+  const f3 rel = dev::sub(pos, dev::make(c.px, c.py, c.pz));
+  const float dist = sqrtf(dev::dot(rel, rel));
+  float vx = dev::dot(dev::make(c.sx, c.sy, c.sz), rel);
+  float vy = dev::dot(dev::make(c.ux, c.uy, c.uz), rel);
+  float vz = dev::dot(dev::make(-c.nx, -c.ny, -c.nz), rel);
+  const float sign_v = vz > 0.0f ? 1.0f : (vz < 0.0f ? -1.0f : 0.0f);
+  vx = vx / dist;
+  vy = vy / dist;
+  vz = vz / dist;
+  vz = 1.0f - vz;
+  vx = vx / vz;
+  vy = vy / vz;
+  vz = dist / z_far;
+  vz = vz * -sign_v;
+  vx = (vx + 1.0f) * 0.5f;
+  vy = (vy + 1.0f) * 0.5f;
+  vx = (vx + (float)c.tile_x) * inv_n1d;
+  vy = (vy + (float)c.tile_y) * inv_n1d;
+  vx = vx * side;
+  vy = vy * side;
+  vz = vz * 16777216.0f;
+  value = (dev::to_uint(vz) << 8) | radius_byte;
+  // ivec2(v.xy) truncates toward zero
+  const float fx = truncf(vx), fy = truncf(vy);
+  if (!(fx >= 0.0f && fx < side && fy >= 0.0f && fy < side)) return false;
+  tx = (uint32_t)fx;
+  ty = (uint32_t)fy;
+  return true;
+}
+
+// Stages the 16 candidates of bin b in LDS (lanes 0..15); the caller
+// synchronises.
+__device__ __forceinline__ void stage_candidates(Candidate* cand, const float* __restrict__ vpl, int n_vpl,
+                                                 int n1d, int b) {
+  const int lane = threadIdx.x;
+  if (lane < kMaxTest) cand[lane] = load_candidate(vpl, (b + lane) % n_vpl, n1d);  // index %= totalVplCount
+}
+
+// Point-major splat of bin b of one frame: every lane splats one point per
+// pass with global atomics. The first point record is requested before the
+// staging barrier and each next one while the current one is processed.
 template <int kThreads>
 __device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restrict__ points,
                                           const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
@@ -54,89 +168,29 @@ __device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restr
   // clamp to the point count so malformed offsets cannot index past the buffer
   const uint32_t end = min(bin_offsets[b + 1], n_points);
   const uint32_t begin = min(bin_offsets[b], end);
+  // loads past the bin read its last point (or point 0; n_points > 0) and
+  // are discarded: no select of a record, which would go through scratch
+  const uint32_t last = end > 0 ? end - 1 : 0;
   uint32_t p = begin + lane;
-  float4 rec = p < end ? points[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-  if (lane < kMaxTest) {
-    const int idx = (b + lane) % n_vpl;  // index %= totalVplCount
-    const float4* r = reinterpret_cast<const float4*>(vpl + (size_t)idx * kVplFloats);
-    const float4 r0 = r[kVplPos / 4], r1 = r[kVplNormal / 4], r2 = r[kVplS / 4], r3 = r[kVplU / 4];
-    Candidate c;
-    c.px = r0.x; c.py = r0.y; c.pz = r0.z;
-    c.nx = r1.x; c.ny = r1.y; c.nz = r1.z;
-    c.sx = r2.x; c.sy = r2.y; c.sz = r2.z;
-    c.ux = r3.x; c.uy = r3.y; c.uz = r3.z;
-    c.ism = idx;
-    // int(ismIndex / ismIndices1d), exact for idx < 2^24
-    c.tile_y = idx / n1d;
-    c.tile_x = idx - c.tile_y * n1d;
-    c.pad = 0;
-    cand[lane] = c;
-  }
+  float4 rec = points[min(p, last)];
+  stage_candidates(cand, vpl, n_vpl, n1d, b);
   __syncthreads();
 
   const float inv_n1d = 1.0f / (float)n1d;  // exact: n1d is a power of two
   const float side = (float)atlas_side;
   while (p < end) {
     const uint32_t pn = p + kThreads;
-    const float4 next = pn < end ? points[pn] : rec;
+    const float4 next = points[min(pn, last)];
     const f3 pos = dev::make(rec.x, rec.y, rec.z);
-    const uint32_t bits = __float_as_uint(rec.w);
-    // unpackUnorm4x8: c / 255
-    const float ux = (float)(bits & 0xFFu) / 255.0f;
-    const float uy = (float)((bits >> 8) & 0xFFu) / 255.0f;
-    const float uz = (float)((bits >> 16) & 0xFFu) / 255.0f;
-    const float uw = (float)(bits >> 24) / 255.0f;
-    const f3 pn3 = dev::make(ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, uz * 2.0f - 1.0f);
-    const float radius = uw * 25.0f;
-    const uint32_t radius_byte = dev::to_uint(radius * 10.0f / 2.0f);  // sqrt(4) == 2
-
-    // gather up to kMaxCollect candidates that pass both hemisphere tests
-    uint32_t keep = 0;
-    int found = 0;
-#pragma unroll
-    for (int i = 0; i < kMaxTest; ++i) {
-      const Candidate& c = cand[i];
-      const f3 rel = dev::sub(pos, dev::make(c.px, c.py, c.pz));
-      const f3 nrel = dev::make(-rel.x, -rel.y, -rel.z);
-      const bool cull = dev::dot(dev::make(c.nx, c.ny, c.nz), rel) < 0.0f || dev::dot(pn3, nrel) < 0.0f;
-      if (!cull && found < kMaxCollect) {
-        keep |= 1u << i;
-        ++found;
-      }
-    }
-
+    const uint32_t rb = radius_byte_of(rec.w);
+    uint32_t keep = keep_mask(cand, rec);
     while (keep) {
       const int i = __builtin_ctz(keep);
       keep &= keep - 1u;
-      const Candidate& c = cand[i];
-      const f3 rel = dev::sub(pos, dev::make(c.px, c.py, c.pz));
-      const float dist = sqrtf(dev::dot(rel, rel));
-      // paraboloid_project(rel, dist, n, zFar, idx, n1d, preserveSign = true)
-      float vx = dev::dot(dev::make(c.sx, c.sy, c.sz), rel);
-      float vy = dev::dot(dev::make(c.ux, c.uy, c.uz), rel);
-      float vz = dev::dot(dev::make(-c.nx, -c.ny, -c.nz), rel);
-      const float sign_v = vz > 0.0f ? 1.0f : (vz < 0.0f ? -1.0f : 0.0f);
-      vx = vx / dist;
-      vy = vy / dist;
-      vz = vz / dist;
-      vz = 1.0f - vz;
-      vx = vx / vz;
-      vy = vy / vz;
-      vz = dist / z_far;
-      vz = vz * -sign_v;
-      vx = (vx + 1.0f) * 0.5f;
-      vy = (vy + 1.0f) * 0.5f;
-      vx = (vx + (float)c.tile_x) * inv_n1d;
-      vy = (vy + (float)c.tile_y) * inv_n1d;
-      vx = vx * side;
-      vy = vy * side;
-      vz = vz * 16777216.0f;
-      const uint32_t value = (dev::to_uint(vz) << 8) | radius_byte;
-      // ivec2(v.xy) truncates; imageAtomicMin outside the image is a no-op
-      const float fx = truncf(vx), fy = truncf(vy);
-      if (!(fx >= 0.0f && fx < side && fy >= 0.0f && fy < side)) continue;
-      const size_t texel = (size_t)(uint32_t)fy * (uint32_t)atlas_side + (uint32_t)fx;
-      __hip_atomic_fetch_min(atlas + texel, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t tx, ty, value;
+      if (!project(cand[i], pos, rb, inv_n1d, side, z_far, tx, ty, value)) continue;
+      __hip_atomic_fetch_min(atlas + (size_t)ty * (uint32_t)atlas_side + tx, value, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
     }
     rec = next;
     p = pn;
@@ -153,34 +207,240 @@ __global__ __launch_bounds__(kThreads) void ism_splat_kernel(
                       blockIdx.x);
 }
 
-// Batched splat: persistent workgroups over the (frame, bin) items.
-__global__ __launch_bounds__(256) void batch_splat_kernel(const BatchFrame* __restrict__ frames, int n,
+// ------------------------------------------------------------ ISM-major form
+
+// Mask pass of bin b. The workgroup of bin 0 also resets the frame's spill
+// state (count, overflow) for the tile pass that follows.
+__device__ __forceinline__ void mask_bin(Candidate* cand, const float4* __restrict__ points,
+                                         const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
+                                         const float* __restrict__ vpl, int n_vpl, int n1d,
+                                         uint16_t* __restrict__ mask, uint32_t* __restrict__ spill_state,
+                                         int b) {
+  const int lane = threadIdx.x;
+  const uint32_t end = min(bin_offsets[b + 1], n_points);
+  const uint32_t begin = min(bin_offsets[b], end);
+  const uint32_t last = end > 0 ? end - 1 : 0;  // as in splat_bin
+  uint32_t p = begin + lane;
+  float4 rec = points[min(p, last)];
+  if (b == 0 && lane < 2) spill_state[lane] = 0u;
+  stage_candidates(cand, vpl, n_vpl, n1d, b);
+  __syncthreads();
+  while (p < end) {
+    const uint32_t pn = p + 256;
+    const float4 next = points[min(pn, last)];
+    mask[p] = (uint16_t)keep_mask(cand, rec);
+    rec = next;
+    p = pn;
+  }
+}
+
+constexpr int kTileMax = 64;  // largest ISM side of the ISM-major form
+constexpr int kQueue = 128;   // per-wave queue: < 64 left over + 64 appended
+
+struct TileLds {
+  uint32_t tile[kTileMax * kTileMax];
+  float4 queue[4][kQueue];
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct TileJob {
+  const float4* points;
+  const uint32_t* bin_offsets;
+  uint32_t n_points;
+  const float* vpl;
+  int n_vpl, n1d, ism_side, atlas_side;
+  float z_far;
+  uint32_t* atlas;
+  const uint16_t* mask;
+  uint32_t* spill_state;
+  uint2* spill;
+};
+
+// One queued point splatted into ISM c (tile origin ox, oy).
+__device__ __forceinline__ void splat_queued(const TileJob& j, TileLds& sm, const Candidate& c, float4 rec,
+                                             uint32_t ox, uint32_t oy, float inv_n1d, float side) {
+  uint32_t tx, ty, value;
+  if (!project(c, dev::make(rec.x, rec.y, rec.z), radius_byte_of(rec.w), inv_n1d, side, j.z_far, tx, ty, value))
+    return;
+  const uint32_t lx = tx - ox, ly = ty - oy;  // wraps above S when left of / above the tile
+  if (lx < (uint32_t)j.ism_side && ly < (uint32_t)j.ism_side) {
+    __hip_atomic_fetch_min(&sm.tile[ly * j.ism_side + lx], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else {
+    const uint32_t k = __hip_atomic_fetch_add(&j.spill_state[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k < kSpillCap)
+      j.spill[k] = make_uint2(ty * (uint32_t)j.atlas_side + tx, value);
+    else
+      __hip_atomic_fetch_or(&j.spill_state[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Tile pass of ISM v (all 256 lanes; ends after the tile is written).
+__device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int S = j.ism_side;
+  const int tile_y = v / j.n1d, tile_x = v - tile_y * j.n1d;
+  const int n4 = S * S / 4;
+  uint4* t4 = reinterpret_cast<uint4*>(sm.tile);
+  const uint4 ones = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+  for (int i = tid; i < n4; i += 256) t4[i] = ones;
+  __syncthreads();
+
+  if (v < j.n_vpl && j.n_points > 0) {  // ISMs >= N receive no splats
+    const Candidate c = load_candidate(j.vpl, v, j.n1d);
+    const float inv_n1d = 1.0f / (float)j.n1d;
+    const float side = (float)j.atlas_side;
+    const uint32_t ox = (uint32_t)(tile_x * S), oy = (uint32_t)(tile_y * S);
+    float4* q = sm.queue[wave];
+    int qn = 0;  // wave-uniform queue length
+    for (int i = wave; i < kMaxTest; i += 4) {
+      const int b = ((v - i) % j.n_vpl + j.n_vpl) % j.n_vpl;  // (b + i) mod N == v
+      const uint32_t end = min(j.bin_offsets[b + 1], j.n_points);
+      const uint32_t begin = min(j.bin_offsets[b], end);
+      for (uint32_t p0 = begin; p0 < end; p0 += 64) {
+        const uint32_t p = p0 + lane;
+        const bool hit = p < end && ((j.mask[p] >> i) & 1u);
+        const uint64_t bal = __ballot(hit);
+        if (hit) {
+          const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+          q[slot] = j.points[p];
+        }
+        qn += __builtin_popcountll(bal);
+        if (qn >= 64) {
+          wave_sync();
+          splat_queued(j, sm, c, q[qn - 64 + lane], ox, oy, inv_n1d, side);
+          wave_sync();
+          qn -= 64;
+        }
+      }
+    }
+    wave_sync();
+    if (lane < qn) splat_queued(j, sm, c, q[lane], ox, oy, inv_n1d, side);
+  }
+  __syncthreads();
+  const size_t origin = (size_t)tile_y * S * j.atlas_side + (size_t)tile_x * S;
+  for (int i = tid; i < n4; i += 256) {
+    const int row = (i * 4) / S, col = (i * 4) % S;
+    *reinterpret_cast<uint4*>(j.atlas + origin + (size_t)row * j.atlas_side + col) = t4[i];
+  }
+}
+
+// Spill pass, item b of the frame's n_bins: every spill entry j == b (mod
+// n_bins) in the normal case; bin b's point-major splat after an overflow.
+__device__ __forceinline__ void spill_bin(Candidate* cand, const TileJob& j, int n_bins, int b) {
+  const uint32_t overflow = j.spill_state[1];
+  if (overflow) {
+    splat_bin<256>(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, j.atlas_side, j.z_far,
+                   j.atlas, b);
+    return;
+  }
+  const uint32_t n = min(j.spill_state[0], kSpillCap);
+  for (uint32_t k = (uint32_t)b * 256u + threadIdx.x; k < n; k += (uint32_t)n_bins * 256u) {
+    const uint2 e = j.spill[k];
+    __hip_atomic_fetch_min(j.atlas + e.x, e.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+TileJob tile_job(const float* points, const uint32_t* bin_offsets, int n_points, const float* vpl, int n_vpl,
+                 int n1d, int ism_side, int atlas_side, float z_far, uint32_t* atlas, const SplatWork& w) {
+  return TileJob{reinterpret_cast<const float4*>(points), bin_offsets, (uint32_t)n_points, vpl, n_vpl, n1d,
+                 ism_side, atlas_side, z_far, atlas, w.mask, w.state, w.spill};
+}
+
+__device__ __forceinline__ TileJob tile_job(const BatchFrame& f) {
+  return TileJob{reinterpret_cast<const float4*>(f.points), f.bin_offsets, f.n_points, f.g.vpl, f.g.n_vpl, f.n1d,
+                 f.ism_side, f.atlas_side, f.z_far, f.atlas, f.mask, f.spill_state, f.spill};
+}
+
+// Single-frame kernels. Grids are multiples of 8; workgroup k serves XCD
+// k % 8, which walks one contiguous eighth of the items.
+__global__ __launch_bounds__(256) void ism_mask_kernel(TileJob j, int n_bins) {
+  __shared__ Candidate cand[kMaxTest];
+  const XcdRange r = xcd_range((uint32_t)n_bins, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
+  for (uint32_t b = r.start; b < r.end; b += r.stride) {
+    mask_bin(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, const_cast<uint16_t*>(j.mask),
+             j.spill_state, (int)b);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void ism_tiles_kernel(TileJob j, int n_tiles) {
+  __shared__ TileLds sm;
+  const XcdRange r = xcd_range((uint32_t)n_tiles, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
+  for (uint32_t v = r.start; v < r.end; v += r.stride) {
+    splat_tile(j, sm, (int)v);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void ism_spill_kernel(TileJob j, int n_bins) {
+  __shared__ Candidate cand[kMaxTest];
+  for (int b = blockIdx.x; b < n_bins; b += gridDim.x) {
+    spill_bin(cand, j, n_bins, b);
+    __syncthreads();
+  }
+}
+
+// Batched kernels: persistent workgroups over the (frame, item) space of a
+// stage; frame f's items start at its first_bin / first_tile.
+__global__ __launch_bounds__(256) void batch_mask_kernel(const BatchFrame* __restrict__ frames, int n,
+                                                         uint32_t total) {
+  __shared__ Candidate cand[kMaxTest];
+  const XcdRange r = xcd_range(total, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
+  for (uint32_t item = r.start; item < r.end; item += r.stride) {
+    const BatchFrame& f = frames[batch_find<&BatchFrame::first_bin>(frames, n, item)];
+    mask_bin(cand, reinterpret_cast<const float4*>(f.points), f.bin_offsets, f.n_points, f.g.vpl, f.g.n_vpl,
+             f.n1d, f.mask, f.spill_state, (int)(item - f.first_bin));
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_tiles_kernel(const BatchFrame* __restrict__ frames, int n,
+                                                          uint32_t total) {
+  __shared__ TileLds sm;
+  const XcdRange r = xcd_range(total, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
+  for (uint32_t item = r.start; item < r.end; item += r.stride) {
+    const BatchFrame& f = frames[batch_find<&BatchFrame::first_tile>(frames, n, item)];
+    splat_tile(tile_job(f), sm, (int)(item - f.first_tile));
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_spill_kernel(const BatchFrame* __restrict__ frames, int n,
                                                           uint32_t total) {
   __shared__ Candidate cand[kMaxTest];
   for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
     const BatchFrame& f = frames[batch_find<&BatchFrame::first_bin>(frames, n, item)];
-    splat_bin<256>(cand, reinterpret_cast<const float4*>(f.points), f.bin_offsets, f.n_points, f.g.vpl,
-                   f.g.n_vpl, f.n1d, f.atlas_side, f.z_far, f.atlas, (int)(item - f.first_bin));
-    __syncthreads();  // the next item restages the candidates
+    spill_bin(cand, tile_job(f), f.g.n_vpl, (int)(item - f.first_bin));
+    __syncthreads();
   }
 }
 
-// Batched clear: one 64 KiB chunk (4096 x 16 bytes) of one frame's atlas per item.
-constexpr uint32_t kClearChunk = 4096;
-__global__ __launch_bounds__(256) void batch_clear_kernel(const BatchFrame* __restrict__ frames, int n,
-                                                          uint32_t total) {
-  const uint4 ones = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-    const BatchFrame& f = frames[batch_find<&BatchFrame::first_clear>(frames, n, item)];
-    const size_t n4 = (size_t)f.atlas_side * f.atlas_side / 4;
-    const size_t c0 = (size_t)(item - f.first_clear) * kClearChunk;
-    const size_t c1 = min(n4, c0 + kClearChunk);
-    uint4* a = reinterpret_cast<uint4*>(f.atlas);
-    for (size_t i = c0 + threadIdx.x; i < c1; i += 256) a[i] = ones;
-  }
-}
+unsigned xcd_grid(uint32_t items, unsigned per_cu) { return persistent_grid(items, per_cu); }
 
 }  // namespace
+
+bool ism_major_supported(int ism_side) { return ism_side >= 4 && ism_side <= kTileMax && ism_side % 4 == 0; }
+
+size_t splat_work_bytes(uint32_t n_points) {
+  return 256 + (size_t)kSpillCap * sizeof(uint2) + (((size_t)n_points * 2 + 255) & ~(size_t)255);
+}
+
+SplatWork splat_work(void* base, uint32_t n_points) {
+  (void)n_points;
+  char* b = static_cast<char*>(base);
+  SplatWork w;
+  w.state = reinterpret_cast<uint32_t*>(b);
+  w.spill = reinterpret_cast<uint2*>(b + 256);
+  w.mask = reinterpret_cast<uint16_t*>(b + 256 + (size_t)kSpillCap * sizeof(uint2));
+  return w;
+}
 
 hipError_t launch_atlas_clear(hipStream_t s, uint32_t* atlas, size_t texels) {
   const size_t n4 = texels / 4;  // atlas sides are multiples of 4
@@ -209,21 +469,41 @@ hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* 
   return hipGetLastError();
 }
 
-hipError_t launch_batch_clear(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
-  if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_clear_kernel, dim3(persistent_grid(total, 8)), dim3(256), 0, s, frames, n, total);
+hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint32_t* bin_offsets, int n_bins,
+                                  int n_points, const float* vpl_records, int n_vpl, int n1d, int ism_side,
+                                  int atlas_side, float z_far, uint32_t* atlas, const SplatWork& w) {
+  const TileJob j = tile_job(points, bin_offsets, n_points, vpl_records, n_vpl, n1d, ism_side, atlas_side, z_far,
+                             atlas, w);
+  const bool any = n_bins > 0 && n_points > 0;
+  if (any) {
+    hipLaunchKernelGGL(ism_mask_kernel, dim3(xcd_grid((uint32_t)n_bins, 8)), dim3(256), 0, s, j, n_bins);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int n_tiles = n1d * n1d;
+  hipLaunchKernelGGL(ism_tiles_kernel, dim3(xcd_grid((uint32_t)n_tiles, 4)), dim3(256), 0, s, j, n_tiles);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !any) return e;
+  hipLaunchKernelGGL(ism_spill_kernel, dim3(n_bins < 2048 ? n_bins : 2048), dim3(256), 0, s, j, n_bins);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_splat(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+hipError_t launch_batch_mask(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_splat_kernel, dim3(persistent_grid(total, 8)), dim3(256), 0, s, frames, n, total);
+  hipLaunchKernelGGL(batch_mask_kernel, dim3(xcd_grid(total, 8)), dim3(256), 0, s, frames, n, total);
   return hipGetLastError();
 }
 
-uint32_t batch_clear_chunks(int atlas_side) {
-  const uint64_t n4 = (uint64_t)atlas_side * atlas_side / 4;
-  return (uint32_t)((n4 + kClearChunk - 1) / kClearChunk);
+hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_tiles_kernel, dim3(xcd_grid(total, 4)), dim3(256), 0, s, frames, n, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_spill(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_spill_kernel, dim3(persistent_grid(total, 8)), dim3(256), 0, s, frames, n, total);
+  return hipGetLastError();
 }
 
 }  // namespace mlgi

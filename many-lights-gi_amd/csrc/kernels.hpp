@@ -31,6 +31,22 @@ hipError_t launch_atlas_clear(hipStream_t s, uint32_t* atlas, size_t texels);
 hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* bin_offsets,
                             int n_bins, int n_points, const float* vpl_records, int n_vpl, int n1d,
                             int atlas_side, float z_far, uint32_t* atlas, int block);
+// ISM-major splat: per-point candidate masks, one LDS tile per ISM, then the
+// spill list of splats that land outside their ISM's tile. Needs a work area
+// of splat_work_bytes(n_points) bytes (no initial contents) and a 16-byte
+// aligned atlas; writes every atlas texel, so no clear precedes it.
+constexpr uint32_t kSpillCap = 16384;  // spill entries per frame
+struct SplatWork {
+  uint32_t* state;  // [0] spill count, [1] overflow flag
+  uint2* spill;     // (texel, value) x kSpillCap
+  uint16_t* mask;   // per point: kept candidates as bits
+};
+bool ism_major_supported(int ism_side);
+size_t splat_work_bytes(uint32_t n_points);
+SplatWork splat_work(void* base, uint32_t n_points);
+hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint32_t* bin_offsets, int n_bins,
+                                  int n_points, const float* vpl_records, int n_vpl, int n1d, int ism_side,
+                                  int atlas_side, float z_far, uint32_t* atlas, const SplatWork& w);
 
 // pullpush.hip
 hipError_t launch_pullpush(hipStream_t s, const uint32_t* atlas, int atlas_side, int ism_side,
@@ -101,6 +117,9 @@ struct BatchFrame {
   const uint32_t* bin_offsets;
   uint32_t n_points;
   uint32_t* atlas;
+  uint16_t* mask;          // splat work area (SplatWork)
+  uint32_t* spill_state;
+  uint2* spill;
   int atlas_side, ism_side, n1d, levels;
   float z_far;
   uint32_t pp_e1[4];  // pull level 1 of an all-empty block (bits of depth, maxDepth, radius, disp)
@@ -109,19 +128,20 @@ struct BatchFrame {
   int wave_w_log2;  // gather wave strip: 2^wave_w_log2 x 64 / 2^wave_w_log2 class pixels
   int strips_x;     // strips per class row
   // first work item per stage
-  uint32_t first_vpl, first_clear, first_bin, first_ism, first_wave;
+  uint32_t first_vpl, first_tile, first_bin, first_ism, first_wave;
 };
 
 // Work-item counts of a frame, in each stage's unit.
 struct BatchCounts {
-  uint32_t vpl, clear, bins, isms, waves;
+  uint32_t vpl, tiles, bins, isms, waves;
 };
 BatchCounts batch_counts(const BatchFrame& f);
 
 hipError_t launch_batch_vpl(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
-hipError_t launch_batch_clear(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
-uint32_t batch_clear_chunks(int atlas_side);  // clear items of one atlas
-hipError_t launch_batch_splat(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
+// ISM-major splat of the batch: mask (bin items), tiles (ISM items), spill (bin items)
+hipError_t launch_batch_mask(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
+hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
+hipError_t launch_batch_spill(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
 void pullpush_empty_texel(int ism_side, float z_far, uint32_t out[4]);
 // frames[0..n) all have the given ISM side and level count
 hipError_t launch_batch_pullpush(hipStream_t s, const BatchFrame* frames, int n, uint32_t total,

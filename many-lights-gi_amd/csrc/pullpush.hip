@@ -175,29 +175,30 @@ __host__ __device__ inline Decoded decode_texel(uint32_t s, float r, float madd,
 // to the output centre, validity) into one output texel.
 __host__ __device__ inline Texel pull_combine(const float* ds, const float* mds, const float* rs,
                                               const float* dx, const float* dy, bool* valid) {
+  // Branch-free selection: fminf(d, minimum) == d is d <= minimum (no NaNs
+  // here), and a term of an invalid input is +0, which leaves the sums'
+  // bits unchanged (none of them can be -0).
   float minimum = __builtin_inff();
   float max_depth = 0.0f;
   bool any = false;
   for (int i = 0; i < 4; ++i) {
-    if (!valid[i]) continue;
-    any = true;
-    minimum = fminf(ds[i], minimum);
-    if (minimum == ds[i]) max_depth = mds[i];
+    const bool take = valid[i] && !(ds[i] > minimum);
+    minimum = take ? ds[i] : minimum;
+    max_depth = take ? mds[i] : max_depth;
+    any = any || valid[i];
   }
   Texel res{1.0f, 0.0f, 0.0f, 0u};
   if (!any) return res;
-  for (int i = 0; i < 4; ++i)
-    if (ds[i] > max_depth) valid[i] = false;
   float dacc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f, mvm = 0.0f;
   uint32_t nv = 0;
   for (int i = 0; i < 4; ++i) {
-    if (!valid[i]) continue;
-    dacc = dacc + ds[i];
-    xacc = xacc + dx[i];
-    yacc = yacc + dy[i];
-    racc = racc + rs[i];
-    mvm = fmaxf(mvm, mds[i]);
-    ++nv;
+    const bool v = valid[i] && !(ds[i] > max_depth);
+    dacc = dacc + (v ? ds[i] : 0.0f);
+    xacc = xacc + (v ? dx[i] : 0.0f);
+    yacc = yacc + (v ? dy[i] : 0.0f);
+    racc = racc + (v ? rs[i] : 0.0f);
+    mvm = (v && mvm < mds[i]) ? mds[i] : mvm;
+    nv += v ? 1u : 0u;
   }
   if (nv > 0) {
     const float fn = (float)nv;
@@ -217,7 +218,11 @@ constexpr int ism_log2() {
 }
 
 __host__ __device__ __forceinline__ uint16_t to_r16(float depth) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float r = __builtin_amdgcn_fmed3f(depth, 0.0f, 1.0f);  // clamp (depth is never NaN)
+#else
   const float r = fminf(fmaxf(depth, 0.0f), 1.0f);
+#endif
   return (uint16_t)rintf(r * 65535.0f);
 }
 
@@ -321,10 +326,10 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
     float minimum = 9001.0f;
     float max_depth = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (w[i] == 0.0f) continue;
-      minimum = fminf(cd[i], minimum);
-      if (minimum == cd[i]) max_depth = cmd[i];
+    for (int i = 0; i < 4; ++i) {  // fminf(cd, minimum) == cd is cd <= minimum
+      const bool take = w[i] != 0.0f && !(cd[i] > minimum);
+      minimum = take ? cd[i] : minimum;
+      max_depth = take ? cmd[i] : max_depth;
     }
     float wacc = 0.0f;
 #pragma unroll
@@ -347,13 +352,22 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
 #pragma unroll
     for (int i = 0; i < 4; ++i) occluded = occluded || (w[i] > 0.0f && orig.depth > cmd[i]);
     Texel res = orig;
-    if (wacc > 0.0f && (invalid || occluded)) {
+    const bool fill = wacc > 0.0f && (invalid || occluded);
+    if constexpr (l == 0) {
+      // depth only: computed for every lane and selected (rcp[0] = 0 keeps
+      // the unused wacc = 0 case finite)
+      const float y = rcp[(int)wacc];
+      float dacc = cd[0] * w[0];
+#pragma unroll
+      for (int i = 1; i < 4; ++i) dacc = dacc + cd[i] * w[i];
+      res.depth = fill ? div_by_recip(dacc, wacc, y) : orig.depth;
+    } else if (fill) {
       const float y = rcp[(int)wacc];
       float dacc = cd[0] * w[0];
 #pragma unroll
       for (int i = 1; i < 4; ++i) dacc = dacc + cd[i] * w[i];
       res.depth = div_by_recip(dacc, wacc, y);
-      if constexpr (l > 0) {
+      {
         float macc = cmd[0] * w[0], racc = cr[0] * w[0], xacc = ddx[0] * w[0], yacc = ddy[0] * w[0];
 #pragma unroll
         for (int i = 1; i < 4; ++i) {

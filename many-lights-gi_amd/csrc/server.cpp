@@ -111,10 +111,11 @@ struct Tune {
 // Work buffers of one admitted request, carved out of one arena range.
 struct Footprint {
   int atlas_side, ism_side, levels;
-  size_t vpl, atlas, ism, rsm_diffuse, rsm_normal, rsm_depth, image, total;
+  size_t vpl, atlas, ism, rsm_diffuse, rsm_normal, rsm_depth, image, work, total;
 };
 
-Footprint footprint(int w, int h, int n, int rsm) {
+// n_points: the scene's point count (sizes the splat work area, last in the range)
+Footprint footprint(int w, int h, int n, int rsm, int n_points) {
   Footprint f;
   f.ism_side = n <= 1024 ? 32 : 64;
   f.atlas_side = mlgi_ism_indices_1d(n) * f.ism_side;
@@ -127,7 +128,8 @@ Footprint footprint(int w, int h, int n, int rsm) {
   f.rsm_normal = round_up((size_t)rsm * rsm * 12);
   f.rsm_depth = round_up((size_t)rsm * rsm * 4);
   f.image = round_up((size_t)w * h * 12);
-  f.total = f.vpl + f.atlas + f.ism + f.rsm_diffuse + f.rsm_normal + f.rsm_depth + f.image;
+  f.work = round_up(mlgi::splat_work_bytes((uint32_t)n_points));
+  f.total = f.vpl + f.atlas + f.ism + f.rsm_diffuse + f.rsm_normal + f.rsm_depth + f.image + f.work;
   return f;
 }
 
@@ -169,6 +171,8 @@ int dev_alloc(mlgi_server* s, size_t bytes, void** p) {
   return rc;
 }
 
+int points_of(const mlgi_server* s, const mlgi_request& r) { return s->scenes[r.scene].points.at(r.n_vpl).count; }
+
 mlgi_params request_params(const mlgi_server* s, const Footprint& fp, int n, const Tune& t) {
   mlgi_params p;
   mlgi_params_default(&p);
@@ -188,7 +192,7 @@ mlgi_params request_params(const mlgi_server* s, const Footprint& fp, int n, con
 mlgi_frame_desc request_frame(mlgi_server* s, const mlgi_request& r, size_t off, const Tune& t,
                               const float inv_bvp[16], float** image) {
   const int rs = s->cfg.rsm_size;
-  const Footprint fp = footprint(r.width, r.height, r.n_vpl, rs);
+  const Footprint fp = footprint(r.width, r.height, r.n_vpl, rs, points_of(s, r));
   char* b = s->arena_base + off;
   float* vpl = (float*)b;
   uint32_t* atlas = (uint32_t*)(b + fp.vpl);
@@ -220,7 +224,7 @@ mlgi_frame_desc request_frame(mlgi_server* s, const mlgi_request& r, size_t off,
 // inverse biased light matrix.
 int render_request_rsm(mlgi_server* s, const mlgi_request& r, size_t off, float inv_bvp[16]) {
   const int rs = s->cfg.rsm_size;
-  const Footprint fp = footprint(r.width, r.height, r.n_vpl, rs);
+  const Footprint fp = footprint(r.width, r.height, r.n_vpl, rs, points_of(s, r));
   char* b = s->arena_base + off;
   float* dif = (float*)(b + fp.vpl + fp.atlas + fp.ism);
   float* nrm = (float*)((char*)dif + fp.rsm_diffuse);
@@ -332,9 +336,9 @@ int mlgi_server_create(mlgi_context* c, const mlgi_server_config* cfg, const mlg
   for (auto& wh : s->resolutions) {
     for (int n : s->vpl_counts) {
       if (rc) break;
-      const Footprint fp = footprint(wh.first, wh.second, n, cfg->rsm_size);
-      if (fp.total > cfg->hbm_budget) continue;  // never admissible; requests of it are rejected
       mlgi_request r{0, wh.first, wh.second, n, 0, 0.0, nullptr};
+      const Footprint fp = footprint(wh.first, wh.second, n, cfg->rsm_size, points_of(s, r));
+      if (fp.total > cfg->hbm_budget) continue;  // never admissible; requests of it are rejected
       size_t off = 0;
       s->arena.alloc(fp.total, &off);
       float inv[16];
@@ -414,7 +418,7 @@ int mlgi_server_run(mlgi_server* s, const mlgi_request* reqs, int32_t n, mlgi_se
       while (!queue.empty() && (int)adm.size() < s->cfg.max_batch && inflight_reqs < s->cfg.max_inflight) {
         const int i = queue.front();
         const mlgi_request& r = reqs[i];
-        const Footprint fp = footprint(r.width, r.height, r.n_vpl, s->cfg.rsm_size);
+        const Footprint fp = footprint(r.width, r.height, r.n_vpl, s->cfg.rsm_size, points_of(s, r));
         if (fp.total > s->arena.size()) {  // can never fit: reject
           queue.pop_front();
           stats->rejected++;
@@ -438,6 +442,15 @@ int mlgi_server_run(mlgi_server* s, const mlgi_request* reqs, int32_t n, mlgi_se
         std::vector<mlgi::BatchFrame> table;
         BatchPlan plan;
         if (!(rc = batch_plan(c, descs.data(), (int)descs.size(), &table, &plan))) {
+          for (size_t k = 0; k < adm.size(); ++k) {  // splat work areas: the end of each request's range
+            const mlgi_request& r = reqs[adm[k].req];
+            const Footprint fp = footprint(r.width, r.height, r.n_vpl, s->cfg.rsm_size, points_of(s, r));
+            mlgi::BatchFrame& row = table[plan.row_of[k]];
+            const mlgi::SplatWork w = mlgi::splat_work(s->arena_base + adm[k].off + fp.total - fp.work, row.n_points);
+            row.mask = w.mask;
+            row.spill_state = w.state;
+            row.spill = w.spill;
+          }
           mlgi::BatchFrame* host = s->table_host + (size_t)slot * s->cfg.max_batch;
           mlgi::BatchFrame* dev = s->table_dev + (size_t)slot * s->cfg.max_batch;
           std::memcpy(host, table.data(), sizeof(mlgi::BatchFrame) * table.size());

@@ -399,6 +399,12 @@ class ImperfectShadowmap:
         self.ctx.check(lib.mlgi_ism_splat(self.ctx.h, C.byref(points), C.byref(vpl_processor.desc()),
                                           C.byref(params), C.byref(self.atlas_desc())), "mlgi_ism_splat")
 
+    def spill_stats(self) -> tuple[int, bool]:
+        """(spills, overflowed) of the context's last ISM-major splat."""
+        n, o = C.c_uint32(0), C.c_int32(0)
+        self.ctx.check(lib.mlgi_splat_spill_stats(self.ctx.h, C.byref(n), C.byref(o)), "mlgi_splat_spill_stats")
+        return int(n.value), bool(o.value)
+
     def pullpush(self, params: Params) -> None:
         self.ctx.check(lib.mlgi_ism_pullpush(self.ctx.h, C.byref(self.atlas_desc()), C.byref(params),
                                              C.byref(self.depth_desc())), "mlgi_ism_pullpush")

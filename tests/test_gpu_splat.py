@@ -1,0 +1,115 @@
+"""GPU: the ISM-major splat (candidate masks, one LDS tile per ISM, spill
+list) against the oracle's splat and the point-major splat, bit for bit.
+
+The points are placed so that projections land exactly on a tile's far
+border, which belongs to the neighbouring ISM: every VPL sits at the origin
+with normal +z (lookAtRH gives s = -x, u = +y exactly), and points on the -x
+axis (normal +x) or the +y axis (normal -y) lie in the VPL's tangent plane,
+so their paraboloid coordinate is exactly 1 and ivec2 truncation puts them on
+the first column / row of the next tile. A few hundred such points exercise
+the spill list; several thousand overflow it, and the point-major splat then
+reruns over the tiles. Random points around them splat inside the tiles.
+"""
+import numpy as np
+import pytest
+
+import mlgi
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+N_VPL = 64        # ISM grid 8 x 8
+ATLAS = 256       # ISM side 32
+Z_FAR = 50.0
+
+
+def unorm4x8(x, y, z, w):
+    b = [int(np.clip(np.rint(v * 255.0), 0, 255)) for v in (x, y, z, w)]
+    return np.uint32(b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24)).view(np.float32)
+
+
+def point_set(n_axis, n_random, seed):
+    rng = np.random.default_rng(seed)
+    pts = []
+    for k in range(n_axis):
+        t = float(1 + k % 40)           # exact squares: dist == t
+        rb = (k * 37) % 256 / 255.0
+        if k % 2 == 0:                  # -x axis, normal +x
+            pts.append([-t, 0.0, 0.0, unorm4x8(1.0, 0.5, 0.5, rb)])
+        else:                           # +y axis, normal -y
+            pts.append([0.0, t, 0.0, unorm4x8(0.5, 0.0, 0.5, rb)])
+    for _ in range(n_random):
+        p = rng.uniform(-5.0, 5.0, 3)
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        pts.append([p[0], p[1], p[2], unorm4x8(*(n * 0.5 + 0.5), rng.uniform())])
+    pts = np.array(pts, np.float32)
+    # random bins, points stored bin by bin
+    bins = rng.integers(0, N_VPL, len(pts))
+    order = np.argsort(bins, kind="stable")
+    pts, bins = pts[order], bins[order]
+    off = np.zeros(N_VPL + 1, np.uint32)
+    np.add.at(off, bins + 1, 1)
+    return pts, np.cumsum(off).astype(np.uint32)
+
+
+def vpl_records():
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    s, u = oracle.look_at_rh(n, np.float32)
+    rec = np.zeros((N_VPL, mlgi.VPL_RECORD_FLOATS), np.float32)
+    rec[:, 4:7] = n
+    rec[:, 8:11] = s
+    rec[:, 12:15] = u
+    return rec
+
+
+def run(ctx, pts, off, splat_block):
+    vp = mlgi.VPLProcessor(ctx, N_VPL)
+    vp.records.copy_from(vpl_records())
+    ism = mlgi.ImperfectShadowmap(ctx, ATLAS, N_VPL)
+    p = mlgi.default_params(atlas_side=ATLAS, z_far=Z_FAR, splat_block=splat_block)
+    ism.render(mlgi.points_desc(ctx.upload(pts), ctx.upload(off)), vp, p)
+    ctx.sync()
+    stats = ism.spill_stats() if splat_block == 0 else None
+    return ism.softrender.numpy(), stats
+
+
+@pytest.mark.parametrize("n_axis,overflow", [(600, False), (6000, True)])
+def test_spills_match_oracle(ctx, n_axis, overflow):
+    print(f"[splat] spills n_axis={n_axis}", flush=True)
+    pts, off = point_set(n_axis, 4000, seed=n_axis)
+    rec = vpl_records()
+    ref = oracle.ism_splat(pts, off, rec[:, 0:3], rec[:, 4:7], ATLAS, Z_FAR)
+    tiled, (spills, overflowed) = run(ctx, pts, off, 0)
+    point_major, _ = run(ctx, pts, off, 256)
+    print(f"[splat] spills={spills} overflowed={overflowed}", flush=True)
+    assert spills > 0
+    assert overflowed == overflow
+    assert np.array_equal(point_major, ref)
+    assert np.array_equal(tiled, ref)
+
+
+@pytest.mark.parametrize("name", ["tiny", "small"])
+def test_workload_tiled_equals_point_major(ctx, name):
+    from mlgi import workloads as W
+    wl = W.WORKLOADS[name]
+    inp = W.build_inputs(wl)
+    p = wl.params()
+    st = mlgi.GIStage(ctx, p, wl.n_vpl, wl.width, wl.height)
+    rsm = mlgi.rsm_desc(ctx.upload(inp.rsm_diffuse), ctx.upload(inp.rsm_normal), ctx.upload(inp.rsm_depth),
+                        inp.inv_bvp)
+    pts = mlgi.points_desc(ctx.upload(inp.points), ctx.upload(inp.bin_offsets))
+    st.vpl_processor.process(rsm, p.light_intensity, bool(p.shuffle_lights), p)
+    st.params.splat_block = 0
+    st.ism.render(pts, st.vpl_processor, st.params)
+    ctx.sync()
+    tiled = st.ism.softrender.numpy()
+    spills, overflowed = st.ism.spill_stats()
+    print(f"[splat] {name}: spills={spills} overflowed={overflowed}", flush=True)
+    st.params.splat_block = 256
+    st.ism.render(pts, st.vpl_processor, st.params)
+    ctx.sync()
+    assert np.array_equal(st.ism.softrender.numpy(), tiled)
+    rec = st.vpl_processor.host_records()
+    ref = oracle.ism_splat(inp.points, inp.bin_offsets, rec[:, 0:3], rec[:, 4:7], wl.atlas_side, p.z_far)
+    assert np.array_equal(tiled, ref)
