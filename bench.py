@@ -8,21 +8,30 @@ on inputs already resident in HBM -- VPL extraction from the RSM, ISM atlas
 clear + point splat, pull-push, interleaved final gathering -- at 1920x1080,
 4,096 VPLs (128^2 RSM), 1M splat points into 4,096 64^2 ISMs of a 4096^2
 atlas, 4 pull-push levels, 8x8 interleave, 40k-triangle procedural room.
-Two forms run on the same inputs:
-  fp32   the plain FP32 form through the single-frame entry point (mlgi_frame)
-  mixed  the mixed-precision form through the persistent batched kernels
-         (mlgi_batch_run of the one frame)
+Three forms run on the same inputs:
+  fp32          the plain FP32 form through the single-frame entry point
+                (mlgi_frame)
+  fp32_batched  the FP32 form through the persistent batched kernels
+                (mlgi_batch_run of the one frame)
+  mixed         the mixed-precision form through the batched kernels
 The timed region is exactly K frames of a form bracketed by a device
 synchronize on both sides; ms_per_step is wall time / K; HIP events on the
 context stream bracket every stage of every timed frame. `value` is the
-mixed form's ms/frame when it runs. The mixed batch is also replayed from a
-captured HIP graph (reported as graph_ms_per_frame).
+mixed form's ms/frame when it runs. Batched forms are also replayed from a
+captured HIP graph (graph_ms_per_frame). mixed_speedup_vs_fp32 compares
+mixed with the plain fp32 path; mixed_speedup_same_path isolates precision
+(fp32_batched / mixed).
 
 roofline: for the kernel with the largest mean duration in the timed frames:
+its VALU issue rate ("valu": SQ_INSTS_VALU per launch from the committed
+rocprofv3 counters, profiles/sq_counters_<workload>.json, over the live kernel
+time, against the issue rate of an FMA probe kernel measured in the same run)
+and
 pull-push against HBM (6 bytes per atlas texel: the u32 atlas read once, the
 unorm16 ISM written once), the final gather against the FP32 vector peak (84
-flops per lit pixel-VPL pair, pairs counted on the device), the splat against
-HBM (16 bytes per point record read). Every kernel's figure is under
+flops per lit pixel-VPL pair, pairs counted on the device), the splat (mask,
+tile and spill kernels together) against HBM (16 bytes per point record read
+plus 4 bytes per atlas texel written). Every kernel's figure is under
 "kernels"; "traffic" is the HBM bytes per launch measured with rocprofv3
 counters (profiles/traffic_<workload>.json), null until measured.
 cpu_baseline: the oracle's FP32 instantiation of the same four stages on the
@@ -55,6 +64,8 @@ sys.path.insert(0, str(ROOT / "many-lights-gi_amd"))
 
 FLOPS_PER_LIT_PAIR = 84
 FP32_VECTOR_PEAK_TFLOPS = 157.3
+# 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction
+VALU_PEAK_GINST_S = 614.4
 HBM_PEAK_GBS = 8000.0
 DATA = "synthetic: seeded procedural room, ray-cast G-buffer/RSM, area-proportional points"
 
@@ -74,13 +85,14 @@ def upload_frame(ctx, inp):
 
 
 def run_form(ctx, wl, inp, form: str, steps: int, warmup: int):
-    """fp32: plain single-frame path; mixed: batched path (batch of one)."""
+    """fp32: plain single-frame path; fp32_batched / mixed: batched path
+    (batch of one) in FP32 / mixed precision."""
     import mlgi
 
-    prec = mlgi.PRECISION_FP32 if form == "fp32" else mlgi.PRECISION_MIXED
+    prec = mlgi.PRECISION_MIXED if form == "mixed" else mlgi.PRECISION_FP32
     st = mlgi.GIStage(ctx, wl.params(precision=prec), wl.n_vpl, wl.width, wl.height)
     rsm, pts, gb = upload_frame(ctx, inp)
-    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb)]) if form == "mixed" else None
+    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb)]) if form != "fp32" else None
     step = batch.run if batch else (lambda: st.process(rsm, pts, gb))
     for _ in range(warmup):
         step()
@@ -111,7 +123,22 @@ def run_form(ctx, wl, inp, form: str, steps: int, warmup: int):
     return out
 
 
-def kernel_rooflines(wl, stages, lit):
+def valu_roof(kernel_ms, sq, peak_fma_s):
+    """VALU issue roofline of a kernel: VALU wave-instructions per launch
+    (rocprofv3 SQ_INSTS_VALU, committed profile) over the live kernel time,
+    against the measured FP32 FMA issue rate (one wave64 instruction = 64
+    lane FMAs)."""
+    if not sq or kernel_ms <= 0:
+        return None
+    peak = peak_fma_s / 64.0 / 1e9 if peak_fma_s else VALU_PEAK_GINST_S
+    ach = sq["SQ_INSTS_VALU"] / (kernel_ms * 1e-3) / 1e9
+    return {"insts_per_launch": sq["SQ_INSTS_VALU"], "achieved": round(ach, 1), "peak": round(peak, 1),
+            "unit": "G wave64-inst/s", "frac": round(ach / peak, 4),
+            "peak_source": "measured FMA probe" if peak_fma_s else "spec"}
+
+
+def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None):
+    sq = sq or {}
     a2 = wl.atlas_side * wl.atlas_side
     out = {}
     t = stages["pullpush_ms"] * 1e-3
@@ -119,12 +146,16 @@ def kernel_rooflines(wl, stages, lit):
         gbs = 6.0 * a2 / t / 1e9
         out["pullpush"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": 6 * a2,
-                           "kernel_ms": round(stages["pullpush_ms"], 4)}
+                           "kernel_ms": round(stages["pullpush_ms"], 4),
+                           "valu": valu_roof(stages["pullpush_ms"], sq.get("pullpush"), peak_fma_s)}
     t = stages["ism_splat_ms"] * 1e-3
     if t > 0:
-        gbs = 16.0 * wl.n_points / t / 1e9
+        # ISM-major splat: point records read once, the atlas written once
+        # (that write is the clear); the 2-byte masks stay in L2/MALL
+        b = 16 * wl.n_points + 4 * a2
+        gbs = b / t / 1e9
         out["ism_splat"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": 16 * wl.n_points,
+                            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": b,
                             "kernel_ms": round(stages["ism_splat_ms"], 4)}
     t = stages["final_gather_ms"] * 1e-3
     if t > 0:
@@ -132,7 +163,8 @@ def kernel_rooflines(wl, stages, lit):
         out["final_gather"] = {"bound": "valu", "achieved": round(tf, 3), "peak": FP32_VECTOR_PEAK_TFLOPS,
                                "unit": "TFLOP/s", "frac": round(tf / FP32_VECTOR_PEAK_TFLOPS, 4),
                                "lit_pairs": lit, "flops_per_lit_pair": FLOPS_PER_LIT_PAIR,
-                               "kernel_ms": round(stages["final_gather_ms"], 4)}
+                               "kernel_ms": round(stages["final_gather_ms"], 4),
+                               "valu": valu_roof(stages["final_gather_ms"], sq.get("final_gather"), peak_fma_s)}
     return out
 
 
@@ -180,7 +212,7 @@ def frame_bench(a, ctx, wl):
     t0 = time.perf_counter()
     inp = W.build_inputs(wl)
     log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s")
-    forms = {"fp32": ["fp32"], "mixed": ["mixed"], "both": ["fp32", "mixed"]}[a.form]
+    forms = {"fp32": ["fp32"], "mixed": ["mixed"], "both": ["fp32", "fp32_batched", "mixed"]}[a.form]
     res = {}
     for form in forms:
         log(f"[bench] form {form}: warmup {a.warmup}, steps {a.steps}")
@@ -188,7 +220,13 @@ def frame_bench(a, ctx, wl):
         log(f"[bench] {form}: {res[form]['wall_ms']:.4f} ms/frame, stages {res[form]['stages']}")
     head = "mixed" if "mixed" in res else "fp32"
     h = res[head]
-    kernels = kernel_rooflines(wl, h["stages"], h["lit"])
+    peak_fma_s = ctx.probe_valu()
+    log(f"[bench] FP32 FMA probe: {peak_fma_s * 2 / 1e12:.1f} TFLOP/s")
+    sq = None
+    sq_path = ROOT / "profiles" / f"sq_counters_{wl.name}.json"
+    if sq_path.exists():
+        sq = {k: v for k, v in json.loads(sq_path.read_text()).items() if not k.startswith("_")}
+    kernels = kernel_rooflines(wl, h["stages"], h["lit"], sq, peak_fma_s)
     dominant = max(kernels, key=lambda k: kernels[k]["kernel_ms"])
     roof = dict(kernels[dominant])
     roof["kernel"] = dominant
@@ -222,8 +260,16 @@ def frame_bench(a, ctx, wl):
         "roofline": roof,
         "kernels": kernels,
     }
+    out["fp32_fma_probe_tflops"] = round(peak_fma_s * 2 / 1e12, 2)
     if "fp32" in res and "mixed" in res:
+        # path and precision together: single-frame FP32 vs batched mixed
         out["mixed_speedup_vs_fp32"] = round(res["fp32"]["wall_ms"] / res["mixed"]["wall_ms"], 4)
+    if "fp32_batched" in res and "mixed" in res:
+        # precision alone: both through the batched kernels
+        out["mixed_speedup_same_path"] = round(res["fp32_batched"]["wall_ms"] / res["mixed"]["wall_ms"], 4)
+        g32, gmx = res["fp32_batched"].get("graph_ms"), res["mixed"].get("graph_ms")
+        if g32 and gmx:
+            out["mixed_speedup_same_path_graph"] = round(g32 / gmx, 4)
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, inp, a.cpu_threads)

@@ -178,6 +178,12 @@ int mlgi_memcpy_dtoh(mlgi_context* ctx, void* dst, const void* src, size_t bytes
 /* Device-to-device copy, asynchronous on the context stream. */
 int mlgi_memcpy_dtod(mlgi_context* ctx, void* dst, const void* src, size_t bytes);
 
+/* Measured FP32 vector FMA rate of the context's device (lane FMAs per
+ * second; one FMA = 2 flops), from a probe kernel of independent FMA chains
+ * timed with HIP events (waits for the stream). A measurement aid for the
+ * VALU rooflines; no reference counterpart. */
+int mlgi_probe_valu(mlgi_context* ctx, double* fma_per_s);
+
 /* ----------------------------------------------------------------- stages */
 /* VPL extraction: count = n VPLs regularly sampled from the RSM. */
 int mlgi_vpl_extract(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_params* p,

@@ -44,6 +44,7 @@ SOURCES = [
     ("final_gather.hip", HIP_FLAGS),
     ("raycast.hip", HIP_FLAGS + EXACT),
     ("blur.hip", HIP_FLAGS + EXACT),
+    ("diag.hip", HIP_FLAGS),
     ("capi.cpp", HIP_FLAGS),
     ("server.cpp", HIP_FLAGS),
     ("scene.cpp", ["-ffp-contract=off"]),

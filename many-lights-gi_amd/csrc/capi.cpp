@@ -265,6 +265,34 @@ int mlgi_memcpy_dtod(mlgi_context* c, void* dst, const void* src, size_t bytes) 
                    "hipMemcpyAsync(DtoD)");
 }
 
+int mlgi_probe_valu(mlgi_context* c, double* fma_per_s) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!fma_per_s) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "fma_per_s is NULL");
+  *fma_per_s = 0.0;
+  const int blocks = 2048, iters = 4096;
+  float* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = hip_check(c, hipMalloc(&out, blocks * sizeof(float)), "hipMalloc(probe)");
+  if (!rc) rc = hip_check(c, hipEventCreate(&e0), "hipEventCreate");
+  if (!rc) rc = hip_check(c, hipEventCreate(&e1), "hipEventCreate");
+  float best = 0.0f;
+  for (int rep = 0; rep < 4 && !rc; ++rep) {  // the first run warms up
+    if ((rc = hip_check(c, hipEventRecord(e0, c->stream), "hipEventRecord"))) break;
+    if ((rc = hip_check(c, mlgi::launch_valu_probe(c->stream, out, blocks, iters), "valu_probe"))) break;
+    if ((rc = hip_check(c, hipEventRecord(e1, c->stream), "hipEventRecord"))) break;
+    if ((rc = hip_check(c, hipEventSynchronize(e1), "hipEventSynchronize"))) break;
+    float ms = 0.0f;
+    if ((rc = hip_check(c, hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"))) break;
+    if (rep > 0 && (best == 0.0f || ms < best)) best = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (out) (void)hipFree(out);
+  if (rc) return rc;
+  *fma_per_s = best > 0.0f ? (double)mlgi::valu_probe_fmas(blocks, iters) / (best * 1e-3) : 0.0;
+  return MLGI_OK;
+}
+
 int mlgi_shuffle_indices(int32_t n, uint32_t seed, int32_t* out) {
   if (n < 0 || (!out && n)) return MLGI_ERR_INVALID_ARGUMENT;
   std::vector<int> v(n);

@@ -234,12 +234,14 @@ __device__ __forceinline__ void mask_bin(Candidate* cand, const float4* __restri
   }
 }
 
-constexpr int kTileMax = 64;  // largest ISM side of the ISM-major form
-constexpr int kQueue = 128;   // per-wave queue: < 64 left over + 64 appended
+constexpr int kTileMax = 64;                   // largest ISM side of the ISM-major form
+constexpr int kBinsPerWave = kMaxTest / 4;     // the 16 candidate bins over 4 waves
+constexpr int kStep = 256;                     // points per bin per step
+constexpr int kQueue = kBinsPerWave * kStep;   // hit indices of one step
 
 struct TileLds {
-  uint32_t tile[kTileMax * kTileMax];
-  float4 queue[4][kQueue];
+  uint32_t tile[kTileMax * kTileMax];  // 16 KiB
+  uint32_t queue[4][kQueue];           // 16 KiB: per-wave point-index queues
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -296,32 +298,58 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v)
     const float inv_n1d = 1.0f / (float)j.n1d;
     const float side = (float)j.atlas_side;
     const uint32_t ox = (uint32_t)(tile_x * S), oy = (uint32_t)(tile_y * S);
-    float4* q = sm.queue[wave];
-    int qn = 0;  // wave-uniform queue length
-    for (int i = wave; i < kMaxTest; i += 4) {
+    // Wave w owns bins i = w, w + 4, w + 8, w + 12. Per step it reads the
+    // masks of up to kStep points of each of its four bins (16 loads in
+    // flight per lane), queues the hits' point indices in LDS (ballot +
+    // mbcnt), then splats the queued points 64 at a time with every lane
+    // busy, the next 64 records requested while the current ones project.
+    uint32_t* q = sm.queue[wave];
+    uint32_t beg[kBinsPerWave], cnt[kBinsPerWave];
+    uint32_t longest = 0;
+#pragma unroll
+    for (int t = 0; t < kBinsPerWave; ++t) {
+      const int i = wave + 4 * t;
       const int b = ((v - i) % j.n_vpl + j.n_vpl) % j.n_vpl;  // (b + i) mod N == v
       const uint32_t end = min(j.bin_offsets[b + 1], j.n_points);
-      const uint32_t begin = min(j.bin_offsets[b], end);
-      for (uint32_t p0 = begin; p0 < end; p0 += 64) {
-        const uint32_t p = p0 + lane;
-        const bool hit = p < end && ((j.mask[p] >> i) & 1u);
-        const uint64_t bal = __ballot(hit);
-        if (hit) {
-          const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-          q[slot] = j.points[p];
+      beg[t] = min(j.bin_offsets[b], end);
+      cnt[t] = end - beg[t];
+      longest = max(longest, cnt[t]);
+    }
+    for (uint32_t s0 = 0; s0 < longest; s0 += kStep) {
+      uint32_t m[kBinsPerWave][4];
+#pragma unroll
+      for (int t = 0; t < kBinsPerWave; ++t)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t o = s0 + (uint32_t)(k * 64 + lane);
+          m[t][k] = o < cnt[t] ? (uint32_t)j.mask[beg[t] + o] : 0u;
         }
-        qn += __builtin_popcountll(bal);
-        if (qn >= 64) {
-          wave_sync();
-          splat_queued(j, sm, c, q[qn - 64 + lane], ox, oy, inv_n1d, side);
-          wave_sync();
-          qn -= 64;
+      int qn = 0;  // wave-uniform
+#pragma unroll
+      for (int t = 0; t < kBinsPerWave; ++t)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const bool hit = (m[t][k] >> (wave + 4 * t)) & 1u;
+          const uint64_t bal = __ballot(hit);
+          if (hit) {
+            const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            q[slot] = beg[t] + s0 + (uint32_t)(k * 64 + lane);
+          }
+          qn += __builtin_popcountll(bal);
+        }
+      wave_sync();
+      if (qn > 0) {
+        // indices past the queue end read its last entry (a valid point)
+        float4 rec = j.points[q[min(lane, qn - 1)]];
+        for (int c0 = 0; c0 < qn; c0 += 64) {
+          const float4 next = j.points[q[min(c0 + 64 + lane, qn - 1)]];
+          if (c0 + lane < qn) splat_queued(j, sm, c, rec, ox, oy, inv_n1d, side);
+          rec = next;
         }
       }
+      wave_sync();  // the queue is read before the next step refills it
     }
-    wave_sync();
-    if (lane < qn) splat_queued(j, sm, c, q[lane], ox, oy, inv_n1d, side);
   }
   __syncthreads();
   const size_t origin = (size_t)tile_y * S * j.atlas_side + (size_t)tile_x * S;
@@ -481,7 +509,7 @@ hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint
     if (e != hipSuccess) return e;
   }
   const int n_tiles = n1d * n1d;
-  hipLaunchKernelGGL(ism_tiles_kernel, dim3(xcd_grid((uint32_t)n_tiles, 4)), dim3(256), 0, s, j, n_tiles);
+  hipLaunchKernelGGL(ism_tiles_kernel, dim3(xcd_grid((uint32_t)n_tiles, 16)), dim3(256), 0, s, j, n_tiles);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !any) return e;
   hipLaunchKernelGGL(ism_spill_kernel, dim3(n_bins < 2048 ? n_bins : 2048), dim3(256), 0, s, j, n_bins);
@@ -496,7 +524,7 @@ hipError_t launch_batch_mask(hipStream_t s, const BatchFrame* frames, int n, uin
 
 hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_tiles_kernel, dim3(xcd_grid(total, 4)), dim3(256), 0, s, frames, n, total);
+  hipLaunchKernelGGL(batch_tiles_kernel, dim3(xcd_grid(total, 16)), dim3(256), 0, s, frames, n, total);
   return hipGetLastError();
 }
 

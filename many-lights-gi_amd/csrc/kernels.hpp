@@ -85,6 +85,10 @@ hipError_t launch_count_lit_pairs(hipStream_t s, const GatherArgs& a,
 hipError_t launch_gi_blur(hipStream_t s, const float* gi, const float* normal, const float* depth, int w, int h,
                           const float* projection16, float* out);
 
+// diag.hip: FP32 FMA rate probe (iters must be a multiple of 4)
+uint64_t valu_probe_fmas(int blocks, int iters);
+hipError_t launch_valu_probe(hipStream_t s, float* out, int blocks, int iters);
+
 // raycast.hip: G-buffer (rsm = 0: a = depth, b = normal) or RSM (rsm = 1:
 // a = diffuse, b = normal, c = depth) of a scene uploaded to the device
 }  // namespace mlgi

@@ -124,6 +124,12 @@ class Context:
     def sync(self) -> None:
         self.check(lib.mlgi_sync(self.h), "mlgi_sync")
 
+    def probe_valu(self) -> float:
+        """Measured FP32 vector FMA rate (lane FMAs per second)."""
+        v = C.c_double(0.0)
+        self.check(lib.mlgi_probe_valu(self.h, C.byref(v)), "mlgi_probe_valu")
+        return float(v.value)
+
     # -- profiling (PerfCounter analogue)
     def profile(self, on: bool = True) -> None:
         self.check(lib.mlgi_profile_enable(self.h, 1 if on else 0))

@@ -177,6 +177,7 @@ SIGNATURES = {
     "mlgi_vpl_extract": (C.c_int, [vp, P(Rsm), P(Params), P(Vpls)]),
     "mlgi_ism_splat": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas)]),
     "mlgi_splat_spill_stats": (C.c_int, [vp, P(C.c_uint32), P(C.c_int32)]),
+    "mlgi_probe_valu": (C.c_int, [vp, P(C.c_double)]),
     "mlgi_ism_pullpush": (C.c_int, [vp, P(Atlas), P(Params), P(IsmDepth)]),
     "mlgi_ism_process": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas), P(IsmDepth)]),
     "mlgi_final_gather": (C.c_int, [vp, P(GBuffer), P(Vpls), P(IsmDepth), P(Params), P(Image)]),
