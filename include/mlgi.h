@@ -224,6 +224,37 @@ int mlgi_frame(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_points* pts,
 int mlgi_gi_blur(mlgi_context* ctx, const mlgi_gbuffer* gb, const float* projection16,
                  const mlgi_image* gi, mlgi_image* out);
 
+/* Deferred composite (DeferredShadingStage::process,
+ * DeferredShadingStage.cpp:76-131; deferredshading.frag:47-81): direct light
+ * with a variance shadow map, GI ambient (gi * diffuse * occlusion),
+ * specular, Reinhard tone mapping (exposure) and gamma 2.2. Device buffers of
+ * width*height texels (row 0 = bottom); the VSM holds the moments
+ * (E[d], E[d^2]) of the light distance per texel, sampled bilinearly with a
+ * (0, 0) border. specular and occlusion may be NULL (read as 0 and 1).
+ * Matrices are column-major. Asynchronous. */
+typedef struct mlgi_composite_inputs {
+  const float* depth;      /* window depth */
+  const float* normal;     /* encoded n*0.5+0.5, 3 floats */
+  const float* diffuse;    /* sRGB albedo, 3 floats */
+  const float* specular;   /* sRGB, 3 floats, or NULL */
+  const float* gi;         /* irradiance, 3 floats */
+  const float* occlusion;  /* ambient occlusion factor, or NULL */
+  int32_t width, height;
+  const float* vsm;        /* vsm_width*vsm_height*2 moments */
+  int32_t vsm_width, vsm_height;
+} mlgi_composite_inputs;
+typedef struct mlgi_composite_params {
+  float view_projection_inverse[16];   /* inverse(P*V) of the camera */
+  float biased_light_view_projection[16];
+  float camera_eye[3];
+  float light_position[3];
+  float inverse_light_direction[3];    /* -normalize(lightDirection) */
+  float light_intensity;
+  float exposure;                      /* default 1 */
+} mlgi_composite_params;
+int mlgi_deferred_composite(mlgi_context* ctx, const mlgi_composite_inputs* in,
+                            const mlgi_composite_params* p, mlgi_image* out);
+
 /* ------------------------------------------------------------ HIP graphs */
 typedef struct mlgi_graph mlgi_graph;
 /* Capture one mlgi_frame with these descriptors into an instantiated HIP

@@ -99,6 +99,16 @@ void mlgi_scene_device_destroy(mlgi_scene_device* d);
 int mlgi_scene_device_render_gbuffer(mlgi_context* ctx, const mlgi_scene_device* d,
                                      const mlgi_camera* cam, int32_t width, int32_t height,
                                      float* depth, float* face_normal, float* vp_inv16);
+/* The point set of mlgi_scene_sample_points generated on the device
+ * (ism.tesc:11-23, ism.geom:49-69: the reference emits it per frame from the
+ * tessellation and geometry stages). records (n_points*4 floats) and
+ * bin_offsets (n_bins+1) are device buffers. Record i and its bin equal the
+ * host sampler's; within a bin the records come in no fixed order (the splat
+ * does not depend on it), bin_offsets equal the host's. Allocates its work
+ * buffers and waits for the stream: scene set-up, not per frame. */
+int mlgi_scene_device_sample_points(mlgi_context* ctx, const mlgi_scene_device* d, int64_t n_points,
+                                    int32_t n_bins, int32_t ism_count, uint64_t seed, float* records,
+                                    uint32_t* bin_offsets);
 /* diffuse (w*h*3), face_normal (w*h*3), depth (w*h); inv_bvp16 (host,
  * optional) receives inverse(bias * P * V). */
 int mlgi_scene_device_render_rsm(mlgi_context* ctx, const mlgi_scene_device* d,

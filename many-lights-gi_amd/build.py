@@ -45,6 +45,8 @@ SOURCES = [
     ("raycast.hip", HIP_FLAGS + EXACT),
     ("blur.hip", HIP_FLAGS + EXACT),
     ("diag.hip", HIP_FLAGS),
+    ("points.hip", HIP_FLAGS + EXACT),
+    ("composite.hip", HIP_FLAGS + EXACT),
     ("capi.cpp", HIP_FLAGS),
     ("server.cpp", HIP_FLAGS),
     ("scene.cpp", ["-ffp-contract=off"]),

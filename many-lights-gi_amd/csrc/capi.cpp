@@ -377,6 +377,42 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
                    "ism_splat");
 }
 
+int mlgi_deferred_composite(mlgi_context* c, const mlgi_composite_inputs* in, const mlgi_composite_params* p,
+                            mlgi_image* out) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!in || !p || !out || !out->rgb) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!in->depth || !in->normal || !in->diffuse || !in->gi || !in->vsm)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "composite input buffer is NULL");
+  if (in->width <= 0 || in->height <= 0 || in->width > 16384 || in->height > 16384 || in->vsm_width <= 0 ||
+      in->vsm_height <= 0 || in->vsm_width > 16384 || in->vsm_height > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "composite sizes out of range");
+  if (out->width != in->width || out->height != in->height)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "output %dx%d != input %dx%d", out->width, out->height, in->width,
+                in->height);
+  if (((uintptr_t)in->vsm & 7u) != 0) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "vsm must be 8-byte aligned");
+  mlgi::CompositeArgs a;
+  a.depth = in->depth;
+  a.normal = in->normal;
+  a.diffuse = in->diffuse;
+  a.specular = in->specular;
+  a.gi = in->gi;
+  a.occlusion = in->occlusion;
+  a.w = in->width;
+  a.h = in->height;
+  a.vsm = in->vsm;
+  a.vw = in->vsm_width;
+  a.vh = in->vsm_height;
+  std::memcpy(a.vp_inv, p->view_projection_inverse, sizeof a.vp_inv);
+  std::memcpy(a.blvp, p->biased_light_view_projection, sizeof a.blvp);
+  std::memcpy(a.eye, p->camera_eye, sizeof a.eye);
+  std::memcpy(a.light_pos, p->light_position, sizeof a.light_pos);
+  std::memcpy(a.inv_light_dir, p->inverse_light_direction, sizeof a.inv_light_dir);
+  a.light_intensity = p->light_intensity;
+  a.exposure = p->exposure;
+  a.out = out->rgb;
+  return hip_check(c, mlgi::launch_composite(c->stream, a), "deferred_composite");
+}
+
 int mlgi_splat_spill_stats(mlgi_context* c, uint32_t* spills, int32_t* overflowed) {
   if (!c) return MLGI_ERR_INVALID_ARGUMENT;
   if (!spills || !overflowed) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -1050,6 +1086,8 @@ struct mlgi_scene_device {
   mlgi_scene_impl::Node* nodes = nullptr;
   mlgi_scene_impl::Tri* tris = nullptr;
   int* order = nullptr;
+  double* cdf = nullptr;  // running triangle areas (point sampling)
+  double total_area = 0.0;
   int n_nodes = 0, n_tris = 0;
   mlgi_scene_impl::SceneView view() const {
     return mlgi_scene_impl::SceneView{nodes, tris, order, n_nodes, n_tris};
@@ -1064,6 +1102,7 @@ void mlgi_scene_device_destroy(mlgi_scene_device* d) {
   (void)hipFree(d->nodes);
   (void)hipFree(d->tris);
   (void)hipFree(d->order);
+  (void)hipFree(d->cdf);
   delete d;
 }
 
@@ -1088,12 +1127,47 @@ int mlgi_scene_upload(mlgi_context* c, const mlgi_scene* s, mlgi_scene_device** 
                    "hipMemcpy(tris)");
   if (!rc)
     rc = hip_check(c, hipMemcpy(d->order, v.order, sizeof(int) * v.n_tris, hipMemcpyHostToDevice), "hipMemcpy(order)");
+  if (!rc) {
+    std::vector<double> cdf(v.n_tris);
+    d->total_area = mlgi_scene_area_cdf(s, cdf.data());
+    rc = hip_check(c, hipMalloc(&d->cdf, sizeof(double) * v.n_tris), "hipMalloc(cdf)");
+    if (!rc)
+      rc = hip_check(c, hipMemcpy(d->cdf, cdf.data(), sizeof(double) * v.n_tris, hipMemcpyHostToDevice),
+                     "hipMemcpy(cdf)");
+  }
   if (rc) {
     mlgi_scene_device_destroy(d);
     return rc;
   }
   *out = d;
   return MLGI_OK;
+}
+
+int mlgi_scene_device_sample_points(mlgi_context* c, const mlgi_scene_device* d, int64_t n_points, int32_t n_bins,
+                                    int32_t ism_count, uint64_t seed, float* records, uint32_t* bin_offsets) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!d || !bin_offsets || (!records && n_points > 0)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n_points < 0 || n_points > (1ll << 31) - 1 || n_bins <= 0 || n_bins > (1 << 20) || ism_count <= 0)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "points %lld / bins %d / ism count %d out of range", (long long)n_points,
+                n_bins, ism_count);
+  const int n = (int)n_points;
+  const float r_packed = mlgi_scene_point_radius(d->total_area, n_points, ism_count);
+  float* tmp = nullptr;
+  int* bin = nullptr;
+  uint32_t* cnt = nullptr;
+  int rc = hip_check(c, hipMalloc(&tmp, sizeof(float) * 4 * (size_t)std::max(n, 1)), "hipMalloc(points tmp)");
+  if (!rc) rc = hip_check(c, hipMalloc(&bin, sizeof(int) * (size_t)std::max(n, 1)), "hipMalloc(points bin)");
+  if (!rc) rc = hip_check(c, hipMalloc(&cnt, sizeof(uint32_t) * 2 * (size_t)n_bins), "hipMalloc(points count)");
+  if (!rc)
+    rc = hip_check(c,
+                   mlgi::launch_sample_points(c->stream, d->tris, d->cdf, d->n_tris, d->total_area, seed, r_packed,
+                                              n_bins, n, tmp, bin, cnt, cnt + n_bins, records, bin_offsets),
+                   "sample_points");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (tmp) (void)hipFree(tmp);
+  if (bin) (void)hipFree(bin);
+  if (cnt) (void)hipFree(cnt);
+  return rc;
 }
 
 int mlgi_scene_device_render_gbuffer(mlgi_context* c, const mlgi_scene_device* d, const mlgi_camera* cam,

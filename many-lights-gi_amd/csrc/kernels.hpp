@@ -85,6 +85,25 @@ hipError_t launch_count_lit_pairs(hipStream_t s, const GatherArgs& a,
 hipError_t launch_gi_blur(hipStream_t s, const float* gi, const float* normal, const float* depth, int w, int h,
                           const float* projection16, float* out);
 
+// composite.hip: deferred composite (direct light + VSM shadow + GI ambient,
+// Reinhard, gamma 2.2). Matrices column-major; specular / occlusion may be null.
+struct CompositeArgs {
+  const float* depth;
+  const float* normal;
+  const float* diffuse;
+  const float* specular;
+  const float* gi;
+  const float* occlusion;
+  int w, h;
+  const float* vsm;  // moments (E[d], E[d^2]) per texel
+  int vw, vh;
+  float vp_inv[16], blvp[16];
+  float eye[3], light_pos[3], inv_light_dir[3];
+  float light_intensity, exposure;
+  float* out;
+};
+hipError_t launch_composite(hipStream_t s, const CompositeArgs& a);
+
 // diag.hip: FP32 FMA rate probe (iters must be a multiple of 4)
 uint64_t valu_probe_fmas(int blocks, int iters);
 hipError_t launch_valu_probe(hipStream_t s, float* out, int blocks, int iters);
@@ -95,8 +114,14 @@ hipError_t launch_valu_probe(hipStream_t s, float* out, int blocks, int iters);
 namespace mlgi_scene_impl {
 struct SceneView;
 struct RayCam;
+struct Tri;
 }  // namespace mlgi_scene_impl
 namespace mlgi {
+// points.hip: device point set of a scene (records grouped by bin). Work
+// buffers: tmp (4n floats), bin (n), count and cursor (n_bins each).
+hipError_t launch_sample_points(hipStream_t s, const mlgi_scene_impl::Tri* tris, const double* cdf, int nt,
+                                double total, uint64_t seed, float r_packed, int n_bins, int n, float* tmp, int* bin,
+                                uint32_t* count, uint32_t* cursor, float* records, uint32_t* offsets);
 hipError_t launch_raycast(hipStream_t st, const mlgi_scene_impl::SceneView& s,
                           const mlgi_scene_impl::RayCam& rc, int w, int h, int rsm, float* a, float* b,
                           float* c);

@@ -19,12 +19,8 @@ namespace {
 
 using namespace mlgi_scene_impl;
 
-// splitmix64: sequential generator and counter-based hash
-inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
+// splitmix64 sequential generator (mix64 and the counter-based draws of the
+// point sampler are in scene_trace.hpp)
 struct Rng {
   uint64_t s;
   explicit Rng(uint64_t seed) : s(seed) {}
@@ -33,11 +29,6 @@ struct Rng {
     return (double)(mix64(s) >> 11) * (1.0 / 9007199254740992.0);
   }
 };
-inline double counter_uniform(uint64_t seed, uint64_t i, uint64_t k) {
-  uint64_t z = mix64(seed ^ mix64(i * 4 + k + 0x632BE59BD9B4E019ull));
-  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
-}
-
 }  // namespace
 
 struct mlgi_scene {
@@ -365,40 +356,22 @@ int mlgi_scene_render_rsm(const mlgi_scene* s, const mlgi_camera* light, int32_t
 
 }  // extern "C"
 
-namespace {
-
-inline uint32_t f2u(float f) {
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  return u;
-}
-inline float u2f(uint32_t u) {
-  float f;
-  std::memcpy(&f, &u, 4);
-  return f;
-}
-// random.glsl:34-66: random(vec3) = floatConstruct(hash(uvec3 bits))
-inline uint32_t jhash(uint32_t x) {
-  x += (x << 10u);
-  x ^= (x >> 6u);
-  x += (x << 3u);
-  x ^= (x >> 11u);
-  x += (x << 15u);
-  return x;
-}
-inline float glsl_random3(float a, float b, float c) {
-  uint32_t m = jhash(f2u(a) ^ jhash(f2u(b)) ^ jhash(f2u(c)));
-  m &= 0x007FFFFFu;
-  m |= 0x3F800000u;
-  return u2f(m) - 1.0f;
-}
-// packUnorm4x8 component: round-to-nearest-even of clamp(c, 0, 1) * 255
-inline uint32_t unorm8(float c) {
-  float v = std::min(std::max(c, 0.0f), 1.0f) * 255.0f;
-  return (uint32_t)std::nearbyint(v);
+double mlgi_scene_area_cdf(const mlgi_scene* s, double* cdf) {
+  double acc = 0.0;
+  for (size_t i = 0; i < s->tris.size(); ++i) {
+    const Tri& t = s->tris[i];
+    acc += 0.5 * len(cross(t.v1 - t.v0, t.v2 - t.v0));
+    cdf[i] = acc;
+  }
+  return acc;
 }
 
-}  // namespace
+float mlgi_scene_point_radius(double total_area, int64_t n_points, int32_t ism_count) {
+  // each point stands for ism_count points: radius boosted by sqrt(ism_count) (ism.geom:60-63)
+  const double r_world = std::sqrt(total_area / ((double)std::max<int64_t>(n_points, 1) * M_PI)) *
+                         std::sqrt((double)ism_count);
+  return (float)(r_world / 25.0);
+}
 
 extern "C" int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, int32_t n_bins,
                                         int32_t ism_count, uint64_t seed, int32_t threads,
@@ -408,17 +381,8 @@ extern "C" int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, i
     return MLGI_ERR_INVALID_ARGUMENT;
   const size_t nt = s->tris.size();
   std::vector<double> cdf(nt);
-  double acc = 0.0;
-  for (size_t i = 0; i < nt; ++i) {
-    const Tri& t = s->tris[i];
-    acc += 0.5 * len(cross(t.v1 - t.v0, t.v2 - t.v0));
-    cdf[i] = acc;
-  }
-  const double total = acc;
-  // each point stands for ism_count points: radius boosted by sqrt(ism_count) (ism.geom:60-63)
-  const double r_world = std::sqrt(total / ((double)std::max<int64_t>(n_points, 1) * M_PI)) *
-                         std::sqrt((double)ism_count);
-  const float r_packed = (float)(r_world / 25.0);
+  const double total = mlgi_scene_area_cdf(s, cdf.data());
+  const float r_packed = mlgi_scene_point_radius(total, n_points, ism_count);
   std::vector<float> tmp((size_t)n_points * 4);
   std::vector<int32_t> bin((size_t)n_points);
   const int nth = nthreads(threads);
@@ -427,24 +391,9 @@ extern "C" int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, i
   for (int t = 0; t < nth; ++t) {
     pool.emplace_back([&, t]() {
       const int64_t b = t * chunk, e = std::min<int64_t>(n_points, b + chunk);
-      for (int64_t i = b; i < e; ++i) {
-        double u0 = counter_uniform(seed, (uint64_t)i, 0) * total;
-        size_t ti = (size_t)(std::upper_bound(cdf.begin(), cdf.end(), u0) - cdf.begin());
-        if (ti >= nt) ti = nt - 1;
-        const Tri& tr = s->tris[ti];
-        double r1 = std::sqrt(counter_uniform(seed, (uint64_t)i, 1));
-        double r2 = counter_uniform(seed, (uint64_t)i, 2);
-        D3 p = tr.v0 * (1.0 - r1) + tr.v1 * (r1 * (1.0 - r2)) + tr.v2 * (r1 * r2);
-        float px = (float)p.x, py = (float)p.y, pz = (float)p.z;
-        uint32_t bits = unorm8((float)tr.n.x * 0.5f + 0.5f) | (unorm8((float)tr.n.y * 0.5f + 0.5f) << 8) |
-                        (unorm8((float)tr.n.z * 0.5f + 0.5f) << 16) | (unorm8(r_packed) << 24);
-        tmp[i * 4 + 0] = px;
-        tmp[i * 4 + 1] = py;
-        tmp[i * 4 + 2] = pz;
-        tmp[i * 4 + 3] = u2f(bits);
-        int bi = (int)(glsl_random3(px, py, pz) * (float)n_bins);
-        bin[i] = std::min(std::max(bi, 0), n_bins - 1);
-      }
+      for (int64_t i = b; i < e; ++i)
+        sample_point(s->tris.data(), cdf.data(), (int)nt, total, seed, (uint64_t)i, r_packed, n_bins, &tmp[i * 4],
+                     &bin[i]);
     });
   }
   for (auto& th : pool) th.join();

@@ -204,6 +204,78 @@ MLGI_HD inline void shade_rsm(const SceneView& s, const RayCam& rc, int x, int y
   for (int k = 0; k < 3; ++k) diffuse[p * 3 + k] = in_cone ? t.albedo[k] : 0.0f;
 }
 
+// ------------------------------------------------------------ point sampler
+// Area-proportional surface points (the reference's tessellation + geometry
+// pass output, ism.tesc:11-23, ism.geom:49-69), shared by the host sampler
+// (scene.cpp) and the device sampler (points.hip) so both produce the same
+// records. Randomness is counter based: draw k of point i is a splitmix64
+// hash of (seed, i, k), so every point is independent of every other.
+
+MLGI_HD inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+MLGI_HD inline double counter_uniform(uint64_t seed, uint64_t i, uint64_t k) {
+  const uint64_t z = mix64(seed ^ mix64(i * 4 + k + 0x632BE59BD9B4E019ull));
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+// random.glsl:34-66: random(vec3) = floatConstruct(hash(uvec3 bits))
+MLGI_HD inline uint32_t jhash(uint32_t x) {
+  x += (x << 10u);
+  x ^= (x >> 6u);
+  x += (x << 3u);
+  x ^= (x >> 11u);
+  x += (x << 15u);
+  return x;
+}
+MLGI_HD inline float glsl_random3(float a, float b, float c) {
+  uint32_t m = jhash(__builtin_bit_cast(uint32_t, a) ^ jhash(__builtin_bit_cast(uint32_t, b)) ^
+                     jhash(__builtin_bit_cast(uint32_t, c)));
+  m &= 0x007FFFFFu;
+  m |= 0x3F800000u;
+  return __builtin_bit_cast(float, m) - 1.0f;
+}
+// packUnorm4x8 component: round-to-nearest-even of clamp(c, 0, 1) * 255
+MLGI_HD inline uint32_t unorm8(float c) {
+  const float v = (c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c)) * 255.0f;
+  return (uint32_t)rintf(v);
+}
+// std::upper_bound over cdf[0, n): first index with cdf[i] > u
+MLGI_HD inline int upper_bound_d(const double* cdf, int n, double u) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] > u)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+// Point i: a triangle drawn by area, a uniform point on it, its record
+// (pos, packUnorm4x8(n * 0.5 + 0.5, r / 25)) and its bin
+// int(random(pos) * n_bins), clamped to [0, n_bins).
+MLGI_HD inline void sample_point(const Tri* tris, const double* cdf, int nt, double total, uint64_t seed,
+                                 uint64_t i, float r_packed, int n_bins, float rec[4], int* bin) {
+  const double u0 = counter_uniform(seed, i, 0) * total;
+  int ti = upper_bound_d(cdf, nt, u0);
+  if (ti >= nt) ti = nt - 1;
+  const Tri& tr = tris[ti];
+  const double r1 = sqrt(counter_uniform(seed, i, 1));
+  const double r2 = counter_uniform(seed, i, 2);
+  const D3 p = tr.v0 * (1.0 - r1) + tr.v1 * (r1 * (1.0 - r2)) + tr.v2 * (r1 * r2);
+  const float px = (float)p.x, py = (float)p.y, pz = (float)p.z;
+  const uint32_t bits = unorm8((float)tr.n.x * 0.5f + 0.5f) | (unorm8((float)tr.n.y * 0.5f + 0.5f) << 8) |
+                        (unorm8((float)tr.n.z * 0.5f + 0.5f) << 16) | (unorm8(r_packed) << 24);
+  rec[0] = px;
+  rec[1] = py;
+  rec[2] = pz;
+  rec[3] = __builtin_bit_cast(float, bits);
+  const int bi = (int)(glsl_random3(px, py, pz) * (float)n_bins);
+  *bin = bi < 0 ? 0 : (bi > n_bins - 1 ? n_bins - 1 : bi);
+}
+
 }  // namespace mlgi_scene_impl
 
 struct mlgi_scene;
@@ -214,3 +286,7 @@ mlgi_scene_impl::SceneView mlgi_scene_view(const mlgi_scene* s);
 bool mlgi_scene_ray_cam(const struct mlgi_camera* cam, mlgi_scene_impl::RayCam* out);
 // inverse(bias * P * V) of a light as floats (column-major).
 bool mlgi_scene_inv_biased_vp(const struct mlgi_camera* light, float* out16);
+// Running triangle areas (cdf[i] = sum of areas 0..i, in order); returns the total.
+double mlgi_scene_area_cdf(const mlgi_scene* s, double* cdf);
+// Packed point radius r / 25 for n points standing for ism_count points each.
+float mlgi_scene_point_radius(double total_area, int64_t n_points, int32_t ism_count);

@@ -312,6 +312,15 @@ class DeviceScene:
                                                             inv.ctypes.data), "mlgi_scene_device_render_gbuffer")
         return inv
 
+    def sample_points(self, n_points: int, n_bins: int, ism_count: int, seed: int):
+        """Point set on the device: (records (n, 4) f32, bin_offsets (n_bins + 1) u32) device arrays."""
+        rec = self.ctx.empty((max(n_points, 1), 4), np.float32)
+        off = self.ctx.empty((n_bins + 1,), np.uint32)
+        self.ctx.check(lib.mlgi_scene_device_sample_points(self.ctx.h, self.h, n_points, n_bins, ism_count, seed,
+                                                           C.c_void_p(rec.ptr), C.c_void_p(off.ptr)),
+                       "mlgi_scene_device_sample_points")
+        return rec, off
+
     def render_rsm(self, light: Camera, diffuse: "DeviceArray", normal: "DeviceArray", depth: "DeviceArray"):
         """Into device arrays; returns inverse(bias*P*V) (host)."""
         height, width = depth.shape
@@ -483,6 +492,30 @@ class GIStage:
         self.ctx.check(lib.mlgi_count_lit_pairs(self.ctx.h, C.byref(gbuffer), C.byref(self.vpl_processor.desc()),
                                                 C.byref(self.params), C.byref(out)), "mlgi_count_lit_pairs")
         return int(out.value)
+
+
+def deferred_composite(ctx: "Context", depth, normal, diffuse, gi, vsm, out, *, vp_inv, biased_light_vp, eye,
+                       light_pos, inv_light_dir, light_intensity: float, exposure: float = 1.0, specular=None,
+                       occlusion=None) -> None:
+    """DeferredShadingStage::process on device arrays: depth (h, w), normal / diffuse / gi / out (h, w, 3),
+    vsm (vh, vw, 2), optional specular (h, w, 3) and occlusion (h, w)."""
+    h, w = depth.shape
+    vh, vw = vsm.shape[:2]
+    i = N.CompositeInputs()
+    i.depth, i.normal, i.diffuse, i.gi, i.vsm = depth.ptr, normal.ptr, diffuse.ptr, gi.ptr, vsm.ptr
+    i.specular = specular.ptr if specular is not None else None
+    i.occlusion = occlusion.ptr if occlusion is not None else None
+    i.width, i.height, i.vsm_width, i.vsm_height = w, h, vw, vh
+    p = N.CompositeParams()
+    p.view_projection_inverse[:] = [float(x) for x in np.ravel(vp_inv)]
+    p.biased_light_view_projection[:] = [float(x) for x in np.ravel(biased_light_vp)]
+    p.camera_eye[:] = [float(x) for x in eye]
+    p.light_position[:] = [float(x) for x in light_pos]
+    p.inverse_light_direction[:] = [float(x) for x in inv_light_dir]
+    p.light_intensity, p.exposure = light_intensity, exposure
+    o = N.Image()
+    o.rgb, o.width, o.height = out.ptr, w, h
+    ctx.check(lib.mlgi_deferred_composite(ctx.h, C.byref(i), C.byref(p), C.byref(o)), "mlgi_deferred_composite")
 
 
 class Batch:

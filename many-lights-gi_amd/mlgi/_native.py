@@ -97,6 +97,19 @@ class Image(C.Structure):
     _fields_ = [("rgb", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32)]
 
 
+class CompositeInputs(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("diffuse", C.c_void_p), ("specular", C.c_void_p),
+                ("gi", C.c_void_p), ("occlusion", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
+                ("vsm", C.c_void_p), ("vsm_width", C.c_int32), ("vsm_height", C.c_int32)]
+
+
+class CompositeParams(C.Structure):
+    _fields_ = [("view_projection_inverse", C.c_float * 16), ("biased_light_view_projection", C.c_float * 16),
+                ("camera_eye", C.c_float * 3), ("light_position", C.c_float * 3),
+                ("inverse_light_direction", C.c_float * 3), ("light_intensity", C.c_float),
+                ("exposure", C.c_float)]
+
+
 class FrameDesc(C.Structure):
     _fields_ = [("rsm", Rsm), ("points", Points), ("gbuffer", GBuffer), ("params", Params),
                 ("vpls", Vpls), ("atlas", Atlas), ("ism", IsmDepth), ("image", Image)]
@@ -178,6 +191,7 @@ SIGNATURES = {
     "mlgi_ism_splat": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas)]),
     "mlgi_splat_spill_stats": (C.c_int, [vp, P(C.c_uint32), P(C.c_int32)]),
     "mlgi_probe_valu": (C.c_int, [vp, P(C.c_double)]),
+    "mlgi_deferred_composite": (C.c_int, [vp, P(CompositeInputs), P(CompositeParams), P(Image)]),
     "mlgi_ism_pullpush": (C.c_int, [vp, P(Atlas), P(Params), P(IsmDepth)]),
     "mlgi_ism_process": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas), P(IsmDepth)]),
     "mlgi_final_gather": (C.c_int, [vp, P(GBuffer), P(Vpls), P(IsmDepth), P(Params), P(Image)]),
@@ -218,6 +232,7 @@ SIGNATURES = {
     "mlgi_scene_device_destroy": (None, [vp]),
     "mlgi_scene_device_render_gbuffer": (C.c_int, [vp, vp, P(Camera), i32, i32, vp, vp, vp]),
     "mlgi_scene_device_render_rsm": (C.c_int, [vp, vp, P(Camera), i32, i32, vp, vp, vp, vp]),
+    "mlgi_scene_device_sample_points": (C.c_int, [vp, vp, C.c_int64, i32, i32, C.c_uint64, vp, vp]),
     "mlgi_scene_sample_points": (C.c_int, [vp, C.c_int64, i32, i32, C.c_uint64, i32, vp, vp]),
 }
 

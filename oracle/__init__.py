@@ -67,6 +67,8 @@ _sigs = {
                                             vp, i32, f32, f32, f32, i32, i32, i32, vp, vp]),
     "oracle_gi_blur_f32": (None, [vp, vp, vp, i32, i32, vp, vp]),
     "oracle_gi_blur_f64": (None, [vp, vp, vp, i32, i32, vp, vp]),
+    "oracle_deferred_composite_f64": (None, [vp, vp, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp, vp, vp, vp, vp,
+                                             f32, f32, vp]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(_lib, _n)
@@ -226,4 +228,22 @@ def gi_blur(gi, normal, depth, projection, dtype=np.float64):
     out = np.empty((h, w, 3), dtype)
     fn = _lib.oracle_gi_blur_f64 if dtype == np.float64 else _lib.oracle_gi_blur_f32
     fn(_p(g), _p(nm), _p(dp), w, h, _p(_c(projection, np.float32)), _p(out))
+    return out
+
+
+def deferred_composite(depth, normal, diffuse, gi, vsm, *, vp_inv, biased_light_vp, eye, light_pos, inv_light_dir,
+                       light_intensity, exposure=1.0, specular=None, occlusion=None):
+    """deferredshading.frag:47-81 (VSM shadow, Reinhard, gamma 2.2) in FP64; returns (h, w, 3)."""
+    dp = _c(depth, np.float32)
+    h, w = dp.shape
+    v = _c(vsm, np.float32)
+    vh, vw = v.shape[:2]
+    args = [_c(normal, np.float32), _c(diffuse, np.float32),
+            None if specular is None else _c(specular, np.float32), _c(gi, np.float32),
+            None if occlusion is None else _c(occlusion, np.float32)]
+    mats = [_c(vp_inv, np.float32), _c(biased_light_vp, np.float32), _c(eye, np.float32), _c(light_pos, np.float32),
+            _c(inv_light_dir, np.float32)]
+    out = np.empty((h, w, 3), np.float64)
+    _lib.oracle_deferred_composite_f64(_p(dp), *[_p(a) for a in args], w, h, _p(v), vw, vh, *[_p(m) for m in mats],
+                                       light_intensity, exposure, _p(out))
     return out

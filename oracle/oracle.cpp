@@ -673,7 +673,123 @@ void gi_blur(const float* gi, const float* normal, const float* depth, int w, in
   gi_blur_pass<T>(tmp.data(), normal, depth, w, h, proj, 0, 1, out);
 }
 
+// ------------------------------------------------------ deferred composite
+// deferredshading.frag:47-81 with shadowmapping.glsl:6-34 (VSM) and
+// srgb_utils.glsl (gamma 2.2), as DeferredShadingStage::process binds it
+// (DeferredShadingStage.cpp:76-131). Choices:
+//   * the texel (x, y) is sampled at its centre, uv = ((x+0.5)/w, (y+0.5)/h);
+//     the G-buffer, GI and occlusion textures are NEAREST, so they read
+//     texel (x, y);
+//   * the world position is inverse(P*V) applied to the pixel-centre NDC
+//     (2uv-1, 2 depth-1, 1), divided by w: the same point as
+//     viewInverted * (linearDepth * viewRay) for a perspective camera;
+//   * the VSM moments are sampled bilinearly (GL_LINEAR, texel centres at
+//     (i+0.5)/W) with CLAMP_TO_BORDER, border (0, 0) (RasterizationStage.cpp:97-99);
+//   * pow(x, y) = exp2(y log2 x) with pow(0, y) = 0 for y > 0;
+//   * a null specular buffer reads 0, a null occlusion buffer reads 1.
+template <class T> T c_clamp01(T x) { return std::min(std::max(x, T(0)), T(1)); }
+template <class T> T c_smoothstep(T e0, T e1, T x) {
+  const T t = c_clamp01((x - e0) / (e1 - e0));
+  return t * t * (T(3) - T(2) * t);
+}
+template <class T> T c_pow(T x, T y) { return x > T(0) ? std::pow(x, y) : T(0); }
+
+template <class T>
+T c_vsm(T mx, T my, T compare) {
+  const T p = c_smoothstep(compare - T(0.001f), compare, mx);
+  const T variance = std::max(my - mx * mx, T(0.001f));
+  const T d = compare - mx;
+  const T p_max = c_clamp01((variance / (variance + d * d) - T(0.1f)) / (T(1.0) - T(0.1f)));  // linstep(0.1, 1, .)
+  return c_clamp01(std::max(p, p_max));
+}
+
+template <class T>
+void c_moments(const float* vsm, int W, int H, T u, T v, T* mx, T* my) {
+  const T s = u * T(W) - T(0.5), t = v * T(H) - T(0.5);
+  const T fs = std::floor(s), ft = std::floor(t);
+  T m[2] = {T(0), T(0)};
+  *mx = *my = T(0);
+  if (!(std::fabs(fs) < T(1e9) && std::fabs(ft) < T(1e9))) return;  // far outside (or w = 0): border
+  const int i0 = (int)fs, j0 = (int)ft;
+  const T a = s - fs, b = t - ft;
+  for (int k = 0; k < 4; ++k) {
+    const int i = i0 + (k & 1), j = j0 + (k >> 1);
+    const T w = ((k & 1) ? a : T(1) - a) * ((k >> 1) ? b : T(1) - b);
+    if (i < 0 || j < 0 || i >= W || j >= H) continue;  // border (0, 0)
+    const size_t q = ((size_t)j * W + i) * 2;
+    m[0] += w * T(vsm[q]);
+    m[1] += w * T(vsm[q + 1]);
+  }
+  *mx = m[0];
+  *my = m[1];
+}
+
+template <class T>
+void deferred_composite(const float* depth, const float* normal, const float* diffuse, const float* specular,
+                        const float* gi, const float* occlusion, int w, int h, const float* vsm, int vw, int vh,
+                        const float* vp_inv, const float* blvp, const float* eye, const float* light_pos,
+                        const float* inv_light_dir, float light_intensity, float exposure, T* out) {
+  const T gamma = T(2.2f);
+  for (int y = 0; y < h; ++y) {
+    for (int x = 0; x < w; ++x) {
+      const size_t c = (size_t)y * w + x;
+      const T u = (T(x) + T(0.5)) / T(w), v = (T(y) + T(0.5)) / T(h);
+      const T ndc[4] = {u * T(2) - T(1), v * T(2) - T(1), T(depth[c]) * T(2) - T(1), T(1)};
+      T r[4];
+      for (int row = 0; row < 4; ++row) {
+        r[row] = T(0);
+        for (int col = 0; col < 4; ++col) r[row] += T(vp_inv[col * 4 + row]) * ndc[col];
+      }
+      const T wx = r[0] / r[3], wy = r[1] / r[3], wz = r[2] / r[3];
+      const T nx = T(normal[c * 3]) * T(2) - T(1), ny = T(normal[c * 3 + 1]) * T(2) - T(1),
+              nz = T(normal[c * 3 + 2]) * T(2) - T(1);
+      const T lx = T(inv_light_dir[0]), ly = T(inv_light_dir[1]), lz = T(inv_light_dir[2]);
+      T vx = T(eye[0]) - wx, vy = T(eye[1]) - wy, vz = T(eye[2]) - wz;
+      const T vl = std::sqrt(vx * vx + vy * vy + vz * vz);
+      vx /= vl, vy /= vl, vz /= vl;
+      T hx = lx + vx, hy = ly + vy, hz = lz + vz;
+      const T hl = std::sqrt(hx * hx + hy * hy + hz * hz);
+      hx /= hl, hy /= hl, hz /= hl;
+      const T ndotl = nx * lx + ny * ly + nz * lz;
+      const T ndoth = nx * hx + ny * hy + nz * hz;
+      T sc[4];
+      const T wp[4] = {wx, wy, wz, T(1)};
+      for (int row = 0; row < 4; ++row) {
+        sc[row] = T(0);
+        for (int col = 0; col < 4; ++col) sc[row] += T(blvp[col * 4 + row]) * wp[col];
+      }
+      T mx, my;
+      c_moments<T>(vsm, vw, vh, sc[0] / sc[3], sc[1] / sc[3], &mx, &my);
+      const T dlx = wx - T(light_pos[0]), dly = wy - T(light_pos[1]), dlz = wz - T(light_pos[2]);
+      const T dist = std::sqrt(dlx * dlx + dly * dly + dlz * dlz);
+      T shadow = c_vsm(mx, my, dist);
+      shadow *= sc[3] > T(0) ? T(1) : T(0);  // step(0, sign(w))
+      const T occ = occlusion ? T(occlusion[c]) : T(1);
+      const T spec_f = c_pow(std::max(T(0), ndoth), T(20));
+      for (int k = 0; k < 3; ++k) {
+        const T dif = c_pow(T(diffuse[c * 3 + k]), gamma);
+        const T spc = specular ? c_pow(T(specular[c * 3 + k]), gamma) : T(0);
+        const T ambient = T(gi[c * 3 + k]) * dif * occ;
+        const T dterm = dif * (std::max(T(0), ndotl) * shadow) * T(light_intensity);
+        const T sterm = T(0.75f) * spc * spec_f * shadow;
+        T col = (ambient + dterm + sterm) * T(exposure);
+        col = col / (T(1) + col);
+        out[c * 3 + k] = c_pow(col, T(1) / gamma);
+      }
+    }
+  }
+}
+
 extern "C" {
+
+void oracle_deferred_composite_f64(const float* depth, const float* normal, const float* diffuse,
+                                   const float* specular, const float* gi, const float* occlusion, int w, int h,
+                                   const float* vsm, int vw, int vh, const float* vp_inv, const float* blvp,
+                                   const float* eye, const float* light_pos, const float* inv_light_dir,
+                                   float light_intensity, float exposure, double* out) {
+  deferred_composite<double>(depth, normal, diffuse, specular, gi, occlusion, w, h, vsm, vw, vh, vp_inv, blvp, eye,
+                             light_pos, inv_light_dir, light_intensity, exposure, out);
+}
 
 uint32_t oracle_pack3sn_f32(float x, float y, float z) { return pack3sn<float>(x, y, z); }
 uint32_t oracle_pack3sn_f64(double x, double y, double z) { return pack3sn<double>(x, y, z); }

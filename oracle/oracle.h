@@ -127,6 +127,14 @@ void oracle_gi_blur_f32(const float* gi, const float* normal, const float* depth
 void oracle_gi_blur_f64(const float* gi, const float* normal, const float* depth, int w, int h,
                         const float* projection16, double* out);
 
+/* Deferred composite (deferredshading.frag:47-81, VSM shadow, Reinhard,
+ * gamma 2.2), FP64. specular / occlusion may be NULL (0 / 1). */
+void oracle_deferred_composite_f64(const float* depth, const float* normal, const float* diffuse,
+                                   const float* specular, const float* gi, const float* occlusion, int w, int h,
+                                   const float* vsm, int vw, int vh, const float* vp_inv, const float* blvp,
+                                   const float* eye, const float* light_pos, const float* inv_light_dir,
+                                   float light_intensity, float exposure, double* out);
+
 #ifdef __cplusplus
 }
 #endif
