@@ -20,6 +20,51 @@ __device__ __forceinline__ int batch_find(const BatchFrame* __restrict__ f, int 
   return lo;
 }
 
+// Loads and stores through global-address-space pointers. Pointers read from
+// a frame table (or any struct in memory) are generic, and generic accesses
+// compile to flat instructions, which count against both the vector-memory
+// and the LDS wait counters: every LDS wait then also waits for outstanding
+// global loads. These helpers make such accesses global_load / global_store.
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+// 16-byte loads: through the native vector type (the HIP vector class would
+// be copied through a generic reference, which is flat again).
+__device__ __forceinline__ float4 gload4(const float4* p) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f v = *(const __attribute__((address_space(1))) v4f*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gload4(const uint4* p) {
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+  const v4u v = *(const __attribute__((address_space(1))) v4u*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore4(uint4* p, uint4 v) {
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+  v4u w = {v.x, v.y, v.z, v.w};
+  *(__attribute__((address_space(1))) v4u*)p = w;
+}
+// Relaxed, agent-scope atomics on global memory (no return value needed
+// except for the add).
+__device__ __forceinline__ uint32_t gatomic_add(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)p, v, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gatomic_min(uint32_t* p, uint32_t v) {
+  (void)__hip_atomic_fetch_min((__attribute__((address_space(1))) uint32_t*)p, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gatomic_or(uint32_t* p, uint32_t v) {
+  (void)__hip_atomic_fetch_or((__attribute__((address_space(1))) uint32_t*)p, v, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+
 constexpr int kBatchXcds = 8;
 
 // Workgroups are dispatched to the XCDs round robin. A persistent workgroup

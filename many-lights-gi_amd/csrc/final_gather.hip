@@ -50,7 +50,7 @@ struct VplLds {
 
 __device__ __forceinline__ VplLds load_vpl(const float* rec) {
   const float4* r = reinterpret_cast<const float4*>(rec);
-  const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
+  const float4 a = gload4(r), b = gload4(r + 1), c = gload4(r + 2), d = gload4(r + 3), e = gload4(r + 4);
   return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
 }
 
@@ -113,9 +113,9 @@ __device__ __forceinline__ Pixel load_pixel(const GatherArgs& a, int cx, int cy,
   p.valid = p.x < a.width && p.y < a.height;
   const int sx = p.valid ? p.x : 0, sy = p.valid ? p.y : 0;
   const size_t idx = (size_t)sy * a.width + sx;
-  const float d = a.depth[idx];
+  const float d = gload(a.depth + idx);
   p.depth = d;
-  const float n0 = a.normal[idx * 3 + 0], n1 = a.normal[idx * 3 + 1], n2 = a.normal[idx * 3 + 2];
+  const float n0 = gload(a.normal + idx * 3), n1 = gload(a.normal + idx * 3 + 1), n2 = gload(a.normal + idx * 3 + 2);
   const float u = (float)sx / (float)a.width, v = (float)sy / (float)a.height;
   const float c0 = u * 2.0f - 1.0f, c1 = v * 2.0f - 1.0f, c2 = d * 2.0f - 1.0f;
   const float* m = a.vp_inv.m;
@@ -285,7 +285,7 @@ __device__ __forceinline__ float pair_weight(const GatherArgs& a, float px, floa
     const int tile_xi = vi & (a.n1d - 1);
     const int gx = (tile_xi * a.ism_side + tx) & lc.amask;  // GL_REPEAT wrap
     const int gy = (tile_y * a.ism_side + ty) & lc.amask;
-    const float occ = (float)a.ism[(uint32_t)((gy << lc.alog) | gx)];
+    const float occ = (float)gload(a.ism + (uint32_t)((gy << lc.alog) | gx));
     // smoothstep(1 - bias, 1, 1 - (dist/zFar - occ)): t = 1 + (occ - dist/zFar) / bias
     float t = __builtin_fmaf(occ, lc.k2, __builtin_fmaf(-dist, lc.k1, 1.0f));
     t = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
@@ -432,9 +432,9 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   }
   if (p.valid) {
     float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
-    o[0] = ar * a.out_scale;
-    o[1] = ag * a.out_scale;
-    o[2] = ab * a.out_scale;
+    gstore(o, ar * a.out_scale);
+    gstore(o + 1, ag * a.out_scale);
+    gstore(o + 2, ab * a.out_scale);
   }
 }
 
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
 // 1080p: each queued pair moves ~130 bytes through LDS, and the kernel
 // becomes LDS-bandwidth bound.)
 template <bool kMixed>
-__global__ __launch_bounds__(256) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
                                                            uint32_t total) {
   __shared__ VplLds vtiles[4][64];
   const int lane = threadIdx.x & 63;
@@ -516,9 +516,9 @@ __global__ __launch_bounds__(256) void batch_gather_kernel(const BatchFrame* __r
     }
     if (p.valid) {
       float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
-      o[0] = ar * a.out_scale;
-      o[1] = ag * a.out_scale;
-      o[2] = ab * a.out_scale;
+      gstore(o, ar * a.out_scale);
+      gstore(o + 1, ag * a.out_scale);
+      gstore(o + 2, ab * a.out_scale);
     }
   }
 }

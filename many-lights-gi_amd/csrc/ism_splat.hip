@@ -62,7 +62,8 @@ struct Candidate {
 
 __device__ __forceinline__ Candidate load_candidate(const float* __restrict__ vpl, int idx, int n1d) {
   const float4* r = reinterpret_cast<const float4*>(vpl + (size_t)idx * kVplFloats);
-  const float4 r0 = r[kVplPos / 4], r1 = r[kVplNormal / 4], r2 = r[kVplS / 4], r3 = r[kVplU / 4];
+  const float4 r0 = gload4(r + kVplPos / 4), r1 = gload4(r + kVplNormal / 4), r2 = gload4(r + kVplS / 4),
+               r3 = gload4(r + kVplU / 4);
   Candidate c;
   c.px = r0.x; c.py = r0.y; c.pz = r0.z;
   c.nx = r1.x; c.ny = r1.y; c.nz = r1.z;
@@ -166,13 +167,13 @@ __device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restr
                                           int b) {
   const int lane = threadIdx.x;
   // clamp to the point count so malformed offsets cannot index past the buffer
-  const uint32_t end = min(bin_offsets[b + 1], n_points);
-  const uint32_t begin = min(bin_offsets[b], end);
+  const uint32_t end = min(gload(bin_offsets + b + 1), n_points);
+  const uint32_t begin = min(gload(bin_offsets + b), end);
   // loads past the bin read its last point (or point 0; n_points > 0) and
   // are discarded: no select of a record, which would go through scratch
   const uint32_t last = end > 0 ? end - 1 : 0;
   uint32_t p = begin + lane;
-  float4 rec = points[min(p, last)];
+  float4 rec = gload4(points + min(p, last));
   stage_candidates(cand, vpl, n_vpl, n1d, b);
   __syncthreads();
 
@@ -180,7 +181,7 @@ __device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restr
   const float side = (float)atlas_side;
   while (p < end) {
     const uint32_t pn = p + kThreads;
-    const float4 next = points[min(pn, last)];
+    const float4 next = gload4(points + min(pn, last));
     const f3 pos = dev::make(rec.x, rec.y, rec.z);
     const uint32_t rb = radius_byte_of(rec.w);
     uint32_t keep = keep_mask(cand, rec);
@@ -189,8 +190,7 @@ __device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restr
       keep &= keep - 1u;
       uint32_t tx, ty, value;
       if (!project(cand[i], pos, rb, inv_n1d, side, z_far, tx, ty, value)) continue;
-      __hip_atomic_fetch_min(atlas + (size_t)ty * (uint32_t)atlas_side + tx, value, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+      gatomic_min(atlas + (size_t)ty * (uint32_t)atlas_side + tx, value);
     }
     rec = next;
     p = pn;
@@ -217,18 +217,18 @@ __device__ __forceinline__ void mask_bin(Candidate* cand, const float4* __restri
                                          uint16_t* __restrict__ mask, uint32_t* __restrict__ spill_state,
                                          int b) {
   const int lane = threadIdx.x;
-  const uint32_t end = min(bin_offsets[b + 1], n_points);
-  const uint32_t begin = min(bin_offsets[b], end);
+  const uint32_t end = min(gload(bin_offsets + b + 1), n_points);
+  const uint32_t begin = min(gload(bin_offsets + b), end);
   const uint32_t last = end > 0 ? end - 1 : 0;  // as in splat_bin
   uint32_t p = begin + lane;
-  float4 rec = points[min(p, last)];
-  if (b == 0 && lane < 2) spill_state[lane] = 0u;
+  float4 rec = gload4(points + min(p, last));
+  if (b == 0 && lane < 2) gstore(spill_state + lane, 0u);
   stage_candidates(cand, vpl, n_vpl, n1d, b);
   __syncthreads();
   while (p < end) {
     const uint32_t pn = p + 256;
-    const float4 next = points[min(pn, last)];
-    mask[p] = (uint16_t)keep_mask(cand, rec);
+    const float4 next = gload4(points + min(pn, last));
+    gstore(mask + p, (uint16_t)keep_mask(cand, rec));
     rec = next;
     p = pn;
   }
@@ -273,11 +273,11 @@ __device__ __forceinline__ void splat_queued(const TileJob& j, TileLds& sm, cons
   if (lx < (uint32_t)j.ism_side && ly < (uint32_t)j.ism_side) {
     __hip_atomic_fetch_min(&sm.tile[ly * j.ism_side + lx], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else {
-    const uint32_t k = __hip_atomic_fetch_add(&j.spill_state[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t k = gatomic_add(j.spill_state, 1u);
     if (k < kSpillCap)
-      j.spill[k] = make_uint2(ty * (uint32_t)j.atlas_side + tx, value);
+      gstore(reinterpret_cast<uint64_t*>(j.spill + k), (uint64_t)(ty * (uint32_t)j.atlas_side + tx) | ((uint64_t)value << 32));
     else
-      __hip_atomic_fetch_or(&j.spill_state[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      gatomic_or(j.spill_state + 1, 1u);
   }
 }
 
@@ -310,8 +310,8 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v)
     for (int t = 0; t < kBinsPerWave; ++t) {
       const int i = wave + 4 * t;
       const int b = ((v - i) % j.n_vpl + j.n_vpl) % j.n_vpl;  // (b + i) mod N == v
-      const uint32_t end = min(j.bin_offsets[b + 1], j.n_points);
-      beg[t] = min(j.bin_offsets[b], end);
+      const uint32_t end = min(gload(j.bin_offsets + b + 1), j.n_points);
+      beg[t] = min(gload(j.bin_offsets + b), end);
       cnt[t] = end - beg[t];
       longest = max(longest, cnt[t]);
     }
@@ -322,7 +322,7 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint32_t o = s0 + (uint32_t)(k * 64 + lane);
-          m[t][k] = o < cnt[t] ? (uint32_t)j.mask[beg[t] + o] : 0u;
+          m[t][k] = o < cnt[t] ? (uint32_t)gload(j.mask + beg[t] + o) : 0u;
         }
       int qn = 0;  // wave-uniform
 #pragma unroll
@@ -341,9 +341,9 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v)
       wave_sync();
       if (qn > 0) {
         // indices past the queue end read its last entry (a valid point)
-        float4 rec = j.points[q[min(lane, qn - 1)]];
+        float4 rec = gload4(j.points + q[min(lane, qn - 1)]);
         for (int c0 = 0; c0 < qn; c0 += 64) {
-          const float4 next = j.points[q[min(c0 + 64 + lane, qn - 1)]];
+          const float4 next = gload4(j.points + q[min(c0 + 64 + lane, qn - 1)]);
           if (c0 + lane < qn) splat_queued(j, sm, c, rec, ox, oy, inv_n1d, side);
           rec = next;
         }
@@ -355,23 +355,23 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v)
   const size_t origin = (size_t)tile_y * S * j.atlas_side + (size_t)tile_x * S;
   for (int i = tid; i < n4; i += 256) {
     const int row = (i * 4) / S, col = (i * 4) % S;
-    *reinterpret_cast<uint4*>(j.atlas + origin + (size_t)row * j.atlas_side + col) = t4[i];
+    gstore4(reinterpret_cast<uint4*>(j.atlas + origin + (size_t)row * j.atlas_side + col), t4[i]);
   }
 }
 
 // Spill pass, item b of the frame's n_bins: every spill entry j == b (mod
 // n_bins) in the normal case; bin b's point-major splat after an overflow.
 __device__ __forceinline__ void spill_bin(Candidate* cand, const TileJob& j, int n_bins, int b) {
-  const uint32_t overflow = j.spill_state[1];
+  const uint32_t overflow = gload(j.spill_state + 1);
   if (overflow) {
     splat_bin<256>(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, j.atlas_side, j.z_far,
                    j.atlas, b);
     return;
   }
-  const uint32_t n = min(j.spill_state[0], kSpillCap);
+  const uint32_t n = min(gload(j.spill_state), kSpillCap);
   for (uint32_t k = (uint32_t)b * 256u + threadIdx.x; k < n; k += (uint32_t)n_bins * 256u) {
-    const uint2 e = j.spill[k];
-    __hip_atomic_fetch_min(j.atlas + e.x, e.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t e = gload(reinterpret_cast<const uint64_t*>(j.spill + k));
+    gatomic_min(j.atlas + (uint32_t)e, (uint32_t)(e >> 32));
   }
 }
 

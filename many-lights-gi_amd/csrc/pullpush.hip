@@ -475,7 +475,7 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
   // ---- load the ISM's atlas tile (S*S u32) with 16-byte loads
   for (int i = tid; i < S * S / 4; i += kThreads) {
     const int row = (i * 4) / S, col = (i * 4) % S;
-    const uint4 v = *reinterpret_cast<const uint4*>(atlas + a0 + (size_t)row * atlas_side + col);
+    const uint4 v = gload4(reinterpret_cast<const uint4*>(atlas + a0 + (size_t)row * atlas_side + col));
     *reinterpret_cast<uint4*>(&sm.base[row * S + col]) = v;
   }
   __syncthreads();
@@ -490,8 +490,8 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
   for (int i = tid; i < S * S / 4; i += kThreads) {
     const int row = (i * 4) / S, col = (i * 4) % S;
     const uint4 v = *reinterpret_cast<const uint4*>(&sm.base[row * S + col]);
-    *reinterpret_cast<uint2*>(out + a0 + (size_t)row * atlas_side + col) =
-        make_uint2(v.x | (v.y << 16), v.z | (v.w << 16));
+    gstore(reinterpret_cast<uint64_t*>(out + a0 + (size_t)row * atlas_side + col),
+           (uint64_t)(v.x | (v.y << 16)) | ((uint64_t)(v.z | (v.w << 16)) << 32));
   }
 }
 

@@ -113,3 +113,43 @@ def test_workload_tiled_equals_point_major(ctx, name):
     rec = st.vpl_processor.host_records()
     ref = oracle.ism_splat(inp.points, inp.bin_offsets, rec[:, 0:3], rec[:, 4:7], wl.atlas_side, p.z_far)
     assert np.array_equal(tiled, ref)
+
+
+@pytest.mark.parametrize("n_vpl,atlas", [(1, 64), (4, 16), (9, 64), (16, 64), (64, 128), (64, 16), (256, 64)])
+def test_small_vpl_counts_and_ism_sides(ctx, n_vpl, atlas):
+    """Candidate lists that wrap ((b + i) mod N repeats VPLs when N < 16),
+    ISM sides 2 (point-major only), 4, 8, 16, 64; random VPLs and points."""
+    print(f"[splat] N={n_vpl} atlas={atlas}", flush=True)
+    rng = np.random.default_rng(n_vpl * 1000 + atlas)
+    rec = np.zeros((n_vpl, mlgi.VPL_RECORD_FLOATS), np.float32)
+    rec[:, 0:3] = rng.uniform(-2, 2, (n_vpl, 3))
+    nrm = rng.normal(size=(n_vpl, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    rec[:, 4:7] = nrm
+    for v in range(n_vpl):
+        s, u = oracle.look_at_rh(rec[v, 4:7], np.float32)
+        rec[v, 8:11], rec[v, 12:15] = s, u
+    n_pts = 3000
+    pts = np.zeros((n_pts, 4), np.float32)
+    pts[:, :3] = rng.uniform(-3, 3, (n_pts, 3))
+    for k in range(n_pts):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        pts[k, 3] = unorm4x8(*(n * 0.5 + 0.5), rng.uniform())
+    bins = np.sort(rng.integers(0, n_vpl, n_pts))
+    off = np.zeros(n_vpl + 1, np.uint32)
+    np.add.at(off, bins + 1, 1)
+    off = np.cumsum(off).astype(np.uint32)
+    ref = oracle.ism_splat(pts, off, rec[:, 0:3], rec[:, 4:7], atlas, Z_FAR)
+    outs = []
+    for blk in (0, 256):
+        vp = mlgi.VPLProcessor(ctx, n_vpl)
+        vp.records.copy_from(rec)
+        ism = mlgi.ImperfectShadowmap(ctx, atlas, n_vpl)
+        p = mlgi.default_params(atlas_side=atlas, z_far=Z_FAR, splat_block=blk)
+        ism.render(mlgi.points_desc(ctx.upload(pts), ctx.upload(off)), vp, p)
+        ctx.sync()
+        outs.append(ism.softrender.numpy())
+    assert (ref != 0xFFFFFFFF).any()
+    assert np.array_equal(outs[1], ref)
+    assert np.array_equal(outs[0], ref)
