@@ -85,6 +85,14 @@ int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, int32_t n_bi
                              int32_t ism_count, uint64_t seed, int32_t threads, float* records,
                              uint32_t* bin_offsets);
 
+/* Deferred-shading inputs of a view (model.frag:65-132): albedo (w*h*3, 0
+ * for the background) and/or VSM moments (w*h*2: distance d of the hit from
+ * the camera's eye and d*d; the reference's screen-space derivative term is
+ * left out; background (0, 0)). Rendered from the light, the moments are the
+ * light's variance shadow map. Either output may be NULL. */
+int mlgi_scene_render_surface(const mlgi_scene* s, const mlgi_camera* cam, int32_t width, int32_t height,
+                              int32_t threads, float* albedo, float* moments);
+
 /* ---------------------------------------------------- device producer
  * The scene's BVH on the device, for inputs that change per request or per
  * frame (light poses, views). The kernels run the host renderer's double
@@ -109,6 +117,9 @@ int mlgi_scene_device_render_gbuffer(mlgi_context* ctx, const mlgi_scene_device*
 int mlgi_scene_device_sample_points(mlgi_context* ctx, const mlgi_scene_device* d, int64_t n_points,
                                     int32_t n_bins, int32_t ism_count, uint64_t seed, float* records,
                                     uint32_t* bin_offsets);
+/* mlgi_scene_render_surface on the device (device buffers, asynchronous). */
+int mlgi_scene_device_render_surface(mlgi_context* ctx, const mlgi_scene_device* d, const mlgi_camera* cam,
+                                     int32_t width, int32_t height, float* albedo, float* moments);
 /* diffuse (w*h*3), face_normal (w*h*3), depth (w*h); inv_bvp16 (host,
  * optional) receives inverse(bias * P * V). */
 int mlgi_scene_device_render_rsm(mlgi_context* ctx, const mlgi_scene_device* d,

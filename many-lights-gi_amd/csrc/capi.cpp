@@ -1170,6 +1170,18 @@ int mlgi_scene_device_sample_points(mlgi_context* c, const mlgi_scene_device* d,
   return rc;
 }
 
+int mlgi_scene_device_render_surface(mlgi_context* c, const mlgi_scene_device* d, const mlgi_camera* cam, int32_t w,
+                                     int32_t h, float* albedo, float* moments) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!d || !cam || (!albedo && !moments)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (w <= 0 || h <= 0 || w > 16384 || h > 16384)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "size %dx%d out of range", w, h);
+  mlgi_scene_impl::RayCam rc;
+  if (!mlgi_scene_ray_cam(cam, &rc)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "bad camera");
+  return hip_check(c, mlgi::launch_raycast(c->stream, d->view(), rc, w, h, 2, albedo, moments, nullptr),
+                   "raycast(surface)");
+}
+
 int mlgi_scene_device_render_gbuffer(mlgi_context* c, const mlgi_scene_device* d, const mlgi_camera* cam,
                                      int32_t w, int32_t h, float* depth, float* normal, float* vp_inv16) {
   if (!c) return MLGI_ERR_INVALID_ARGUMENT;

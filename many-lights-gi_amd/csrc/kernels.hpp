@@ -108,8 +108,9 @@ hipError_t launch_composite(hipStream_t s, const CompositeArgs& a);
 uint64_t valu_probe_fmas(int blocks, int iters);
 hipError_t launch_valu_probe(hipStream_t s, float* out, int blocks, int iters);
 
-// raycast.hip: G-buffer (rsm = 0: a = depth, b = normal) or RSM (rsm = 1:
-// a = diffuse, b = normal, c = depth) of a scene uploaded to the device
+// raycast.hip: G-buffer (rsm = 0: a = depth, b = normal), RSM (rsm = 1:
+// a = diffuse, b = normal, c = depth) or surface (rsm = 2: a = albedo,
+// b = VSM moments; either may be null) of a scene uploaded to the device
 }  // namespace mlgi
 namespace mlgi_scene_impl {
 struct SceneView;

@@ -18,8 +18,10 @@ __global__ __launch_bounds__(256) void raycast_kernel(SceneView s, RayCam rc, in
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= w || y >= h) return;
-  if (rsm)
+  if (rsm == 1)
     shade_rsm(s, rc, x, y, w, h, a, b, c);  // diffuse, normal, depth
+  else if (rsm == 2)
+    shade_surface(s, rc, x, y, w, h, a, b);  // albedo, VSM moments
   else
     shade_gbuffer(s, rc, x, y, w, h, a, b);  // depth, normal
 }

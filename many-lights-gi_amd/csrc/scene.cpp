@@ -354,6 +354,17 @@ int mlgi_scene_render_rsm(const mlgi_scene* s, const mlgi_camera* light, int32_t
   return MLGI_OK;
 }
 
+int mlgi_scene_render_surface(const mlgi_scene* s, const mlgi_camera* cam, int32_t w, int32_t h, int32_t threads,
+                              float* albedo, float* moments) {
+  if (!s || !cam || w <= 0 || h <= 0 || (!albedo && !moments)) return MLGI_ERR_INVALID_ARGUMENT;
+  const RayCam rc = ray_cam(cam);
+  const SceneView sv = view_of(s);
+  parallel_rows(h, nthreads(threads), [&](int y) {
+    for (int x = 0; x < w; ++x) shade_surface(sv, rc, x, y, w, h, albedo, moments);
+  });
+  return MLGI_OK;
+}
+
 }  // extern "C"
 
 double mlgi_scene_area_cdf(const mlgi_scene* s, double* cdf) {

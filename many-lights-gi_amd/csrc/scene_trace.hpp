@@ -204,6 +204,33 @@ MLGI_HD inline void shade_rsm(const SceneView& s, const RayCam& rc, int x, int y
   for (int k = 0; k < 3; ++k) diffuse[p * 3 + k] = in_cone ? t.albedo[k] : 0.0f;
 }
 
+// Deferred-shading inputs of texel (x, y) (model.frag:65-132 writes them in
+// the reference): the surface albedo (0 for the background) and the VSM
+// moments (d, d^2) of the hit's distance d from the camera's eye, which is
+// the light's position when rendered from the light (model.frag:130 adds a
+// screen-space derivative term to d^2 that a ray caster has no analogue
+// of; it is left out). Background: moments (0, 0). Either output may be
+// null.
+MLGI_HD inline void shade_surface(const SceneView& s, const RayCam& rc, int x, int y, int w, int h, float* albedo,
+                                  float* moments) {
+  D3 o, d;
+  pixel_ray(rc, x, y, w, h, &o, &d);
+  Hit hit = trace(s, o, d, 1.0);
+  const size_t p = (size_t)y * w + x;
+  if (hit.tri < 0) {
+    if (albedo) albedo[p * 3] = albedo[p * 3 + 1] = albedo[p * 3 + 2] = 0.0f;
+    if (moments) moments[p * 2] = moments[p * 2 + 1] = 0.0f;
+    return;
+  }
+  if (albedo)
+    for (int k = 0; k < 3; ++k) albedo[p * 3 + k] = s.tris[hit.tri].albedo[k];
+  if (moments) {
+    const double dist = len((o + d * hit.t) - rc.eye);
+    moments[p * 2] = (float)dist;
+    moments[p * 2 + 1] = (float)(dist * dist);
+  }
+}
+
 // ------------------------------------------------------------ point sampler
 // Area-proportional surface points (the reference's tessellation + geometry
 // pass output, ism.tesc:11-23, ism.geom:49-69), shared by the host sampler

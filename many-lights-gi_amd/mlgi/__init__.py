@@ -280,6 +280,16 @@ class Scene:
             raise MlgiError(rc, "mlgi_scene_render_rsm")
         return diffuse, normal, depth, inv
 
+    def render_surface(self, cam: Camera, width: int, height: int, threads: int = 0):
+        """(albedo (h, w, 3), VSM moments (h, w, 2)) of the view (mlgi_scene_render_surface)."""
+        albedo = np.empty((height, width, 3), np.float32)
+        moments = np.empty((height, width, 2), np.float32)
+        rc = lib.mlgi_scene_render_surface(self.h, C.byref(cam.to_c()), width, height, threads, albedo.ctypes.data,
+                                           moments.ctypes.data)
+        if rc:
+            raise MlgiError(rc, "mlgi_scene_render_surface")
+        return albedo, moments
+
     def upload(self, ctx: "Context") -> "DeviceScene":
         """The scene's BVH on the device (mlgi_scene_upload)."""
         return DeviceScene(ctx, self)
@@ -320,6 +330,15 @@ class DeviceScene:
                                                            C.c_void_p(rec.ptr), C.c_void_p(off.ptr)),
                        "mlgi_scene_device_sample_points")
         return rec, off
+
+    def render_surface(self, cam: Camera, albedo: "DeviceArray" = None, moments: "DeviceArray" = None):
+        """Albedo (h, w, 3) and / or VSM moments (h, w, 2) into device arrays."""
+        ref = albedo if albedo is not None else moments
+        height, width = ref.shape[:2]
+        self.ctx.check(lib.mlgi_scene_device_render_surface(
+            self.ctx.h, self.h, C.byref(cam.to_c()), width, height,
+            C.c_void_p(albedo.ptr) if albedo is not None else None,
+            C.c_void_p(moments.ptr) if moments is not None else None), "mlgi_scene_device_render_surface")
 
     def render_rsm(self, light: Camera, diffuse: "DeviceArray", normal: "DeviceArray", depth: "DeviceArray"):
         """Into device arrays; returns inverse(bias*P*V) (host)."""

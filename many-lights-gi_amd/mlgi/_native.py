@@ -228,6 +228,8 @@ SIGNATURES = {
     "mlgi_camera_matrices": (C.c_int, [P(Camera), vp, vp, vp]),
     "mlgi_scene_render_gbuffer": (C.c_int, [vp, P(Camera), i32, i32, i32, vp, vp]),
     "mlgi_scene_render_rsm": (C.c_int, [vp, P(Camera), i32, i32, i32, vp, vp, vp, vp]),
+    "mlgi_scene_render_surface": (C.c_int, [vp, P(Camera), i32, i32, i32, vp, vp]),
+    "mlgi_scene_device_render_surface": (C.c_int, [vp, vp, P(Camera), i32, i32, vp, vp]),
     "mlgi_scene_upload": (C.c_int, [vp, vp, P(vp)]),
     "mlgi_scene_device_destroy": (None, [vp]),
     "mlgi_scene_device_render_gbuffer": (C.c_int, [vp, vp, P(Camera), i32, i32, vp, vp, vp]),
