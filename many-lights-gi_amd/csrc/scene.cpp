@@ -94,7 +94,7 @@ void build_bvh(mlgi_scene* s) {
                        return ca < cb || (ca == cb && a < b);
                      });
     nd.left = (int)s->nodes.size();
-    nd.count = 0;
+    nd.count = -1 - axis;  // inner node: split axis
     s->nodes[it.node] = nd;
     s->nodes.push_back(Node{});
     s->nodes.push_back(Node{});

@@ -37,7 +37,9 @@ struct Tri {
 
 struct Node {
   double lo[3], hi[3];
-  int left, count;  // leaf: count > 0, left = first index; inner: left child, right = left+1
+  // leaf: count > 0, left = first index; inner: left child (the smaller
+  // centroids along the split axis), right = left + 1, count = -1 - axis
+  int left, count;
 };
 
 struct Hit {
@@ -90,32 +92,56 @@ MLGI_HD inline bool tri_hit(const Tri& t, D3 o, D3 d, double tmax, double* th) {
   return true;
 }
 
-constexpr int kTraceStack = 64;  // the median-split BVH of 2^30 triangles is < 30 deep
+// Ordered traversal keeps at most one pending node per tree level; the
+// median-split BVH of 4-triangle leaves is ceil(log2(n / 4)) deep (21 for
+// 8M triangles), so 32 entries are enough, and small enough for the device
+// compiler to keep the stack in registers instead of scratch memory.
+constexpr int kTraceStack = 32;
 
-// Nearest hit (smallest t, then smallest triangle index) with t < tmax.
+// Nearest hit with t < tmax. At an inner node both children's boxes are
+// tested and the child on the ray's side of the split axis is visited first
+// (the other pushed): the sign of the direction along the axis, not the box
+// distances, decides, so the 64 coherent rays of a wave take the same order
+// and do not diverge. A popped node's box is tested again against the
+// current nearest hit. Among hits at exactly equal t the first one reached
+// wins.
 MLGI_HD inline Hit trace(const SceneView& s, D3 o, D3 d, double tmax) {
   Hit h{tmax, -1};
   D3 inv = {1.0 / (d.x != 0 ? d.x : 1e-300), 1.0 / (d.y != 0 ? d.y : 1e-300), 1.0 / (d.z != 0 ? d.z : 1e-300)};
   int stack[kTraceStack];
   int sp = 0;
-  stack[sp++] = 0;
-  while (sp) {
-    const Node& nd = s.nodes[stack[--sp]];
-    double tn;
-    if (!box_hit(nd, o, inv, h.t, &tn)) continue;
+  double tn;
+  int node = box_hit(s.nodes[0], o, inv, h.t, &tn) ? 0 : -1;
+  while (node >= 0) {
+    const Node& nd = s.nodes[node];
+    int next = -1;
     if (nd.count > 0) {
       for (int i = nd.left; i < nd.left + nd.count; ++i) {
         double th;
-        int ti = s.order[i];
-        if (tri_hit(s.tris[ti], o, d, h.t, &th) && (th < h.t || (th == h.t && ti < h.tri))) {
+        const int ti = s.order[i];
+        if (tri_hit(s.tris[ti], o, d, h.t, &th)) {
           h.t = th;
           h.tri = ti;
         }
       }
-    } else if (sp + 2 <= kTraceStack) {
-      stack[sp++] = nd.left;
-      stack[sp++] = nd.left + 1;
+    } else {
+      const bool left_first = comp(d, -1 - nd.count) >= 0.0;
+      const int a = left_first ? nd.left : nd.left + 1, b = left_first ? nd.left + 1 : nd.left;
+      double ta, tb;
+      const bool ha = box_hit(s.nodes[a], o, inv, h.t, &ta);
+      const bool hb = box_hit(s.nodes[b], o, inv, h.t, &tb);
+      if (ha) {
+        next = a;
+        if (hb && sp < kTraceStack) stack[sp++] = b;
+      } else if (hb) {
+        next = b;
+      }
     }
+    while (next < 0 && sp > 0) {  // pop until a node still in front of the nearest hit
+      const int c = stack[--sp];
+      if (box_hit(s.nodes[c], o, inv, h.t, &tn)) next = c;
+    }
+    node = next;
   }
   return h;
 }
