@@ -42,20 +42,64 @@ namespace {
 
 const D3 kDefaultEye = {5.0, 5.0, 9.0};
 
-void build_bvh(mlgi_scene* s) {
+// Depth of the subtree at node i (the traversal stack must hold it).
+int bvh_depth(const mlgi_scene* s, int i) {
+  const Node& nd = s->nodes[i];
+  if (nd.count > 0) return 1;
+  return 1 + std::max(bvh_depth(s, nd.left), bvh_depth(s, nd.left + 1));
+}
+
+void build_bvh_with(mlgi_scene* s, int sah_depth) {
   const int n = (int)s->tris.size();
   s->order.resize(n);
   for (int i = 0; i < n; ++i) s->order[i] = i;
   std::vector<D3> cen(n);
   for (int i = 0; i < n; ++i) cen[i] = (s->tris[i].v0 + s->tris[i].v1 + s->tris[i].v2) * (1.0 / 3.0);
   s->nodes.clear();
-  s->nodes.reserve(2 * n);
+  s->nodes.reserve(2 * n + 2);
   struct Item {
-    int node, begin, end;
+    int node, begin, end, depth;
   };
   std::vector<Item> stack;
   s->nodes.push_back(Node{});
-  stack.push_back({0, 0, n});
+  // Root split by size: triangles spanning more than a quarter of the scene
+  // (the room's walls) go to their own subtree, so the boxes of the median
+  // splits below stay tight around the small geometry instead of every level
+  // containing a wall (which each ray would then enter).
+  double slo[3] = {1e300, 1e300, 1e300}, shi[3] = {-1e300, -1e300, -1e300};
+  for (const Tri& t : s->tris)
+    for (const D3* v : {&t.v0, &t.v1, &t.v2})
+      for (int k = 0; k < 3; ++k) {
+        slo[k] = std::min(slo[k], comp(*v, k));
+        shi[k] = std::max(shi[k], comp(*v, k));
+      }
+  const double span = std::max(shi[0] - slo[0], std::max(shi[1] - slo[1], shi[2] - slo[2]));
+  auto big = [&](int i) {
+    const Tri& t = s->tris[i];
+    for (int k = 0; k < 3; ++k) {
+      const double lo = std::min(comp(t.v0, k), std::min(comp(t.v1, k), comp(t.v2, k)));
+      const double hi = std::max(comp(t.v0, k), std::max(comp(t.v1, k), comp(t.v2, k)));
+      if (hi - lo > 0.25 * span) return true;
+    }
+    return false;
+  };
+  const int n_big = (int)(std::stable_partition(s->order.begin(), s->order.end(), big) - s->order.begin());
+  if (n_big > 0 && n_big < n && n > 4) {
+    Node root{};
+    for (int k = 0; k < 3; ++k) {
+      root.lo[k] = slo[k];
+      root.hi[k] = shi[k];
+    }
+    root.left = 1;
+    root.count = -1;  // inner node; children: walls (1), the rest (2); axis 0 for the visiting order
+    s->nodes[0] = root;
+    s->nodes.push_back(Node{});
+    s->nodes.push_back(Node{});
+    stack.push_back({1, 0, n_big, 1});
+    stack.push_back({2, n_big, n, 1});
+  } else {
+    stack.push_back({0, 0, n, 0});
+  }
   while (!stack.empty()) {
     Item it = stack.back();
     stack.pop_back();
@@ -84,23 +128,109 @@ void build_bvh(mlgi_scene* s) {
       s->nodes[it.node] = nd;
       continue;
     }
-    int axis = 0;
-    for (int k = 1; k < 3; ++k)
-      if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
-    const int mid = (it.begin + it.end) / 2;
-    std::nth_element(s->order.begin() + it.begin, s->order.begin() + mid, s->order.begin() + it.end,
-                     [&](int a, int b) {
-                       double ca = comp(cen[a], axis), cb = comp(cen[b], axis);
-                       return ca < cb || (ca == cb && a < b);
-                     });
+    // Binned surface-area heuristic on the top levels: 16 centroid bins per
+    // axis, the split plane of least (area x count) over both sides. Median
+    // splits below (and where all centroids coincide or a side would be
+    // empty), so the depth stays within sah_depth + log2(n / 4) + 1.
+    constexpr int kBins = 16;
+    int axis = -1, split = -1;
+    double best = 1e300;
+    for (int k = 0; k < 3 && it.depth < sah_depth; ++k) {
+      const double ext = chi[k] - clo[k];
+      if (!(ext > 0.0)) continue;
+      int bc[kBins] = {};
+      double blo[kBins][3], bhi[kBins][3];
+      for (int b = 0; b < kBins; ++b)
+        for (int q = 0; q < 3; ++q) {
+          blo[b][q] = 1e300;
+          bhi[b][q] = -1e300;
+        }
+      for (int i = it.begin; i < it.end; ++i) {
+        const int ti = s->order[i];
+        const int b = std::min(kBins - 1, (int)((comp(cen[ti], k) - clo[k]) / ext * kBins));
+        bc[b]++;
+        const Tri& t = s->tris[ti];
+        for (const D3* v : {&t.v0, &t.v1, &t.v2})
+          for (int q = 0; q < 3; ++q) {
+            blo[b][q] = std::min(blo[b][q], comp(*v, q));
+            bhi[b][q] = std::max(bhi[b][q], comp(*v, q));
+          }
+      }
+      auto area = [](const double* lo, const double* hi) {
+        if (lo[0] > hi[0]) return 0.0;
+        const double x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+        return 2.0 * (x * y + y * z + z * x);
+      };
+      double rlo[kBins][3], rhi[kBins][3];
+      int rc[kBins];
+      double alo[3] = {1e300, 1e300, 1e300}, ahi[3] = {-1e300, -1e300, -1e300};
+      int acc = 0;
+      for (int b = kBins - 1; b >= 0; --b) {  // suffix boxes and counts
+        for (int q = 0; q < 3; ++q) {
+          alo[q] = std::min(alo[q], blo[b][q]);
+          ahi[q] = std::max(ahi[q], bhi[b][q]);
+          rlo[b][q] = alo[q];
+          rhi[b][q] = ahi[q];
+        }
+        acc += bc[b];
+        rc[b] = acc;
+      }
+      double llo[3] = {1e300, 1e300, 1e300}, lhi[3] = {-1e300, -1e300, -1e300};
+      int lc = 0;
+      for (int sp = 1; sp < kBins; ++sp) {  // left = bins [0, sp)
+        for (int q = 0; q < 3; ++q) {
+          llo[q] = std::min(llo[q], blo[sp - 1][q]);
+          lhi[q] = std::max(lhi[q], bhi[sp - 1][q]);
+        }
+        lc += bc[sp - 1];
+        if (lc == 0 || rc[sp] == 0) continue;
+        const double cost = area(llo, lhi) * lc + area(rlo[sp], rhi[sp]) * rc[sp];
+        if (cost < best) {
+          best = cost;
+          axis = k;
+          split = sp;
+        }
+      }
+    }
+    int mid = -1;
+    if (axis >= 0) {
+      const double ext = chi[axis] - clo[axis];
+      const int ax = axis, spl = split;
+      const double c0 = clo[axis];
+      mid = (int)(std::stable_partition(s->order.begin() + it.begin, s->order.begin() + it.end,
+                                        [&](int ti) {
+                                          return std::min(kBins - 1, (int)((comp(cen[ti], ax) - c0) / ext * kBins)) <
+                                                 spl;
+                                        }) -
+                  s->order.begin());
+    }
+    if (mid <= it.begin || mid >= it.end) {
+      axis = 0;
+      for (int k = 1; k < 3; ++k)
+        if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+      mid = (it.begin + it.end) / 2;
+      std::nth_element(s->order.begin() + it.begin, s->order.begin() + mid, s->order.begin() + it.end,
+                       [&](int a, int b) {
+                         double ca = comp(cen[a], axis), cb = comp(cen[b], axis);
+                         return ca < cb || (ca == cb && a < b);
+                       });
+    }
     nd.left = (int)s->nodes.size();
     nd.count = -1 - axis;  // inner node: split axis
     s->nodes[it.node] = nd;
     s->nodes.push_back(Node{});
     s->nodes.push_back(Node{});
-    stack.push_back({nd.left, it.begin, mid});
-    stack.push_back({nd.left + 1, mid, it.end});
+    stack.push_back({nd.left, it.begin, mid, it.depth + 1});
+    stack.push_back({nd.left + 1, mid, it.end, it.depth + 1});
   }
+}
+
+// SAH on the top 8 levels; if the tree still outgrows the traversal stack
+// (only for far larger scenes than generated here), median splits only,
+// which are ceil(log2(n / 4)) + 1 <= 31 deep.
+void build_bvh(mlgi_scene* s) {
+  build_bvh_with(s, 8);
+  if (bvh_depth(s, 0) > kTraceStack) build_bvh_with(s, 0);
 }
 
 // 4x4 column-major helpers in double

@@ -92,10 +92,10 @@ MLGI_HD inline bool tri_hit(const Tri& t, D3 o, D3 d, double tmax, double* th) {
   return true;
 }
 
-// Ordered traversal keeps at most one pending node per tree level; the
-// median-split BVH of 4-triangle leaves is ceil(log2(n / 4)) deep (21 for
-// 8M triangles), so 32 entries are enough, and small enough for the device
-// compiler to keep the stack in registers instead of scratch memory.
+// The traversal keeps at most one pending node per tree level; the build
+// keeps the BVH at most 32 levels deep (it falls back to median splits
+// otherwise), and 32 entries are few enough for the device compiler to keep
+// the stack in registers.
 constexpr int kTraceStack = 32;
 
 // Nearest hit with t < tmax. At an inner node both children's boxes are
