@@ -1,9 +1,14 @@
 // Device G-buffer / RSM producer: the scene's BVH ray cast on the GPU, one
-// lane per texel, with the host renderer's double-precision code
+// lane per texel, one 8x8 tile per single-wave workgroup, with the host renderer's double-precision code
 // (scene_trace.hpp), so both write the same texels. Replaces the
 // reference's rasterisation of the G-buffer and RSM (RasterizationStage.cpp
 // and model.frag) for inputs that change per request or per frame.
 // Compiled with -ffp-contract=off like the host renderer.
+//
+// A 128^2 RSM is only 256 such tiles: single-wave workgroups spread them over
+// every CU, where 16x16 workgroups would occupy a quarter of the chip and
+// have four waves share each CU's vector-memory path (a traversal step loads
+// a 64-byte double-precision node per lane).
 #include "kernels.hpp"
 #include "scene_trace.hpp"
 
@@ -12,11 +17,11 @@ namespace {
 
 using namespace mlgi_scene_impl;
 
-__global__ __launch_bounds__(256) void raycast_kernel(SceneView s, RayCam rc, int w, int h, int rsm,
+__global__ __launch_bounds__(64) void raycast_kernel(SceneView s, RayCam rc, int w, int h, int rsm,
                                                       float* __restrict__ a, float* __restrict__ b,
                                                       float* __restrict__ c) {
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int x = blockIdx.x * 8 + (threadIdx.x & 7);
+  const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
   if (x >= w || y >= h) return;
   if (rsm == 1)
     shade_rsm(s, rc, x, y, w, h, a, b, c);  // diffuse, normal, depth
@@ -30,8 +35,8 @@ __global__ __launch_bounds__(256) void raycast_kernel(SceneView s, RayCam rc, in
 
 hipError_t launch_raycast(hipStream_t st, const SceneView& s, const RayCam& rc, int w, int h, int rsm,
                           float* a, float* b, float* c) {
-  const dim3 grid((w + 15) / 16, (h + 15) / 16);
-  hipLaunchKernelGGL(raycast_kernel, grid, dim3(256), 0, st, s, rc, w, h, rsm, a, b, c);
+  const dim3 grid((w + 7) / 8, (h + 7) / 8);
+  hipLaunchKernelGGL(raycast_kernel, grid, dim3(64), 0, st, s, rc, w, h, rsm, a, b, c);
   return hipGetLastError();
 }
 
