@@ -133,6 +133,16 @@ __host__ __device__ inline double sqrt_gt_bound(float R) {
   return m * m;
 }
 
+// The same test against a float: for a float x and a real b, x > b exactly
+// when x > RD(b), the largest float <= b (a float above RD(b) is at least the
+// next float, which lies above b unless b is itself a float, RD(b) = b).
+__host__ __device__ inline float sqrt_gt_bound_f(float R) {
+  const double b = sqrt_gt_bound(R);
+  float f = (float)b;
+  if ((double)f > b) f = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) - 1u);  // f > 0 here
+  return f;
+}
+
 // x / b correctly rounded from y = RN(1 / b): q = RN(x y) lies within an ulp
 // of x / b, the remainder r = x - q b is exact under an FMA, and RN(q + r y)
 // is then RN(x / b) (Markstein's theorem). Holds for every normal x and the
@@ -275,15 +285,19 @@ __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, in
   return pull_combine(ds, mds, rs, dx, dy, valid);
 }
 
-// One push invocation (push.comp:44-165): quad (gx, gy) of level l writes
-// the fine pixels 2g - 1 + {0,1}^2 that lie in the ISM from the four coarse
-// neighbours g - 1 + {0,1}^2 of level l + 1. Level 0 reads the originals from
-// the atlas tile and leaves the unorm16 result in the same slot; other levels
-// update their texels in place. Zero-weight terms add +0 to every sum (all
-// summands are finite), so the sums run over all four neighbours.
+// One push invocation (push.comp:44-165) of a level l >= 1: quad (gx, gy)
+// updates, in place, the fine pixels 2g - 1 + {0,1}^2 of level l that lie in
+// the ISM from the four coarse neighbours g - 1 + {0,1}^2 of level l + 1.
+// The reference's weights become flags: w_i != 0 is `valid` (usable and
+// within the radius), the weight that survives the max-depth test is `keep`.
+// The reference sums all four w_i-weighted terms; a dropped term there is
+// v * 0 = +-0, and a fill always has a kept term, which is nonzero or +0
+// (a displacement c + d with c != 0 that cancels rounds to +0), so summing
+// the kept terms over +0 gives the same bits.
 template <int S, int l>
 __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, Texel* same,
-                                                   uint32_t* base, const float* rcp) {
+                                                   const float* rcp) {
+  static_assert(l >= 1, "level 0 is push_quad_depth");
   constexpr int so = S >> l, sc = so / 2;
   constexpr float scale = 1.0f / (float)(1 << l);
   // offsets {0,0},{0,1},{1,1},{1,0} for both neighbours and fine pixels
@@ -312,79 +326,125 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
   for (int op = 0; op < 4; ++op) {
     const int px = 2 * gx - 1 + ox[op], py = 2 * gy - 1 + oy[op];
     if (px < 0 || py < 0 || px >= so || py >= so) continue;
-    float w[4], ddx[4], ddy[4];
+    float ddx[4], ddy[4];
+    bool valid[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
-      w[i] = usable[i] ? ((wt == 0) ? 9.0f : (wt == 2 ? 1.0f : 3.0f)) : 0.0f;
       // (coarse + 0.5) * 2 - (fine + 0.5) = 2 o_i - o_op - 0.5, exact
-      ddx[i] = (float)(2 * ox[i] - ox[op]) - 0.5f + cdx2[i];
-      ddy[i] = (float)(2 * oy[i] - oy[op]) - 0.5f + cdy2[i];
+      ddx[i] = ((float)(2 * ox[i] - ox[op]) - 0.5f) + cdx2[i];
+      ddy[i] = ((float)(2 * oy[i] - oy[op]) - 0.5f) + cdy2[i];
       const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
-      if ((double)x > bound[i]) w[i] = 0.0f;  // sqrt(x) > radius * 2^-level
+      valid[i] = usable[i] && !((double)x > bound[i]);  // sqrt(x) <= radius * 2^-level
     }
-    float minimum = 9001.0f;
-    float max_depth = 0.0f;
+    float minimum = 9001.0f, max_depth = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {  // fminf(cd, minimum) == cd is cd <= minimum
-      const bool take = w[i] != 0.0f && !(cd[i] > minimum);
+      const bool take = valid[i] && !(cd[i] > minimum);
       minimum = take ? cd[i] : minimum;
       max_depth = take ? cmd[i] : max_depth;
     }
-    float wacc = 0.0f;
+    const int t = py * so + px;
+    Texel res = same[t];
+    bool keep[4], any = false, occluded = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (cd[i] > max_depth) w[i] = 0.0f;
-      wacc = wacc + w[i];
+      keep[i] = valid[i] && !(cd[i] > max_depth);
+      any = any || keep[i];
+      occluded = occluded || (keep[i] && res.depth > cmd[i]);
+    }
+    if (any && (res.depth == 1.0f || occluded)) {
+      float wacc = 0.0f, dacc = 0.0f, macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
+        const float w = (wt == 0) ? 9.0f : (wt == 2 ? 1.0f : 3.0f);
+        wacc = wacc + (keep[i] ? w : 0.0f);
+        dacc = dacc + (keep[i] ? cd[i] * w : 0.0f);
+        macc = macc + (keep[i] ? cmd[i] * w : 0.0f);
+        racc = racc + (keep[i] ? cr[i] * w : 0.0f);
+        xacc = xacc + (keep[i] ? ddx[i] * w : 0.0f);
+        yacc = yacc + (keep[i] ? ddy[i] * w : 0.0f);
+      }
+      const float y = rcp[(int)wacc];
+      res.depth = div_by_recip(dacc, wacc, y);
+      res.max_depth = div_by_recip(macc, wacc, y);
+      res.radius = div_by_recip(racc, wacc, y);
+      res.disp = pack_half2_hd(div_by_recip(xacc, wacc, y), div_by_recip(yacc, wacc, y));
+    }
+    // level 1 is read once more, by the depth-only level-0 push, which needs
+    // its radius only as the bound of the distance test
+    if constexpr (l == 1) res.radius = sqrt_gt_bound_f(res.radius);
+    same[t] = res;
+  }
+}
+
+// Level-0 push (push.comp:44-165 at level 0, depth only): quad (gx, gy) writes
+// the unorm16 depth of the fine pixels 2g - 1 + {0,1}^2 from the coarse
+// neighbours g - 1 + {0,1}^2 of level 1, whose radius slots hold
+// sqrt_gt_bound_f(radius) (level 0 scales the radius by 2^0). The weights of
+// push_quad become per-neighbour flags: w_i != 0 is `valid`, the weight that
+// survives the max-depth test is `keep`, and a fill needs an occluding kept
+// neighbour, so `wacc > 0 && occluded` is just `occluded`. Sums over the kept
+// terms in neighbour order are the sums of push_quad (its zero terms add +0).
+template <int S>
+__device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coarse, uint32_t* base,
+                                                const float* rcp) {
+  constexpr int so = S, sc = S / 2;
+  constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
+  bool usable[4];
+  float cd[4], cd3[4], cd9[4], cmd[4], bnd[4], hx[2][4], hy[2][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
+    const bool inside = cx >= 0 && cy >= 0 && cx < sc && cy < sc;
+    Texel c{1.0f, 0.0f, 0.0f, 0u};
+    if (inside) c = coarse[cy * sc + cx];
+    cd[i] = c.depth;
+    cd3[i] = c.depth * 3.0f;
+    cd9[i] = c.depth * 9.0f;
+    cmd[i] = c.max_depth;
+    bnd[i] = c.radius;
+    usable[i] = inside && c.depth != 1.0f;
+    float dxh, dyh;
+    unpack_half2_hd(c.disp, dxh, dyh);
+    const float dx2 = dxh * 2.0f, dy2 = dyh * 2.0f;
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {  // fine offset f along x / y
+      const float ex = ((float)(2 * ox[i] - f) - 0.5f) + dx2;
+      const float ey = ((float)(2 * oy[i] - f) - 0.5f) + dy2;
+      hx[f][i] = ex * ex;
+      hy[f][i] = ey * ey;
+    }
+  }
+#pragma unroll
+  for (int op = 0; op < 4; ++op) {
+    const int px = 2 * gx - 1 + ox[op], py = 2 * gy - 1 + oy[op];
+    if (px < 0 || py < 0 || px >= so || py >= so) continue;
+    bool valid[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) valid[i] = usable[i] && !(hx[ox[op]][i] + hy[oy[op]][i] > bnd[i]);
+    float minimum = 9001.0f, max_depth = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool take = valid[i] && !(cd[i] > minimum);
+      minimum = take ? cd[i] : minimum;
+      max_depth = take ? cmd[i] : max_depth;
     }
     const int t = py * so + px;
-    Texel orig;
-    bool invalid;
-    if constexpr (l == 0) {
-      // (s >> 8) < 2^24: a level-0 original is never the invalid depth 1
-      orig = Texel{(float)(base[t] >> 8) * (1.0f / 16777216.0f), 0.0f, 0.0f, 0u};
-      invalid = false;
-    } else {
-      orig = same[t];
-      invalid = orig.depth == 1.0f;
-    }
+    const float orig = (float)(base[t] >> 8) * (1.0f / 16777216.0f);
     bool occluded = false;
+    float wacc = 0.0f, dacc = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) occluded = occluded || (w[i] > 0.0f && orig.depth > cmd[i]);
-    Texel res = orig;
-    const bool fill = wacc > 0.0f && (invalid || occluded);
-    if constexpr (l == 0) {
-      // depth only: computed for every lane and selected (rcp[0] = 0 keeps
-      // the unused wacc = 0 case finite)
-      const float y = rcp[(int)wacc];
-      float dacc = cd[0] * w[0];
-#pragma unroll
-      for (int i = 1; i < 4; ++i) dacc = dacc + cd[i] * w[i];
-      res.depth = fill ? div_by_recip(dacc, wacc, y) : orig.depth;
-    } else if (fill) {
-      const float y = rcp[(int)wacc];
-      float dacc = cd[0] * w[0];
-#pragma unroll
-      for (int i = 1; i < 4; ++i) dacc = dacc + cd[i] * w[i];
-      res.depth = div_by_recip(dacc, wacc, y);
-      {
-        float macc = cmd[0] * w[0], racc = cr[0] * w[0], xacc = ddx[0] * w[0], yacc = ddy[0] * w[0];
-#pragma unroll
-        for (int i = 1; i < 4; ++i) {
-          macc = macc + cmd[i] * w[i];
-          racc = racc + cr[i] * w[i];
-          xacc = xacc + ddx[i] * w[i];
-          yacc = yacc + ddy[i] * w[i];
-        }
-        res.max_depth = div_by_recip(macc, wacc, y);
-        res.radius = div_by_recip(racc, wacc, y);
-        res.disp = pack_half2_hd(div_by_recip(xacc, wacc, y), div_by_recip(yacc, wacc, y));
-      }
+    for (int i = 0; i < 4; ++i) {
+      const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
+      const bool keep = valid[i] && !(cd[i] > max_depth);
+      occluded = occluded || (keep && orig > cmd[i]);
+      wacc = wacc + (keep ? (wt == 0 ? 9.0f : (wt == 2 ? 1.0f : 3.0f)) : 0.0f);
+      dacc = dacc + (keep ? (wt == 0 ? cd9[i] : (wt == 2 ? cd[i] : cd3[i])) : 0.0f);
     }
-    if constexpr (l == 0)
-      base[t] = to_r16(res.depth);  // the tile slot now holds the result
-    else
-      same[t] = res;
+    // rcp[0] = 0 keeps the unused wacc = 0 case finite
+    const float depth = occluded ? div_by_recip(dacc, wacc, rcp[(int)wacc]) : orig;
+    base[t] = to_r16(depth);  // the tile slot now holds the result
   }
 }
 
@@ -438,8 +498,12 @@ __device__ __forceinline__ void push_levels(const Tables& tb, uint32_t* base) {
     constexpr int so = S >> l, nq1 = so / 2 + 1;
     const Texel* coarse = tb.lv + level_offset<S>(l + 1);
     Texel* same = tb.lv + level_offset<S>(l);
-    for (int q = threadIdx.x; q < nq1 * nq1; q += kThreads)
-      push_quad<S, l>(q % nq1, q / nq1, coarse, same, base, tb.rcp);
+    for (int q = threadIdx.x; q < nq1 * nq1; q += kThreads) {
+      if constexpr (l == 0)
+        push_quad_depth<S>(q % nq1, q / nq1, coarse, base, tb.rcp);
+      else
+        push_quad<S, l>(q % nq1, q / nq1, coarse, same, tb.rcp);
+    }
     __syncthreads();
     push_levels<S, l - 1>(tb, base);
   }
@@ -484,6 +548,11 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
   tb.empty = decode_texel(0xFFFFFFFFu, sm.rtab[255], sm.mtab[255], z_far, S);
 
   pull_levels<S, 1, L>(tb, pc);
+  if constexpr (L == 1) {
+    // no level-1 push stores the bounds the level-0 push reads
+    for (int t = tid; t < (S / 2) * (S / 2); t += kThreads) sm.lv[t].radius = sqrt_gt_bound_f(sm.lv[t].radius);
+    __syncthreads();
+  }
   push_levels<S, L - 1>(tb, sm.base);
 
   // ---- write the unorm16 tile (held in the low halves of the tile slots)

@@ -1,0 +1,54 @@
+"""GPU: pull-push of every ISM side and level count against the oracle's FP32
+twin, bit for bit, on random atlases.
+
+The workloads cover sides 32 and 64 at three and four levels; this sweep adds
+sides 8 and 16 and every level count from 1 (the level-0 push reads the pull
+directly) to log2(side). Texels are empty (0xFFFFFFFF) with a density that
+differs per ISM (from nearly full to entirely empty), otherwise a random
+24-bit depth and radius byte, so that pulls, occlusion tests and fills all
+take both branches.
+"""
+import numpy as np
+import pytest
+
+import mlgi
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+N_VPL = 16   # 4 x 4 ISMs
+Z_FAR = 50.0
+
+
+def random_atlas(side, ism_side, seed):
+    rng = np.random.default_rng(seed)
+    n1d = side // ism_side
+    empty_p = np.repeat(np.repeat(rng.choice([0.1, 0.5, 0.8, 0.97, 1.0], (n1d, n1d)), ism_side, 0), ism_side, 1)
+    depth = rng.integers(1 << 18, (1 << 24) - 1, (side, side), dtype=np.uint32)
+    # a smooth depth surface under the noise, so that neighbours often agree
+    yy, xx = np.mgrid[0:side, 0:side]
+    base = ((np.sin(xx * 0.3) + np.cos(yy * 0.2) + 2.0) * 0.25 * (1 << 23)).astype(np.uint32)
+    depth = np.where(rng.random((side, side)) < 0.5, base + (depth & 0xFFFF), depth)
+    radius = rng.integers(0, 256, (side, side), dtype=np.uint32)
+    texel = (depth << 8) | radius
+    return np.where(rng.random((side, side)) < empty_p, np.uint32(0xFFFFFFFF), texel).astype(np.uint32)
+
+
+CASES = [(s, l) for s in (8, 16, 32, 64) for l in range(1, int(np.log2(s)) + 1)]
+
+
+@pytest.mark.parametrize("ism_side,levels", CASES)
+def test_pullpush_levels_bit_exact(ctx, ism_side, levels):
+    side = 4 * ism_side
+    print(f"[pullpush] S={ism_side} L={levels} atlas={side}", flush=True)
+    atlas = random_atlas(side, ism_side, seed=ism_side * 10 + levels)
+    ism = mlgi.ImperfectShadowmap(ctx, side, N_VPL)
+    assert ism.ism_side == ism_side
+    ism.softrender.copy_from(atlas)
+    p = mlgi.default_params(atlas_side=side, z_far=Z_FAR, pull_push_levels=levels)
+    ism.pullpush(p)
+    ctx.sync()
+    got = ism.pushpull_result.numpy()
+    ref = oracle.pullpush(atlas, ism_side, levels, Z_FAR)
+    assert len(np.unique(ref)) > 16
+    assert np.array_equal(got, ref), int((got != ref).sum())
