@@ -1083,14 +1083,15 @@ extern "C" int mlgi_autotune(mlgi_context* c, mlgi_frame_desc* frame, int32_t mi
 
 struct mlgi_scene_device {
   mlgi_context* ctx = nullptr;
-  mlgi_scene_impl::Node* nodes = nullptr;
+  mlgi_scene_impl::Node4* nodes4 = nullptr;
+  mlgi_scene_impl::TriGeo* geo = nullptr;
   mlgi_scene_impl::Tri* tris = nullptr;
   int* order = nullptr;
   double* cdf = nullptr;  // running triangle areas (point sampling)
   double total_area = 0.0;
-  int n_nodes = 0, n_tris = 0;
+  int n_nodes4 = 0, n_tris = 0;
   mlgi_scene_impl::SceneView view() const {
-    return mlgi_scene_impl::SceneView{nodes, tris, order, n_nodes, n_tris};
+    return mlgi_scene_impl::SceneView{nodes4, geo, tris, order, n_nodes4, n_tris};
   }
 };
 
@@ -1099,7 +1100,8 @@ extern "C" {
 void mlgi_scene_device_destroy(mlgi_scene_device* d) {
   if (!d) return;
   if (d->ctx) (void)hipStreamSynchronize(d->ctx->stream);
-  (void)hipFree(d->nodes);
+  (void)hipFree(d->nodes4);
+  (void)hipFree(d->geo);
   (void)hipFree(d->tris);
   (void)hipFree(d->order);
   (void)hipFree(d->cdf);
@@ -1111,22 +1113,20 @@ int mlgi_scene_upload(mlgi_context* c, const mlgi_scene* s, mlgi_scene_device** 
   if (!s || !out) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "scene / out is NULL");
   *out = nullptr;
   const mlgi_scene_impl::SceneView v = mlgi_scene_view(s);
-  if (v.n_nodes <= 0 || v.n_tris <= 0) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "empty scene");
+  if (v.n_nodes4 <= 0 || v.n_tris <= 0) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "empty scene");
   auto* d = new mlgi_scene_device();
   d->ctx = c;
-  d->n_nodes = v.n_nodes;
+  d->n_nodes4 = v.n_nodes4;
   d->n_tris = v.n_tris;
-  int rc = hip_check(c, hipMalloc(&d->nodes, sizeof(mlgi_scene_impl::Node) * v.n_nodes), "hipMalloc(bvh)");
-  if (!rc) rc = hip_check(c, hipMalloc(&d->tris, sizeof(mlgi_scene_impl::Tri) * v.n_tris), "hipMalloc(tris)");
-  if (!rc) rc = hip_check(c, hipMalloc(&d->order, sizeof(int) * v.n_tris), "hipMalloc(order)");
-  if (!rc)
-    rc = hip_check(c, hipMemcpy(d->nodes, v.nodes, sizeof(mlgi_scene_impl::Node) * v.n_nodes, hipMemcpyHostToDevice),
-                   "hipMemcpy(bvh)");
-  if (!rc)
-    rc = hip_check(c, hipMemcpy(d->tris, v.tris, sizeof(mlgi_scene_impl::Tri) * v.n_tris, hipMemcpyHostToDevice),
-                   "hipMemcpy(tris)");
-  if (!rc)
-    rc = hip_check(c, hipMemcpy(d->order, v.order, sizeof(int) * v.n_tris, hipMemcpyHostToDevice), "hipMemcpy(order)");
+  auto copy = [&](auto** dst, const auto* src, size_t n, const char* what) {
+    int r = hip_check(c, hipMalloc(dst, sizeof(**dst) * n), what);
+    if (!r) r = hip_check(c, hipMemcpy(*dst, src, sizeof(**dst) * n, hipMemcpyHostToDevice), what);
+    return r;
+  };
+  int rc = copy(&d->nodes4, v.nodes4, (size_t)v.n_nodes4, "upload(bvh)");
+  if (!rc) rc = copy(&d->geo, v.geo, (size_t)v.n_tris + mlgi_scene_impl::kGeoPad, "upload(triangle geometry)");
+  if (!rc) rc = copy(&d->tris, v.tris, (size_t)v.n_tris, "upload(triangles)");
+  if (!rc) rc = copy(&d->order, v.order, (size_t)v.n_tris, "upload(order)");
   if (!rc) {
     std::vector<double> cdf(v.n_tris);
     d->total_area = mlgi_scene_area_cdf(s, cdf.data());

@@ -34,7 +34,9 @@ struct Rng {
 struct mlgi_scene {
   std::vector<Tri> tris;
   std::vector<int> order;  // BVH leaf order
-  std::vector<Node> nodes;
+  std::vector<Node> nodes;    // binary BVH (build)
+  std::vector<Node4> nodes4;  // 4-wide BVH (traversal)
+  std::vector<TriGeo> geo;    // triangle geometry in leaf order
   double area = 0.0;
 };
 
@@ -225,12 +227,73 @@ void build_bvh_with(mlgi_scene* s, int sah_depth) {
   }
 }
 
-// SAH on the top 8 levels; if the tree still outgrows the traversal stack
-// (only for far larger scenes than generated here), median splits only,
-// which are ceil(log2(n / 4)) + 1 <= 31 deep.
+// Float bounds that contain the double ones.
+float round_up_f(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = std::nextafter(f, HUGE_VALF);
+  return f;
+}
+float round_down_any(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = std::nextafter(f, -HUGE_VALF);
+  return f;
+}
+
+// 4-wide node of binary node b (an inner node, or a leaf root): its children,
+// each inner child replaced by its own two children, in binary order.
+int collapse(mlgi_scene* s, int b) {
+  const int me = (int)s->nodes4.size();
+  s->nodes4.push_back(Node4{});
+  int slots[4], n = 0;
+  const Node nd = s->nodes[b];
+  if (nd.count > 0) {
+    slots[n++] = b;
+  } else {
+    for (int c : {nd.left, nd.left + 1}) {
+      const Node& cn = s->nodes[c];
+      if (cn.count > 0) {
+        slots[n++] = c;
+      } else {
+        slots[n++] = cn.left;
+        slots[n++] = cn.left + 1;
+      }
+    }
+  }
+  Node4 q{};
+  for (int k = 0; k < 4; ++k) {
+    q.ref[k] = 0;
+    q.count[k] = -1;
+    for (int a = 0; a < 3; ++a) q.lo[a][k] = q.hi[a][k] = 0.0f;
+  }
+  for (int k = 0; k < n; ++k) {
+    const Node c = s->nodes[slots[k]];
+    for (int a = 0; a < 3; ++a) {
+      q.lo[a][k] = round_down_any(c.lo[a]);
+      q.hi[a][k] = round_up_f(c.hi[a]);
+    }
+    q.ref[k] = c.count > 0 ? c.left : collapse(s, slots[k]);
+    q.count[k] = c.count > 0 ? c.count : 0;
+  }
+  s->nodes4[me] = q;
+  return me;
+}
+
+// SAH on the top 8 levels; if the tree outgrows the traversal stack, which
+// holds three pending children per 4-wide level and so needs a binary depth
+// of at most 32 (only for far larger scenes than generated here), median
+// splits only, which are ceil(log2(n / 4)) + 2 <= 32 deep. Then the 4-wide
+// tree and the leaf-order triangle geometry.
 void build_bvh(mlgi_scene* s) {
   build_bvh_with(s, 8);
-  if (bvh_depth(s, 0) > kTraceStack) build_bvh_with(s, 0);
+  if (bvh_depth(s, 0) > 2 * kTraceStack / 3) build_bvh_with(s, 0);
+  s->nodes4.clear();
+  s->nodes4.reserve(s->nodes.size() / 2 + 2);
+  collapse(s, 0);
+  s->geo.assign(s->tris.size() + kGeoPad, TriGeo{});
+  for (size_t i = 0; i < s->tris.size(); ++i) {
+    const Tri& t = s->tris[s->order[i]];
+    s->geo[i] = TriGeo{t.v0, t.v1 - t.v0, t.v2 - t.v0};
+  }
 }
 
 // 4x4 column-major helpers in double
@@ -465,7 +528,8 @@ int mlgi_scene_render_gbuffer(const mlgi_scene* s, const mlgi_camera* cam, int32
   const RayCam rc = ray_cam(cam);
   const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    for (int x = 0; x < w; ++x) shade_gbuffer(sv, rc, x, y, w, h, depth, normal);
+    LocalStack st;
+    for (int x = 0; x < w; ++x) shade_gbuffer(st, sv, rc, x, y, w, h, depth, normal);
   });
   return MLGI_OK;
 }
@@ -478,7 +542,8 @@ int mlgi_scene_render_rsm(const mlgi_scene* s, const mlgi_camera* light, int32_t
   const RayCam rc = ray_cam(light);
   const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    for (int x = 0; x < w; ++x) shade_rsm(sv, rc, x, y, w, h, diffuse, normal, depth);
+    LocalStack st;
+    for (int x = 0; x < w; ++x) shade_rsm(st, sv, rc, x, y, w, h, diffuse, normal, depth);
   });
   if (inv_bvp16 && !mlgi_scene_inv_biased_vp(light, inv_bvp16)) return MLGI_ERR_INVALID_ARGUMENT;
   return MLGI_OK;
@@ -490,7 +555,8 @@ int mlgi_scene_render_surface(const mlgi_scene* s, const mlgi_camera* cam, int32
   const RayCam rc = ray_cam(cam);
   const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    for (int x = 0; x < w; ++x) shade_surface(sv, rc, x, y, w, h, albedo, moments);
+    LocalStack st;
+    for (int x = 0; x < w; ++x) shade_surface(st, sv, rc, x, y, w, h, albedo, moments);
   });
   return MLGI_OK;
 }
@@ -552,7 +618,8 @@ extern "C" int mlgi_scene_sample_points(const mlgi_scene* s, int64_t n_points, i
 
 namespace {
 SceneView view_of(const mlgi_scene* s) {
-  return SceneView{s->nodes.data(), s->tris.data(), s->order.data(), (int)s->nodes.size(), (int)s->tris.size()};
+  return SceneView{s->nodes4.data(), s->geo.data(), s->tris.data(), s->order.data(), (int)s->nodes4.size(),
+                   (int)s->tris.size()};
 }
 }  // namespace
 

@@ -35,6 +35,8 @@ struct Tri {
   float albedo[3];
 };
 
+// Binary BVH node of the build (scene.cpp); the ray caster walks the 4-wide
+// form below.
 struct Node {
   double lo[3], hi[3];
   // leaf: count > 0, left = first index; inner: left child (the smaller
@@ -42,107 +44,186 @@ struct Node {
   int left, count;
 };
 
+// 4-wide BVH node: the binary tree with every inner node's grandchildren
+// pulled up into its child slots, stored as structures of arrays (the four
+// boxes and references of one node are one 128-byte line). Boxes are floats
+// rounded outward, so each still contains its triangles.
+struct alignas(16) Node4 {
+  float lo[3][4], hi[3][4];
+  int ref[4];    // inner child: node index; leaf child: first triangle (leaf order)
+  int count[4];  // leaf child: triangles (1..4); inner child: 0; empty slot: -1
+};
+
+// Triangle geometry in leaf order for the intersection test: v0 and the two
+// edges v1 - v0, v2 - v0 (computed once, in double, as the test would). The
+// array carries three zero entries past the last triangle, so that a leaf's
+// four slots can be loaded without a bounds test.
+constexpr int kGeoPad = 3;
+struct TriGeo {
+  D3 v0, e1, e2;
+};
+
 struct Hit {
   double t;
   int tri;
 };
 
-// The BVH as flat arrays (host vectors or device copies).
+// The scene as flat arrays (host vectors or device copies).
 struct SceneView {
-  const Node* nodes;
-  const Tri* tris;
-  const int* order;  // leaf triangle order
-  int n_nodes, n_tris;
+  const Node4* nodes4;
+  const TriGeo* geo;  // leaf order
+  const Tri* tris;    // scene order (normals, albedo, sampling)
+  const int* order;   // leaf order -> scene order
+  int n_nodes4, n_tris;
 };
 
-MLGI_HD inline bool box_hit(const Node& nd, D3 o, D3 inv, double tmax, double* tnear) {
+// Slab test of child slot c of a 4-wide node (the float bounds taken exactly
+// into double), without early exits: t0 only grows and t1 only shrinks, so
+// testing once after the three axes gives the same answer.
+MLGI_HD inline bool box4_hit(const Node4& nd, int c, D3 o, D3 inv, double tmax, double* tnear) {
   double t0 = 0.0, t1 = tmax;
   for (int k = 0; k < 3; ++k) {
-    double a = (nd.lo[k] - comp(o, k)) * comp(inv, k);
-    double b = (nd.hi[k] - comp(o, k)) * comp(inv, k);
-    if (a > b) {
-      const double t = a;
-      a = b;
-      b = t;
-    }
-    t0 = (t0 < a) ? a : t0;  // std::max(t0, a)
-    t1 = (b < t1) ? b : t1;  // std::min(t1, b)
-    if (t0 > t1) return false;
+    const double a = ((double)nd.lo[k][c] - comp(o, k)) * comp(inv, k);
+    const double b = ((double)nd.hi[k][c] - comp(o, k)) * comp(inv, k);
+    const double lo = (a > b) ? b : a, hi = (a > b) ? a : b;
+    t0 = (t0 < lo) ? lo : t0;  // std::max(t0, lo)
+    t1 = (hi < t1) ? hi : t1;  // std::min(t1, hi)
   }
   *tnear = t0;
-  return true;
+  return !(t0 > t1);
 }
 
 // Moller-Trumbore, two-sided
-MLGI_HD inline bool tri_hit(const Tri& t, D3 o, D3 d, double tmax, double* th) {
-  D3 e1 = t.v1 - t.v0, e2 = t.v2 - t.v0;
-  D3 p = cross(d, e2);
-  double det = dot(e1, p);
+MLGI_HD inline bool tri_hit(const TriGeo& g, D3 o, D3 d, double tmax, double* th) {
+  D3 p = cross(d, g.e2);
+  double det = dot(g.e1, p);
   if (fabs(det) < 1e-14) return false;
   double inv = 1.0 / det;
-  D3 s = o - t.v0;
+  D3 s = o - g.v0;
   double u = dot(s, p) * inv;
   if (u < 0.0 || u > 1.0) return false;
-  D3 q = cross(s, e1);
+  D3 q = cross(s, g.e1);
   double v = dot(d, q) * inv;
   if (v < 0.0 || u + v > 1.0) return false;
-  double tt = dot(e2, q) * inv;
+  double tt = dot(g.e2, q) * inv;
   if (tt <= 0.0 || tt >= tmax) return false;
   *th = tt;
   return true;
 }
 
-// The traversal keeps at most one pending node per tree level; the build
-// keeps the BVH at most 32 levels deep (it falls back to median splits
-// otherwise), and 32 entries are few enough for the device compiler to keep
-// the stack in registers.
-constexpr int kTraceStack = 32;
+// Pending children of the traversal: at most three per level of the 4-wide
+// tree, whose depth the build keeps within 16 (binary depth 32).
+constexpr int kTraceStack = 48;
 
-// Nearest hit with t < tmax. At an inner node both children's boxes are
-// tested and the child on the ray's side of the split axis is visited first
-// (the other pushed): the sign of the direction along the axis, not the box
-// distances, decides, so the 64 coherent rays of a wave take the same order
-// and do not diverge. A popped node's box is tested again against the
-// current nearest hit. Among hits at exactly equal t the first one reached
-// wins.
-MLGI_HD inline Hit trace(const SceneView& s, D3 o, D3 d, double tmax) {
+// A pending child: ref << 3 | count (inner node: count 0; leaf: 1..4
+// triangles) and its box's entry distance rounded down to a float (the pop
+// test then never drops a box the exact test would keep).
+struct StackEntry {
+  int rc;
+  float tnear;
+};
+
+// Traversal stack in the caller's memory (host: a local array).
+struct LocalStack {
+  StackEntry e[kTraceStack];
+  MLGI_HD void put(int i, const StackEntry& v) { e[i] = v; }
+  MLGI_HD StackEntry get(int i) const { return e[i]; }
+};
+
+// Largest float <= x for x >= 0 (a rounded-up f is then > 0: one step down
+// is the previous float).
+MLGI_HD inline float round_down_f(double x) {
+  float f = (float)x;
+  if ((double)f > x) {
+    union {
+      float f;
+      uint32_t u;
+    } b;
+    b.f = f;
+    b.u -= 1u;
+    f = b.f;
+  }
+  return f;
+}
+
+// Compare-exchange of the child order: by entry distance, then slot.
+MLGI_HD inline void order2(double* t, int* slot, int* rc, int i, int j) {
+  if (t[i] > t[j] || (t[i] == t[j] && slot[i] > slot[j])) {
+    const double tt = t[i];
+    t[i] = t[j];
+    t[j] = tt;
+    const int ss = slot[i];
+    slot[i] = slot[j];
+    slot[j] = ss;
+    const int rr = rc[i];
+    rc[i] = rc[j];
+    rc[j] = rr;
+  }
+}
+
+// Nearest hit with t < tmax. At an inner node the whole 128-byte node is read
+// at once and its four child boxes tested, the hit children are ordered by
+// entry distance (then slot; missed slots last), the nearest is visited and
+// the others pushed, farthest first. A popped child is skipped if its box is
+// entered beyond the current nearest hit. Leaves test their triangles in leaf
+// order; among hits at exactly equal t the first one reached wins. All array
+// indices are compile-time (registers, no private memory), and host and
+// device run this same code, so they take the same decisions.
+template <class Stack>
+MLGI_HD inline Hit trace(const SceneView& s, D3 o, D3 d, double tmax, Stack& st) {
   Hit h{tmax, -1};
   D3 inv = {1.0 / (d.x != 0 ? d.x : 1e-300), 1.0 / (d.y != 0 ? d.y : 1e-300), 1.0 / (d.z != 0 ? d.z : 1e-300)};
-  int stack[kTraceStack];
+  const double kMissed = 1e300;
   int sp = 0;
-  double tn;
-  int node = box_hit(s.nodes[0], o, inv, h.t, &tn) ? 0 : -1;
-  while (node >= 0) {
-    const Node& nd = s.nodes[node];
-    int next = -1;
-    if (nd.count > 0) {
-      for (int i = nd.left; i < nd.left + nd.count; ++i) {
-        double th;
-        const int ti = s.order[i];
-        if (tri_hit(s.tris[ti], o, d, h.t, &th)) {
-          h.t = th;
-          h.tri = ti;
-        }
+  int ref = 0, count = 0;  // the root, an inner node
+  while (true) {
+    if (count == 0) {
+      const Node4 nd = s.nodes4[ref];
+      double tk[4];
+      int slot[4], rc[4];
+      for (int k = 0; k < 4; ++k) {
+        double tn;
+        const bool hit = box4_hit(nd, k, o, inv, h.t, &tn) && nd.count[k] >= 0;
+        tk[k] = hit ? tn : kMissed;
+        slot[k] = k;
+        rc[k] = (nd.ref[k] << 3) | (nd.count[k] & 7);
+      }
+      order2(tk, slot, rc, 0, 1);
+      order2(tk, slot, rc, 2, 3);
+      order2(tk, slot, rc, 0, 2);
+      order2(tk, slot, rc, 1, 3);
+      order2(tk, slot, rc, 1, 2);
+      for (int i = 3; i >= 1; --i)  // never full (see kTraceStack); checked all the same
+        if (tk[i] < kMissed && sp < kTraceStack) st.put(sp++, StackEntry{rc[i], round_down_f(tk[i])});
+      if (tk[0] < kMissed) {
+        ref = rc[0] >> 3;
+        count = rc[0] & 7;
+        continue;
       }
     } else {
-      const bool left_first = comp(d, -1 - nd.count) >= 0.0;
-      const int a = left_first ? nd.left : nd.left + 1, b = left_first ? nd.left + 1 : nd.left;
-      double ta, tb;
-      const bool ha = box_hit(s.nodes[a], o, inv, h.t, &ta);
-      const bool hb = box_hit(s.nodes[b], o, inv, h.t, &tb);
-      if (ha) {
-        next = a;
-        if (hb && sp < kTraceStack) stack[sp++] = b;
-      } else if (hb) {
-        next = b;
+      TriGeo g[4];
+      for (int k = 0; k < 4; ++k) g[k] = s.geo[ref + k];  // padded: in bounds for every leaf
+      for (int k = 0; k < 4; ++k) {
+        double th;
+        if (k < count && tri_hit(g[k], o, d, h.t, &th)) {
+          h.t = th;
+          h.tri = ref + k;
+        }
       }
     }
-    while (next < 0 && sp > 0) {  // pop until a node still in front of the nearest hit
-      const int c = stack[--sp];
-      if (box_hit(s.nodes[c], o, inv, h.t, &tn)) next = c;
+    bool more = false;
+    while (sp > 0) {
+      const StackEntry e = st.get(--sp);
+      if (!((double)e.tnear > h.t)) {
+        ref = e.rc >> 3;
+        count = e.rc & 7;
+        more = true;
+        break;
+      }
     }
-    node = next;
+    if (!more) break;
   }
+  if (h.tri >= 0) h.tri = s.order[h.tri];
   return h;
 }
 
@@ -184,11 +265,12 @@ MLGI_HD inline double window_depth(const RayCam& rc, D3 p) {
 
 // G-buffer texel (x, y): window depth and face normal * 0.5 + 0.5, the
 // normal turned towards the viewer; background = depth 1, normal 0.5.
-MLGI_HD inline void shade_gbuffer(const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
+template <class Stack>
+MLGI_HD inline void shade_gbuffer(Stack& st, const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
                                   float* depth, float* normal) {
   D3 o, d;
   pixel_ray(rc, x, y, w, h, &o, &d);
-  Hit hit = trace(s, o, d, 1.0);
+  Hit hit = trace(s, o, d, 1.0, st);
   const size_t p = (size_t)y * w + x;
   if (hit.tri < 0) {
     depth[p] = 1.0f;
@@ -205,11 +287,12 @@ MLGI_HD inline void shade_gbuffer(const SceneView& s, const RayCam& rc, int x, i
 
 // RSM texel (x, y) of a spot light: as the G-buffer, plus the albedo inside
 // the cone (0 outside) as the diffuse flux channel.
-MLGI_HD inline void shade_rsm(const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
+template <class Stack>
+MLGI_HD inline void shade_rsm(Stack& st, const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
                               float* diffuse, float* normal, float* depth) {
   D3 o, d;
   pixel_ray(rc, x, y, w, h, &o, &d);
-  Hit hit = trace(s, o, d, 1.0);
+  Hit hit = trace(s, o, d, 1.0, st);
   const size_t p = (size_t)y * w + x;
   if (hit.tri < 0) {
     depth[p] = 1.0f;
@@ -237,11 +320,12 @@ MLGI_HD inline void shade_rsm(const SceneView& s, const RayCam& rc, int x, int y
 // screen-space derivative term to d^2 that a ray caster has no analogue
 // of; it is left out). Background: moments (0, 0). Either output may be
 // null.
-MLGI_HD inline void shade_surface(const SceneView& s, const RayCam& rc, int x, int y, int w, int h, float* albedo,
+template <class Stack>
+MLGI_HD inline void shade_surface(Stack& st, const SceneView& s, const RayCam& rc, int x, int y, int w, int h, float* albedo,
                                   float* moments) {
   D3 o, d;
   pixel_ray(rc, x, y, w, h, &o, &d);
-  Hit hit = trace(s, o, d, 1.0);
+  Hit hit = trace(s, o, d, 1.0, st);
   const size_t p = (size_t)y * w + x;
   if (hit.tri < 0) {
     if (albedo) albedo[p * 3] = albedo[p * 3 + 1] = albedo[p * 3 + 2] = 0.0f;
