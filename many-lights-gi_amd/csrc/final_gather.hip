@@ -17,8 +17,8 @@
 //   * the VPL's tangent plane has the whole position box behind it, or
 //   * the VPL lies behind the tangent plane of every pixel,
 // each by a margin far above FP32 rounding. Either way every pixel would get
-// angleFactor == 0 from that VPL. A ballot turns the tests into a 64-bit
-// survivor mask the wave iterates.
+// angleFactor == 0 from that VPL. VPLs of zero flux are dropped too. A ballot
+// turns the tests into a 64-bit survivor mask the wave iterates.
 //
 // Per pair, the hemisphere tests come first and use only fused multiply-adds
 // against the VPL's plane offset n.p (stored in the record) and the pixel's
@@ -166,13 +166,20 @@ __device__ __forceinline__ bool wave_bounds(const Pixel& p, WaveBounds& wb) {
   return mnx <= mxx;
 }
 
-// Survivor mask of the `count` staged VPLs of the current chunk.
+// Survivor mask of the `count` staged VPLs of the current chunk. A VPL with
+// zero flux (an RSM texel outside the spot cone; MIXED: zero after rounding
+// to halves, the values the sums multiply) is dropped as well: its every
+// contribution is flux * weight = +0 with a finite weight, which leaves the
+// sums' bits unchanged.
+template <bool kMixed>
 __device__ __forceinline__ unsigned long long cull_chunk(const VplLds* tile, int count,
                                                          const WaveBounds& w) {
   const int lane = threadIdx.x & 63;
   bool keep = false;
   if (lane < count) {
     const VplLds& v = tile[lane];
+    const bool dark = kMixed ? (__builtin_bit_cast(uint32_t, v.cr) | __builtin_bit_cast(uint32_t, v.cg)) == 0u
+                             : (v.cr == 0.0f && v.cg == 0.0f && v.cb == 0.0f);
     // (1) max over the box of n . (p - p_vpl) = n . (c - p_vpl) + |n| . h
     const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
     const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
@@ -186,7 +193,7 @@ __device__ __forceinline__ unsigned long long cull_chunk(const VplLds* tile, int
                      fmaxf(w.nlz * v.pz, w.nhz * v.pz);
     const float tol2 = 1e-6f * (w.scale + fabsf(v.px) + fabsf(v.py) + fabsf(v.pz));
     const bool behind_pixels = w.plane_lo - mx > tol2;
-    keep = !(behind_vpl || behind_pixels);
+    keep = !(behind_vpl || behind_pixels || dark);
   }
   return __ballot(keep);
 }
@@ -420,7 +427,7 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
     if ((int)threadIdx.x < count) vtile[threadIdx.x] = stage_vpl<kMixed>(crec + (size_t)threadIdx.x * kVplFloats);
     __syncthreads();
     if (!wave_live) continue;
-    unsigned long long mask = cull_chunk(vtile, count, wb);
+    unsigned long long mask = cull_chunk<kMixed>(vtile, count, wb);
     while (mask) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void b
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      unsigned long long mask = cull_chunk(vtile, count, wb);
+      unsigned long long mask = cull_chunk<kMixed>(vtile, count, wb);
       while (mask) {
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1ull;
