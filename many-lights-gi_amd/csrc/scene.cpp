@@ -528,8 +528,8 @@ int mlgi_scene_render_gbuffer(const mlgi_scene* s, const mlgi_camera* cam, int32
   const RayCam rc = ray_cam(cam);
   const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    LocalStack st;
-    for (int x = 0; x < w; ++x) shade_gbuffer(st, sv, rc, x, y, w, h, depth, normal);
+    StackTracer<LocalStack> tr;
+    for (int x = 0; x < w; ++x) shade_gbuffer(tr, sv, rc, x, y, w, h, depth, normal);
   });
   return MLGI_OK;
 }
@@ -542,8 +542,8 @@ int mlgi_scene_render_rsm(const mlgi_scene* s, const mlgi_camera* light, int32_t
   const RayCam rc = ray_cam(light);
   const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    LocalStack st;
-    for (int x = 0; x < w; ++x) shade_rsm(st, sv, rc, x, y, w, h, diffuse, normal, depth);
+    StackTracer<LocalStack> tr;
+    for (int x = 0; x < w; ++x) shade_rsm(tr, sv, rc, x, y, w, h, diffuse, normal, depth);
   });
   if (inv_bvp16 && !mlgi_scene_inv_biased_vp(light, inv_bvp16)) return MLGI_ERR_INVALID_ARGUMENT;
   return MLGI_OK;
@@ -555,8 +555,8 @@ int mlgi_scene_render_surface(const mlgi_scene* s, const mlgi_camera* cam, int32
   const RayCam rc = ray_cam(cam);
   const SceneView sv = view_of(s);
   parallel_rows(h, nthreads(threads), [&](int y) {
-    LocalStack st;
-    for (int x = 0; x < w; ++x) shade_surface(st, sv, rc, x, y, w, h, albedo, moments);
+    StackTracer<LocalStack> tr;
+    for (int x = 0; x < w; ++x) shade_surface(tr, sv, rc, x, y, w, h, albedo, moments);
   });
   return MLGI_OK;
 }

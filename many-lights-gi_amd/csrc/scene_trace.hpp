@@ -77,20 +77,27 @@ struct SceneView {
   int n_nodes4, n_tris;
 };
 
-// Slab test of child slot c of a 4-wide node (the float bounds taken exactly
-// into double), without early exits: t0 only grows and t1 only shrinks, so
-// testing once after the three axes gives the same answer.
-MLGI_HD inline bool box4_hit(const Node4& nd, int c, D3 o, D3 inv, double tmax, double* tnear) {
+// Slab test of a float box (bounds taken exactly into double), without early
+// exits: t0 only grows and t1 only shrinks, so testing once after the three
+// axes gives the same answer.
+MLGI_HD inline bool box_hit(const float lo_[3], const float hi_[3], D3 o, D3 inv, double tmax, double* tnear) {
   double t0 = 0.0, t1 = tmax;
   for (int k = 0; k < 3; ++k) {
-    const double a = ((double)nd.lo[k][c] - comp(o, k)) * comp(inv, k);
-    const double b = ((double)nd.hi[k][c] - comp(o, k)) * comp(inv, k);
+    const double a = ((double)lo_[k] - comp(o, k)) * comp(inv, k);
+    const double b = ((double)hi_[k] - comp(o, k)) * comp(inv, k);
     const double lo = (a > b) ? b : a, hi = (a > b) ? a : b;
     t0 = (t0 < lo) ? lo : t0;  // std::max(t0, lo)
     t1 = (hi < t1) ? hi : t1;  // std::min(t1, hi)
   }
   *tnear = t0;
   return !(t0 > t1);
+}
+
+// Child slot c of a 4-wide node.
+MLGI_HD inline bool box4_hit(const Node4& nd, int c, D3 o, D3 inv, double tmax, double* tnear) {
+  const float lo[3] = {nd.lo[0][c], nd.lo[1][c], nd.lo[2][c]};
+  const float hi[3] = {nd.hi[0][c], nd.hi[1][c], nd.hi[2][c]};
+  return box_hit(lo, hi, o, inv, tmax, tnear);
 }
 
 // Moller-Trumbore, two-sided
@@ -169,11 +176,27 @@ MLGI_HD inline void order2(double* t, int* slot, int* rc, int i, int j) {
 // order; among hits at exactly equal t the first one reached wins. All array
 // indices are compile-time (registers, no private memory), and host and
 // device run this same code, so they take the same decisions.
+MLGI_HD inline D3 ray_inverse(D3 d) {
+  return {1.0 / (d.x != 0 ? d.x : 1e-300), 1.0 / (d.y != 0 ? d.y : 1e-300), 1.0 / (d.z != 0 ? d.z : 1e-300)};
+}
+
+// Entry distance of a missed child slot: orders after every hit one.
+constexpr double kMissed = 1e300;
+
+// The ordering network of an inner step: hit children by entry distance,
+// then slot; missed slots last.
+MLGI_HD inline void order_children(double* tk, int* slot, int* rc) {
+  order2(tk, slot, rc, 0, 1);
+  order2(tk, slot, rc, 2, 3);
+  order2(tk, slot, rc, 0, 2);
+  order2(tk, slot, rc, 1, 3);
+  order2(tk, slot, rc, 1, 2);
+}
+
 template <class Stack>
 MLGI_HD inline Hit trace(const SceneView& s, D3 o, D3 d, double tmax, Stack& st) {
   Hit h{tmax, -1};
-  D3 inv = {1.0 / (d.x != 0 ? d.x : 1e-300), 1.0 / (d.y != 0 ? d.y : 1e-300), 1.0 / (d.z != 0 ? d.z : 1e-300)};
-  const double kMissed = 1e300;
+  const D3 inv = ray_inverse(d);
   int sp = 0;
   int ref = 0, count = 0;  // the root, an inner node
   while (true) {
@@ -188,11 +211,7 @@ MLGI_HD inline Hit trace(const SceneView& s, D3 o, D3 d, double tmax, Stack& st)
         slot[k] = k;
         rc[k] = (nd.ref[k] << 3) | (nd.count[k] & 7);
       }
-      order2(tk, slot, rc, 0, 1);
-      order2(tk, slot, rc, 2, 3);
-      order2(tk, slot, rc, 0, 2);
-      order2(tk, slot, rc, 1, 3);
-      order2(tk, slot, rc, 1, 2);
+      order_children(tk, slot, rc);
       for (int i = 3; i >= 1; --i)  // never full (see kTraceStack); checked all the same
         if (tk[i] < kMissed && sp < kTraceStack) st.put(sp++, StackEntry{rc[i], round_down_f(tk[i])});
       if (tk[0] < kMissed) {
@@ -226,6 +245,17 @@ MLGI_HD inline Hit trace(const SceneView& s, D3 o, D3 d, double tmax, Stack& st)
   if (h.tri >= 0) h.tri = s.order[h.tri];
   return h;
 }
+
+// One ray per thread over a traversal stack (the host renderer; the device's
+// one-lane-per-texel form). The shading functions below call a tracer and
+// store only where it says so (a cooperative device tracer has several lanes
+// per ray, one of which stores).
+template <class Stack>
+struct StackTracer {
+  Stack st;
+  MLGI_HD Hit operator()(const SceneView& s, D3 o, D3 d, double tmax) { return trace(s, o, d, tmax, st); }
+  MLGI_HD bool stores() const { return true; }
+};
 
 // 4x4 column-major, double
 struct M4 {
@@ -265,12 +295,13 @@ MLGI_HD inline double window_depth(const RayCam& rc, D3 p) {
 
 // G-buffer texel (x, y): window depth and face normal * 0.5 + 0.5, the
 // normal turned towards the viewer; background = depth 1, normal 0.5.
-template <class Stack>
-MLGI_HD inline void shade_gbuffer(Stack& st, const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
+template <class Tracer>
+MLGI_HD inline void shade_gbuffer(Tracer& tr, const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
                                   float* depth, float* normal) {
   D3 o, d;
   pixel_ray(rc, x, y, w, h, &o, &d);
-  Hit hit = trace(s, o, d, 1.0, st);
+  Hit hit = tr(s, o, d, 1.0);
+  if (!tr.stores()) return;
   const size_t p = (size_t)y * w + x;
   if (hit.tri < 0) {
     depth[p] = 1.0f;
@@ -287,12 +318,13 @@ MLGI_HD inline void shade_gbuffer(Stack& st, const SceneView& s, const RayCam& r
 
 // RSM texel (x, y) of a spot light: as the G-buffer, plus the albedo inside
 // the cone (0 outside) as the diffuse flux channel.
-template <class Stack>
-MLGI_HD inline void shade_rsm(Stack& st, const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
+template <class Tracer>
+MLGI_HD inline void shade_rsm(Tracer& tr, const SceneView& s, const RayCam& rc, int x, int y, int w, int h,
                               float* diffuse, float* normal, float* depth) {
   D3 o, d;
   pixel_ray(rc, x, y, w, h, &o, &d);
-  Hit hit = trace(s, o, d, 1.0, st);
+  Hit hit = tr(s, o, d, 1.0);
+  if (!tr.stores()) return;
   const size_t p = (size_t)y * w + x;
   if (hit.tri < 0) {
     depth[p] = 1.0f;
@@ -320,12 +352,13 @@ MLGI_HD inline void shade_rsm(Stack& st, const SceneView& s, const RayCam& rc, i
 // screen-space derivative term to d^2 that a ray caster has no analogue
 // of; it is left out). Background: moments (0, 0). Either output may be
 // null.
-template <class Stack>
-MLGI_HD inline void shade_surface(Stack& st, const SceneView& s, const RayCam& rc, int x, int y, int w, int h, float* albedo,
+template <class Tracer>
+MLGI_HD inline void shade_surface(Tracer& tr, const SceneView& s, const RayCam& rc, int x, int y, int w, int h, float* albedo,
                                   float* moments) {
   D3 o, d;
   pixel_ray(rc, x, y, w, h, &o, &d);
-  Hit hit = trace(s, o, d, 1.0, st);
+  Hit hit = tr(s, o, d, 1.0);
+  if (!tr.stores()) return;
   const size_t p = (size_t)y * w + x;
   if (hit.tri < 0) {
     if (albedo) albedo[p * 3] = albedo[p * 3 + 1] = albedo[p * 3 + 2] = 0.0f;

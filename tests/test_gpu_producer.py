@@ -1,5 +1,9 @@
 """GPU: device G-buffer / RSM producer equals the host renderer bit for bit
-(same double-precision ray caster, -ffp-contract=off on both sides)."""
+(same double-precision ray caster, -ffp-contract=off on both sides).
+
+Images of at most 512^2 texels run the cooperative form (four lanes per ray),
+larger ones one lane per ray; the 720x400 G-buffer (288,000 texels) covers the
+latter, the RSMs and smaller G-buffers the former."""
 import numpy as np
 import pytest
 
@@ -9,7 +13,7 @@ from mlgi import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n_random,size", [(500, (96, 64)), (40_000, (128, 128))])
+@pytest.mark.parametrize("n_random,size", [(500, (96, 64)), (40_000, (128, 128)), (40_000, (720, 400))])
 def test_device_renders_equal_host(ctx, n_random, size):
     scene = W._scene(1, n_random)
     dev = scene.upload(ctx)
