@@ -838,6 +838,7 @@ int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::v
     tt += n.tiles;
     tb += n.bins;
     work += mlgi::splat_work_bytes(f.n_points);
+    plan->max_ism_side = std::max(plan->max_ism_side, f.ism_side);
     const int k = key[order[r]];
     const int prec = k >> 16;
     if (r == 0 || key[order[r - 1]] != k) plan->pullpush.push_back({r, 0, 0, f.ism_side, f.levels});
@@ -883,7 +884,8 @@ int batch_enqueue(mlgi_context* c, const mlgi::BatchFrame* dev, const BatchPlan&
   {
     Scope sc(c, kSplat);  // ISM-major: mask, tiles (which clear the atlases), spills
     if ((rc = hip_check(c, mlgi::launch_batch_mask(c->stream, dev, plan.count, plan.total_bins), "batch_mask")) ||
-        (rc = hip_check(c, mlgi::launch_batch_tiles(c->stream, dev, plan.count, plan.total_tiles), "batch_tiles")) ||
+        (rc = hip_check(c, mlgi::launch_batch_tiles(c->stream, dev, plan.count, plan.total_tiles, plan.max_ism_side),
+                          "batch_tiles")) ||
         (rc = hip_check(c, mlgi::launch_batch_spill(c->stream, dev, plan.count, plan.total_bins), "batch_spill")))
       return rc;
   }

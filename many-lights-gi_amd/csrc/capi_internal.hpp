@@ -83,6 +83,7 @@ struct BatchPlan {
   int count = 0;
   std::vector<int> row_of;  // caller's frame index -> table row
   uint32_t total_vpl = 0, total_tiles = 0, total_bins = 0;
+  int max_ism_side = 0;  // largest ISM side of the batch (tile pass LDS)
   size_t work_bytes = 0;  // splat work areas of all frames (batch_attach_work)
   struct Run {
     int begin, count;

@@ -239,10 +239,20 @@ constexpr int kBinsPerWave = kMaxTest / 4;     // the 16 candidate bins over 4 w
 constexpr int kStep = 256;                     // points per bin per step
 constexpr int kQueue = kBinsPerWave * kStep;   // hit indices of one step
 
-struct TileLds {
-  uint32_t tile[kTileMax * kTileMax];  // 16 KiB
-  uint32_t queue[4][kQueue];           // 16 KiB: per-wave point-index queues
-};
+// LDS of the tile pass, sized per launch for the largest ISM side S of its
+// items: the S x S tile, then the four waves' point queues. Up to S = 32 the
+// queue entries are point indices (16 KiB; 20 KiB in all lets eight
+// workgroups share a CU, the wave limit); at S = 64 they are 16-bit codes of
+// the step's bin slot t and offset o, t * kStep + o (8 KiB; 24 KiB in all,
+// six workgroups where 32 KiB allowed five). The codes cost a decode per
+// point, which measured slower where occupancy does not rise.
+extern __shared__ __align__(16) uint32_t tile_lds[];
+static_assert(kBinsPerWave * kStep <= 65536, "queue codes are 16-bit");
+
+template <class QueueT>
+size_t tile_lds_bytes(int max_side) {
+  return (size_t)max_side * max_side * sizeof(uint32_t) + 4 * (size_t)kQueue * sizeof(QueueT);
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -264,14 +274,14 @@ struct TileJob {
 };
 
 // One queued point splatted into ISM c (tile origin ox, oy).
-__device__ __forceinline__ void splat_queued(const TileJob& j, TileLds& sm, const Candidate& c, float4 rec,
+__device__ __forceinline__ void splat_queued(const TileJob& j, uint32_t* tile, const Candidate& c, float4 rec,
                                              uint32_t ox, uint32_t oy, float inv_n1d, float side) {
   uint32_t tx, ty, value;
   if (!project(c, dev::make(rec.x, rec.y, rec.z), radius_byte_of(rec.w), inv_n1d, side, j.z_far, tx, ty, value))
     return;
   const uint32_t lx = tx - ox, ly = ty - oy;  // wraps above S when left of / above the tile
   if (lx < (uint32_t)j.ism_side && ly < (uint32_t)j.ism_side) {
-    __hip_atomic_fetch_min(&sm.tile[ly * j.ism_side + lx], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_min(&tile[ly * j.ism_side + lx], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else {
     const uint32_t k = gatomic_add(j.spill_state, 1u);
     if (k < kSpillCap)
@@ -281,14 +291,17 @@ __device__ __forceinline__ void splat_queued(const TileJob& j, TileLds& sm, cons
   }
 }
 
-// Tile pass of ISM v (all 256 lanes; ends after the tile is written).
-__device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v) {
+// Tile pass of ISM v (all 256 lanes; ends after the tile is written). `tile`
+// holds at least S * S words, `queues` 4 * kQueue entries.
+template <class QueueT>
+__device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, QueueT* queues, int v) {
+  constexpr bool kCoded = sizeof(QueueT) == 2;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int S = j.ism_side;
   const int tile_y = v / j.n1d, tile_x = v - tile_y * j.n1d;
   const int n4 = S * S / 4;
-  uint4* t4 = reinterpret_cast<uint4*>(sm.tile);
+  uint4* t4 = reinterpret_cast<uint4*>(tile);
   const uint4 ones = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
   for (int i = tid; i < n4; i += 256) t4[i] = ones;
   __syncthreads();
@@ -303,7 +316,7 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v)
     // flight per lane), queues the hits' point indices in LDS (ballot +
     // mbcnt), then splats the queued points 64 at a time with every lane
     // busy, the next 64 records requested while the current ones project.
-    uint32_t* q = sm.queue[wave];
+    QueueT* q = queues + wave * kQueue;
     uint32_t beg[kBinsPerWave], cnt[kBinsPerWave];
     uint32_t longest = 0;
 #pragma unroll
@@ -334,17 +347,26 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, TileLds& sm, int v)
           if (hit) {
             const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-            q[slot] = beg[t] + s0 + (uint32_t)(k * 64 + lane);
+            const uint32_t o = (uint32_t)(k * 64 + lane);
+            q[slot] = (QueueT)(kCoded ? (uint32_t)t * kStep + o : beg[t] + s0 + o);
           }
           qn += __builtin_popcountll(bal);
         }
       wave_sync();
       if (qn > 0) {
-        // indices past the queue end read its last entry (a valid point)
-        float4 rec = gload4(j.points + q[min(lane, qn - 1)]);
+        // entry -> point index beg[t] + s0 + o; lanes past the queue end
+        // read its last entry (a valid point)
+        auto point_of = [&](int e) {
+          const uint32_t x = q[min(e, qn - 1)];
+          if (!kCoded) return x;
+          const uint32_t t = x / kStep, o = x % kStep;
+          const uint32_t b = t == 0 ? beg[0] : t == 1 ? beg[1] : t == 2 ? beg[2] : beg[3];
+          return b + s0 + o;
+        };
+        float4 rec = gload4(j.points + point_of(lane));
         for (int c0 = 0; c0 < qn; c0 += 64) {
-          const float4 next = gload4(j.points + q[min(c0 + 64 + lane, qn - 1)]);
-          if (c0 + lane < qn) splat_queued(j, sm, c, rec, ox, oy, inv_n1d, side);
+          const float4 next = gload4(j.points + point_of(c0 + 64 + lane));
+          if (c0 + lane < qn) splat_queued(j, tile, c, rec, ox, oy, inv_n1d, side);
           rec = next;
         }
       }
@@ -398,11 +420,12 @@ __global__ __launch_bounds__(256) void ism_mask_kernel(TileJob j, int n_bins) {
   }
 }
 
+template <class QueueT>
 __global__ __launch_bounds__(256) void ism_tiles_kernel(TileJob j, int n_tiles) {
-  __shared__ TileLds sm;
+  QueueT* queues = reinterpret_cast<QueueT*>(tile_lds + j.ism_side * j.ism_side);
   const XcdRange r = xcd_range((uint32_t)n_tiles, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
   for (uint32_t v = r.start; v < r.end; v += r.stride) {
-    splat_tile(j, sm, (int)v);
+    splat_tile(j, tile_lds, queues, (int)v);
     __syncthreads();
   }
 }
@@ -429,13 +452,14 @@ __global__ __launch_bounds__(256) void batch_mask_kernel(const BatchFrame* __res
   }
 }
 
+template <class QueueT>
 __global__ __launch_bounds__(256) void batch_tiles_kernel(const BatchFrame* __restrict__ frames, int n,
-                                                          uint32_t total) {
-  __shared__ TileLds sm;
+                                                          uint32_t total, int max_side) {
+  QueueT* queues = reinterpret_cast<QueueT*>(tile_lds + max_side * max_side);
   const XcdRange r = xcd_range(total, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
   for (uint32_t item = r.start; item < r.end; item += r.stride) {
     const BatchFrame& f = frames[batch_find<&BatchFrame::first_tile>(frames, n, item)];
-    splat_tile(tile_job(f), sm, (int)(item - f.first_tile));
+    splat_tile(tile_job(f), tile_lds, queues, (int)(item - f.first_tile));
     __syncthreads();
   }
 }
@@ -509,7 +533,11 @@ hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint
     if (e != hipSuccess) return e;
   }
   const int n_tiles = n1d * n1d;
-  hipLaunchKernelGGL(ism_tiles_kernel, dim3(xcd_grid((uint32_t)n_tiles, 16)), dim3(256), 0, s, j, n_tiles);
+  const dim3 grid(xcd_grid((uint32_t)n_tiles, 16));
+  if (ism_side > 32)
+    hipLaunchKernelGGL(ism_tiles_kernel<uint16_t>, grid, dim3(256), tile_lds_bytes<uint16_t>(ism_side), s, j, n_tiles);
+  else
+    hipLaunchKernelGGL(ism_tiles_kernel<uint32_t>, grid, dim3(256), tile_lds_bytes<uint32_t>(ism_side), s, j, n_tiles);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !any) return e;
   hipLaunchKernelGGL(ism_spill_kernel, dim3(n_bins < 2048 ? n_bins : 2048), dim3(256), 0, s, j, n_bins);
@@ -522,9 +550,16 @@ hipError_t launch_batch_mask(hipStream_t s, const BatchFrame* frames, int n, uin
   return hipGetLastError();
 }
 
-hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
+hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total, int max_side) {
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_tiles_kernel, dim3(xcd_grid(total, 16)), dim3(256), 0, s, frames, n, total);
+  if (max_side < 4 || max_side > kTileMax) return hipErrorInvalidValue;
+  const dim3 grid(xcd_grid(total, 16));
+  if (max_side > 32)
+    hipLaunchKernelGGL(batch_tiles_kernel<uint16_t>, grid, dim3(256), tile_lds_bytes<uint16_t>(max_side), s, frames, n,
+                       total, max_side);
+  else
+    hipLaunchKernelGGL(batch_tiles_kernel<uint32_t>, grid, dim3(256), tile_lds_bytes<uint32_t>(max_side), s, frames, n,
+                       total, max_side);
   return hipGetLastError();
 }
 

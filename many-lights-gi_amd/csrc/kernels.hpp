@@ -170,7 +170,8 @@ BatchCounts batch_counts(const BatchFrame& f);
 hipError_t launch_batch_vpl(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
 // ISM-major splat of the batch: mask (bin items), tiles (ISM items), spill (bin items)
 hipError_t launch_batch_mask(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
-hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
+// max_side: the largest ISM side of the batch's frames (sizes the tile LDS)
+hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total, int max_side);
 hipError_t launch_batch_spill(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);
 void pullpush_empty_texel(int ism_side, float z_far, uint32_t out[4]);
 // frames[0..n) all have the given ISM side and level count
