@@ -54,6 +54,16 @@ __device__ __forceinline__ VplLds load_vpl(const float* rec) {
   return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
 }
 
+// A staged record in LDS, padded to 80 bytes: the staging writes and the
+// culling reads are per-lane 16-byte accesses at the slot stride, served
+// eight lanes per cycle, which are spread over all 32 banks only when the
+// stride is an odd multiple of 16 bytes (a 64-byte stride folds them onto a
+// quarter of the banks).
+struct alignas(16) VplSlot {
+  VplLds v;
+  float pad[4];
+};
+
 // Wave64 min / max, result wave-uniform. Floats are mapped to unsigned keys
 // of the same order, so each step is one unsigned min/max with a DPP operand
 // (no NaN canonicalisation). DPP steps: swap within pairs and quads, mirror
@@ -166,35 +176,41 @@ __device__ __forceinline__ bool wave_bounds(const Pixel& p, WaveBounds& wb) {
   return mnx <= mxx;
 }
 
-// Survivor mask of the `count` staged VPLs of the current chunk. A VPL with
-// zero flux (an RSM texel outside the spot cone; MIXED: zero after rounding
-// to halves, the values the sums multiply) is dropped as well: its every
-// contribution is flux * weight = +0 with a finite weight, which leaves the
-// sums' bits unchanged.
+// Culling test of one VPL against a wave's bounds: false when every pixel of
+// the wave gets angleFactor == 0 from it, or when its flux is zero (an RSM
+// texel outside the spot cone; MIXED: zero after rounding to halves, the
+// values the sums multiply): every contribution is then flux * weight = +0
+// with a finite weight, which leaves the sums' bits unchanged. `v` carries
+// the flux as floats.
 template <bool kMixed>
-__device__ __forceinline__ unsigned long long cull_chunk(const VplLds* tile, int count,
-                                                         const WaveBounds& w) {
+__device__ __forceinline__ bool cull_keep(const VplLds& v, const WaveBounds& w) {
+  const bool dark =
+      kMixed ? (__builtin_bit_cast(uint16_t, (_Float16)v.cr) | __builtin_bit_cast(uint16_t, (_Float16)v.cg) |
+                __builtin_bit_cast(uint16_t, (_Float16)v.cb)) == 0
+             : (v.cr == 0.0f && v.cg == 0.0f && v.cb == 0.0f);
+  // (1) max over the box of n . (p - p_vpl) = n . (c - p_vpl) + |n| . h
+  const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
+  const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
+  const float spread = ax * w.hx + ay * w.hy + az * w.hz;
+  const float support = v.nx * dx + v.ny * dy + v.nz * dz + spread;
+  const float mag = ax * fabsf(dx) + ay * fabsf(dy) + az * fabsf(dz) + spread;
+  const float tol = 1e-4f * mag + 1e-6f * (w.scale + fabsf(v.plane));
+  const bool behind_vpl = support < -tol;
+  // (2) min over pixels of n_p . (p - p_vpl) >= plane_lo - max_{n in box} n . p_vpl
+  const float mx = fmaxf(w.nlx * v.px, w.nhx * v.px) + fmaxf(w.nly * v.py, w.nhy * v.py) +
+                   fmaxf(w.nlz * v.pz, w.nhz * v.pz);
+  const float tol2 = 1e-6f * (w.scale + fabsf(v.px) + fabsf(v.py) + fabsf(v.pz));
+  const bool behind_pixels = w.plane_lo - mx > tol2;
+  return !(behind_vpl || behind_pixels || dark);
+}
+
+// Survivor mask of the `count` VPLs of a chunk, lane j testing VPL j, from
+// the chunk's records in global memory (flux as floats).
+template <bool kMixed>
+__device__ __forceinline__ unsigned long long cull_chunk(const float* crec, int count, const WaveBounds& w) {
   const int lane = threadIdx.x & 63;
   bool keep = false;
-  if (lane < count) {
-    const VplLds& v = tile[lane];
-    const bool dark = kMixed ? (__builtin_bit_cast(uint32_t, v.cr) | __builtin_bit_cast(uint32_t, v.cg)) == 0u
-                             : (v.cr == 0.0f && v.cg == 0.0f && v.cb == 0.0f);
-    // (1) max over the box of n . (p - p_vpl) = n . (c - p_vpl) + |n| . h
-    const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
-    const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
-    const float spread = ax * w.hx + ay * w.hy + az * w.hz;
-    const float support = v.nx * dx + v.ny * dy + v.nz * dz + spread;
-    const float mag = ax * fabsf(dx) + ay * fabsf(dy) + az * fabsf(dz) + spread;
-    const float tol = 1e-4f * mag + 1e-6f * (w.scale + fabsf(v.plane));
-    const bool behind_vpl = support < -tol;
-    // (2) min over pixels of n_p . (p - p_vpl) >= plane_lo - max_{n in box} n . p_vpl
-    const float mx = fmaxf(w.nlx * v.px, w.nhx * v.px) + fmaxf(w.nly * v.py, w.nhy * v.py) +
-                     fmaxf(w.nlz * v.pz, w.nhz * v.pz);
-    const float tol2 = 1e-6f * (w.scale + fabsf(v.px) + fabsf(v.py) + fabsf(v.pz));
-    const bool behind_pixels = w.plane_lo - mx > tol2;
-    keep = !(behind_vpl || behind_pixels || dark);
-  }
+  if (lane < count) keep = cull_keep<kMixed>(load_vpl(crec + (size_t)lane * kVplFloats), w);
   return __ballot(keep);
 }
 
@@ -371,6 +387,27 @@ __device__ __forceinline__ VplLds stage_vpl(const float* rec) {
   return v;
 }
 
+// Record of VPL vi (a wave-uniform index) read through the scalar data
+// cache into scalar registers, in the staged form of the precision (stage_vpl):
+// no LDS traffic, and the VALU takes the fields as scalar operands.
+template <bool kMixed>
+__device__ __forceinline__ VplLds uniform_vpl(const float* vpl, int vi) {
+  typedef float cf4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) cf4 ConstF4;
+  const int u = __builtin_amdgcn_readfirstlane(vi);
+  ConstF4* r = (ConstF4*)(vpl + (size_t)u * kVplFloats);
+  const cf4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
+  VplLds v{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
+  if (kMixed) {
+    const uint32_t hr = __builtin_bit_cast(uint16_t, (_Float16)v.cr);
+    const uint32_t hg = __builtin_bit_cast(uint16_t, (_Float16)v.cg);
+    const uint32_t hb = __builtin_bit_cast(uint16_t, (_Float16)v.cb);
+    v.cr = __builtin_bit_cast(float, hr | (hg << 16));
+    v.cg = __builtin_bit_cast(float, hb);
+  }
+  return v;
+}
+
 // Kahan step: total += x.
 __device__ __forceinline__ void kahan_add(float& total, float& comp, float x) {
   const float y = x - comp;
@@ -412,7 +449,7 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   WaveBounds wb;
   const bool wave_live = wave_bounds(p, wb);  // some pixel of this wave is on screen (wave-uniform)
 
-  __shared__ VplLds vtile[64];
+  __shared__ VplSlot vtile[64];
   const int first = k * a.per_class;
   const LitConsts lc = lit_consts(a);
   float ar = 0.0f, ag = 0.0f, ab = 0.0f;     // FP32 form: the sums; MIXED: this chunk's sums
@@ -424,20 +461,20 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
     const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
     // ---- stage the chunk's VPLs in LDS (one record per lane)
     if (chunk > 0) __syncthreads();
-    if ((int)threadIdx.x < count) vtile[threadIdx.x] = stage_vpl<kMixed>(crec + (size_t)threadIdx.x * kVplFloats);
+    if ((int)threadIdx.x < count) vtile[threadIdx.x].v = stage_vpl<kMixed>(crec + (size_t)threadIdx.x * kVplFloats);
     __syncthreads();
     if (!wave_live) continue;
-    unsigned long long mask = cull_chunk<kMixed>(vtile, count, wb);
+    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb);
     while (mask) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
       VplTest vt;
       VplLds vf;
       if (kFetch == kFetchLds) {
-        vf = vtile[j];
+        vf = vtile[j].v;
         vt = VplTest{vf.px, vf.py, vf.pz, vf.plane, vf.nx, vf.ny, vf.nz};
       } else {
-        const VplLds& t = vtile[j];
+        const VplLds& t = vtile[j].v;
         vt = VplTest{t.px, t.py, t.pz, t.plane, t.nx, t.ny, t.nz};
       }
       const int vi = first + chunk + j;
@@ -445,7 +482,7 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
       const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
       const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
       if (ndot > 0.0f && pdot < 0.0f) {
-        const VplLds v = (kFetch == kFetchLdsTest) ? vtile[j] : vf;
+        const VplLds v = (kFetch == kFetchLdsTest) ? vtile[j].v : vf;
         accumulate_pair<kMixed>(a, p, vt, v, vi, ndot, pdot, kShadow, lc, ar, ag, ab);
       }
     }
@@ -495,14 +532,24 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
 template <bool kMixed>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
                                                            uint32_t total) {
-  __shared__ VplLds vtiles[4][64];
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  VplLds* vtile = vtiles[wid];
+  // Items are dealt to the XCDs in chunks of kChunk consecutive items (the
+  // interleave classes of neighbouring strips, which read the same G-buffer
+  // lines), chunk c to XCD c mod 8, and each XCD's waves take its items in
+  // turn. Every XCD thus works all over the screen: strips of very different
+  // cost (lit or not) balance over the chip, where one contiguous eighth per
+  // XCD left the XCDs covering the lit region working alone at the end.
+  constexpr uint32_t kChunk = 64;
   const uint32_t slots = (gridDim.x / kBatchXcds) * 4;
   const uint32_t slot = (blockIdx.x / kBatchXcds) * 4 + wid;
-  const XcdRange xr = xcd_range(total, slot, slots);
-  for (uint32_t item = xr.start; item < xr.end; item += xr.stride) {
+  const uint32_t xcd = blockIdx.x % kBatchXcds;
+  const uint32_t n_chunks = (total + kChunk - 1) / kChunk;
+  for (uint32_t i = slot;; i += slots) {
+    const uint32_t chunk = (i / kChunk) * kBatchXcds + xcd;
+    if (chunk >= n_chunks) break;
+    const uint32_t item = chunk * kChunk + i % kChunk;
+    if (item >= total) continue;  // the tail of the last chunk
     const BatchFrame& f = frames[batch_find<&BatchFrame::first_wave>(frames, n, item)];
     const GatherArgs a = f.g;  // by value: the fields live in registers, not re-read per VPL
     const uint32_t local = item - f.first_wave;
@@ -525,19 +572,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void b
     for (int chunk = 0; chunk < a.per_class; chunk += 64) {
       const int count = min(64, a.per_class - chunk);
       const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-      // wave-local staging: the previous chunk's reads are done before the
-      // overwrite, and the new records are written before any lane reads them
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      if (lane < count) vtile[lane] = stage_vpl<kMixed>(crec + (size_t)lane * kVplFloats);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      unsigned long long mask = cull_chunk<kMixed>(vtile, count, wb);
+      unsigned long long mask = cull_chunk<kMixed>(crec, count, wb);
       while (mask) {
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1ull;
-        const VplLds v = vtile[j];
+        const VplLds v = uniform_vpl<kMixed>(a.vpl, first + chunk + j);
         const VplTest vt{v.px, v.py, v.pz, v.plane, v.nx, v.ny, v.nz};
         const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
         const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
