@@ -6,10 +6,11 @@
 // k = (x mod I) + (y mod I) * I and sums the VPLs [k*N/I^2, (k+1)*N/I^2).
 // A workgroup owns a TW x TH tile of one class's pixels (pixels I apart on
 // screen, like the reference's 8x8 workgroups); each wave64 of it owns 64 of
-// those pixels, so every lane of a wave walks the same VPL list. The class's
-// VPLs are staged in LDS 64 at a time, one record per lane of the workgroup;
-// the waves then read each record with uniform (broadcast) LDS reads, the next
-// one fetched while the current one is processed.
+// those pixels, so every lane of a wave walks the same VPL list, in chunks of
+// 64. Each surviving VPL's record is wave-uniform and read through the
+// scalar data cache into scalar registers (a broadcast LDS read returns the
+// record to all 64 lanes, 1 KiB of LDS bandwidth per 16 bytes, which bound
+// the gather at 4K).
 //
 // Culling (replaces the clustered light lists, light_lists.comp:104-119).
 // Each wave reduces the bounds of its pixels' positions, normals and plane
@@ -40,7 +41,7 @@ namespace {
 
 constexpr int kXcds = 8;
 
-// VPL fields the gather uses, as staged in LDS (64 bytes).
+// VPL fields the gather uses (64 bytes of the 128-byte record).
 struct VplLds {
   float px, py, pz, plane;
   float nx, ny, nz, sx;
@@ -53,16 +54,6 @@ __device__ __forceinline__ VplLds load_vpl(const float* rec) {
   const float4 a = gload4(r), b = gload4(r + 1), c = gload4(r + 2), d = gload4(r + 3), e = gload4(r + 4);
   return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
 }
-
-// A staged record in LDS, padded to 80 bytes: the staging writes and the
-// culling reads are per-lane 16-byte accesses at the slot stride, served
-// eight lanes per cycle, which are spread over all 32 banks only when the
-// stride is an odd multiple of 16 bytes (a 64-byte stride folds them onto a
-// quarter of the banks).
-struct alignas(16) VplSlot {
-  VplLds v;
-  float pad[4];
-};
 
 // Wave64 min / max, result wave-uniform. Floats are mapped to unsigned keys
 // of the same order, so each step is one unsigned min/max with a DPP operand
@@ -372,24 +363,10 @@ __device__ __forceinline__ void accumulate_pair(const GatherArgs& a, const Pixel
   ab = __builtin_fmaf(fb, g, ab);
 }
 
-// A staged VPL record in the form the precision uses (MIXED: flux packed to
-// halves, round to nearest even).
-template <bool kMixed>
-__device__ __forceinline__ VplLds stage_vpl(const float* rec) {
-  VplLds v = load_vpl(rec);
-  if (kMixed) {
-    const uint32_t hr = __builtin_bit_cast(uint16_t, (_Float16)v.cr);
-    const uint32_t hg = __builtin_bit_cast(uint16_t, (_Float16)v.cg);
-    const uint32_t hb = __builtin_bit_cast(uint16_t, (_Float16)v.cb);
-    v.cr = __builtin_bit_cast(float, hr | (hg << 16));
-    v.cg = __builtin_bit_cast(float, hb);
-  }
-  return v;
-}
-
 // Record of VPL vi (a wave-uniform index) read through the scalar data
-// cache into scalar registers, in the staged form of the precision (stage_vpl):
-// no LDS traffic, and the VALU takes the fields as scalar operands.
+// cache into scalar registers, in the form the precision uses (MIXED: flux
+// rounded to halves, round to nearest even, (r, g) packed in cr and b in
+// cg). No LDS traffic, and the VALU takes the fields as scalar operands.
 template <bool kMixed>
 __device__ __forceinline__ VplLds uniform_vpl(const float* vpl, int vi) {
   typedef float cf4 __attribute__((ext_vector_type(4)));
@@ -408,6 +385,16 @@ __device__ __forceinline__ VplLds uniform_vpl(const float* vpl, int vi) {
   return v;
 }
 
+// The hemisphere-test fields of VPL vi (wave-uniform), as uniform_vpl.
+__device__ __forceinline__ VplTest uniform_test(const float* vpl, int vi) {
+  typedef float cf4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) cf4 ConstF4;
+  const int u = __builtin_amdgcn_readfirstlane(vi);
+  ConstF4* r = (ConstF4*)(vpl + (size_t)u * kVplFloats);
+  const cf4 a = r[0], b = r[1];
+  return VplTest{a.x, a.y, a.z, a.w, b.x, b.y, b.z};
+}
+
 // Kahan step: total += x.
 __device__ __forceinline__ void kahan_add(float& total, float& comp, float x) {
   const float y = x - comp;
@@ -416,11 +403,12 @@ __device__ __forceinline__ void kahan_add(float& total, float& comp, float x) {
   total = t;
 }
 
-// Record fetch variants (picked per shape by the autotuner):
-//   kFetchLds     full record read from LDS at the top of each iteration
-//   kFetchLdsTest hemisphere-test fields read per iteration, the rest only
-//                 when a lane is lit
-enum FetchVariant { kFetchLds = 0, kFetchLdsTest = 1 };
+// Record fetch variants of the single-frame kernel (picked per shape by the
+// autotuner); both read the wave-uniform record through scalar loads:
+//   kFetchWhole     the full record at the top of each iteration
+//   kFetchTestFirst the hemisphere-test fields per iteration, the rest only
+//                   when a lane is lit
+enum FetchVariant { kFetchWhole = 0, kFetchTestFirst = 1 };
 
 // Precision forms.
 //   FP32  the reference's arithmetic in FP32, NEAREST texel decisions that
@@ -449,7 +437,6 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   WaveBounds wb;
   const bool wave_live = wave_bounds(p, wb);  // some pixel of this wave is on screen (wave-uniform)
 
-  __shared__ VplSlot vtile[64];
   const int first = k * a.per_class;
   const LitConsts lc = lit_consts(a);
   float ar = 0.0f, ag = 0.0f, ab = 0.0f;     // FP32 form: the sums; MIXED: this chunk's sums
@@ -459,30 +446,25 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   for (int chunk = 0; chunk < a.per_class; chunk += 64) {
     const int count = min(64, a.per_class - chunk);
     const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-    // ---- stage the chunk's VPLs in LDS (one record per lane)
-    if (chunk > 0) __syncthreads();
-    if ((int)threadIdx.x < count) vtile[threadIdx.x].v = stage_vpl<kMixed>(crec + (size_t)threadIdx.x * kVplFloats);
-    __syncthreads();
     if (!wave_live) continue;
     unsigned long long mask = cull_chunk<kMixed>(crec, count, wb);
     while (mask) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
+      const int vi = first + chunk + j;
       VplTest vt;
       VplLds vf;
-      if (kFetch == kFetchLds) {
-        vf = vtile[j].v;
+      if (kFetch == kFetchWhole) {
+        vf = uniform_vpl<kMixed>(a.vpl, vi);
         vt = VplTest{vf.px, vf.py, vf.pz, vf.plane, vf.nx, vf.ny, vf.nz};
       } else {
-        const VplLds& t = vtile[j].v;
-        vt = VplTest{t.px, t.py, t.pz, t.plane, t.nx, t.ny, t.nz};
+        vt = uniform_test(a.vpl, vi);
       }
-      const int vi = first + chunk + j;
       // n_vpl . (p - p_vpl) and n_p . (p - p_vpl) from the plane offsets
       const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
       const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
       if (ndot > 0.0f && pdot < 0.0f) {
-        const VplLds v = (kFetch == kFetchLdsTest) ? vtile[j].v : vf;
+        const VplLds v = (kFetch == kFetchTestFirst) ? uniform_vpl<kMixed>(a.vpl, vi) : vf;
         accumulate_pair<kMixed>(a, p, vt, v, vi, ndot, pdot, kShadow, lc, ar, ag, ab);
       }
     }
@@ -508,11 +490,11 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
 
 // Batched gather: persistent wave64s over the (frame, class, strip) items of
 // all frames of one precision. A strip is 2^w x 64/2^w pixels of one
-// interleave class (w per frame, picked by the autotuner). Each wave stages
-// its own 64-record VPL chunks in LDS and synchronises only within itself, so
-// frames of different shapes share workgroups freely. The per-pixel
-// arithmetic and summation order are the single-frame kernel's, so a frame
-// gives the same bits alone or in any batch.
+// interleave class (w per frame, picked by the autotuner). Waves share no
+// memory and never synchronise, so frames of different shapes share
+// workgroups freely. The per-pixel arithmetic and summation order are the
+// single-frame kernel's, so a frame gives the same bits alone or in any
+// batch.
 //
 // (Compacting the lit pairs of a wave into an LDS queue, so that lanes spend
 // the projection and ISM fetch on lit pairs only, measured 4.4x slower at
@@ -652,10 +634,10 @@ hipError_t launch_tiled(hipStream_t s, const GatherArgs& a) {
 template <int TW, int TH>
 hipError_t launch_variant(hipStream_t s, const GatherArgs& a, int fetch, bool mixed) {
   switch (fetch * 2 + (mixed ? 1 : 0)) {
-    case kFetchLds * 2: return launch_tiled<TW, TH, kFetchLds, false>(s, a);
-    case kFetchLds * 2 + 1: return launch_tiled<TW, TH, kFetchLds, true>(s, a);
-    case kFetchLdsTest * 2: return launch_tiled<TW, TH, kFetchLdsTest, false>(s, a);
-    case kFetchLdsTest * 2 + 1: return launch_tiled<TW, TH, kFetchLdsTest, true>(s, a);
+    case kFetchWhole * 2: return launch_tiled<TW, TH, kFetchWhole, false>(s, a);
+    case kFetchWhole * 2 + 1: return launch_tiled<TW, TH, kFetchWhole, true>(s, a);
+    case kFetchTestFirst * 2: return launch_tiled<TW, TH, kFetchTestFirst, false>(s, a);
+    case kFetchTestFirst * 2 + 1: return launch_tiled<TW, TH, kFetchTestFirst, true>(s, a);
     default: return hipErrorInvalidValue;
   }
 }
