@@ -510,7 +510,10 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
 // the hemisphere tests, then per-lane record reads from 17-word LDS slots),
 // so that a wave runs as many lit passes as its busiest lane, measured 9%
 // slower at 1080p and 7% slower at 4K: per-lane record reads and tile
-// addressing cost more than the passes it saves.)
+// addressing cost more than the passes it saves. Requesting the first
+// chunk's culling records before the pixel loads, to overlap the two
+// latencies, measured 9% slower at 4K and 15% slower for FP32: the records
+// stay live across the pixel reconstruction and the bound reductions.)
 template <bool kMixed>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
                                                            uint32_t total) {
