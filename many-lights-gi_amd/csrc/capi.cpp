@@ -843,8 +843,8 @@ int batch_plan(mlgi_context* c, const mlgi_frame_desc* frames, int count, std::v
     const int prec = k >> 16;
     if (r == 0 || key[order[r - 1]] != k) plan->pullpush.push_back({r, 0, 0, f.ism_side, f.levels});
     auto& pp = plan->pullpush.back();
-    f.first_ism = pp.total;
-    pp.total += n.isms;
+    f.first_pp = pp.total;
+    pp.total += n.pp;
     pp.count++;
     if (r == 0 || (key[order[r - 1]] >> 16) != prec) plan->gather.push_back({r, 0, 0, prec, 0});
     auto& fg = plan->gather.back();
@@ -914,7 +914,7 @@ mlgi::BatchCounts mlgi::batch_counts(const mlgi::BatchFrame& f) {
   n.vpl = (uint32_t)f.g.n_vpl;
   n.tiles = (uint32_t)(f.n1d * f.n1d);
   n.bins = f.n_points > 0 ? (uint32_t)f.g.n_vpl : 0u;
-  n.isms = (uint32_t)(f.n1d * f.n1d);
+  n.pp = (uint32_t)(f.n1d * f.n1d / mlgi::pullpush_group(f.ism_side, f.n1d * f.n1d));
   const int th = 64 >> f.wave_w_log2;
   const int ch = (f.g.height + f.g.interleave - 1) / f.g.interleave;
   const uint32_t strips_y = (uint32_t)((ch + th - 1) / th);

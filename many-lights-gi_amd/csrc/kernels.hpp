@@ -158,13 +158,20 @@ struct BatchFrame {
   int wave_w_log2;  // gather wave strip: 2^wave_w_log2 x 64 / 2^wave_w_log2 class pixels
   int strips_x;     // strips per class row
   // first work item per stage
-  uint32_t first_vpl, first_tile, first_bin, first_ism, first_wave;
+  uint32_t first_vpl, first_tile, first_bin, first_pp, first_wave;  // first_pp: pull-push ISM groups
 };
 
 // Work-item counts of a frame, in each stage's unit.
 struct BatchCounts {
-  uint32_t vpl, tiles, bins, isms, waves;
+  uint32_t vpl, tiles, bins, pp, waves;  // pp: pull-push items (ISM groups, pullpush_group)
 };
+
+// ISMs of side S that one pull-push workgroup takes: as many as make 64^2
+// texels, at most the frame's n_isms (both powers of four, so it divides).
+__host__ __device__ inline int pullpush_group(int ism_side, int n_isms) {
+  const int g = (64 / ism_side) * (64 / ism_side);
+  return g < n_isms ? g : n_isms;
+}
 BatchCounts batch_counts(const BatchFrame& f);
 
 hipError_t launch_batch_vpl(hipStream_t s, const BatchFrame* frames, int n, uint32_t total);

@@ -455,37 +455,59 @@ constexpr int level_offset(int l) {  // texels of pull levels 1..l-1
   return o;
 }
 
+// ISMs per workgroup: as many S x S ISMs as make 64^2 texels (1 at S = 64,
+// 4 at 32, 16 at 16, 64 at 8), so that the small levels of several ISMs share
+// wave iterations (a level of one output costs a full wave iteration per
+// workgroup). At most the frame's ISM count; both are powers of four, so the
+// group divides it.
+template <int S>
+constexpr int kGroupMax = (64 / S) * (64 / S);
+
+// One workgroup's group of ISMs ism0 .. ism0 + groups - 1: tile g at
+// base + g * S * S, its pull levels at lv + g * lv_stride.
 struct Tables {
-  const uint32_t* base;
+  uint32_t* base;
   Texel* lv;
+  int lv_stride;
   const float* rtab;
   const float* mtab;
   const float* rcp;
   Decoded empty;
   float z_far;
-  int ism_x, ism_y;
+  int ism0, n1d_log2, groups;
 };
+
+// The group size, a compile-time 1 where a group holds one ISM (S = 64), so
+// that the group arithmetic folds away there.
+template <int S>
+__device__ __forceinline__ int group_count(int groups) {
+  return kGroupMax<S> == 1 ? 1 : groups;
+}
 
 template <int S, int l, int L>
 __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& pc) {
   if constexpr (l <= L) {
     constexpr int so = S >> l, n = so * so;
-    Texel* dst = tb.lv + level_offset<S>(l);
-    const Texel* src = tb.lv + level_offset<S>(l - 1);
-    for (int t = threadIdx.x; t < n; t += kThreads) {
-      const int ox = t % so, oy = t / so;
+    for (int t = threadIdx.x; t < group_count<S>(tb.groups) * n; t += kThreads) {
+      const int g = t / n, o = t - g * n;
+      const int ox = o % so, oy = o / so;
+      const uint32_t* base = tb.base + g * (S * S);
+      Texel* lv = tb.lv + g * tb.lv_stride;
+      Texel* dst = lv + level_offset<S>(l);
+      const int ism = tb.ism0 + g;
+      const int ism_x = ism & ((1 << tb.n1d_log2) - 1), ism_y = ism >> tb.n1d_log2;
       if constexpr (l == 1) {
         // a wave whose 2x2 blocks are all empty writes E_1
-        const uint2 a = *reinterpret_cast<const uint2*>(&tb.base[(2 * oy) * S + 2 * ox]);
-        const uint2 b = *reinterpret_cast<const uint2*>(&tb.base[(2 * oy + 1) * S + 2 * ox]);
+        const uint2 a = *reinterpret_cast<const uint2*>(&base[(2 * oy) * S + 2 * ox]);
+        const uint2 b = *reinterpret_cast<const uint2*>(&base[(2 * oy + 1) * S + 2 * ox]);
         const bool all_empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
         if (__ballot(!all_empty) == 0ull) {
-          dst[t] = pc.e1;
+          dst[o] = pc.e1;
           continue;
         }
       }
-      dst[t] = pull_one<S, l>(ox, oy, tb.ism_x, tb.ism_y, tb.base, src, tb.rtab, tb.mtab, tb.empty,
-                              tb.z_far);
+      dst[o] = pull_one<S, l>(ox, oy, ism_x, ism_y, base, lv + level_offset<S>(l - 1), tb.rtab, tb.mtab,
+                              tb.empty, tb.z_far);
     }
     __syncthreads();
     pull_levels<S, l + 1, L>(tb, pc);
@@ -493,41 +515,49 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
 }
 
 template <int S, int l>
-__device__ __forceinline__ void push_levels(const Tables& tb, uint32_t* base) {
+__device__ __forceinline__ void push_levels(const Tables& tb) {
   if constexpr (l >= 0) {
-    constexpr int so = S >> l, nq1 = so / 2 + 1;
-    const Texel* coarse = tb.lv + level_offset<S>(l + 1);
-    Texel* same = tb.lv + level_offset<S>(l);
-    for (int q = threadIdx.x; q < nq1 * nq1; q += kThreads) {
+    constexpr int so = S >> l, nq1 = so / 2 + 1, nq = nq1 * nq1;
+    for (int t = threadIdx.x; t < group_count<S>(tb.groups) * nq; t += kThreads) {
+      const int g = t / nq, q = t - g * nq;
+      Texel* lv = tb.lv + g * tb.lv_stride;
+      const Texel* coarse = lv + level_offset<S>(l + 1);
       if constexpr (l == 0)
-        push_quad_depth<S>(q % nq1, q / nq1, coarse, base, tb.rcp);
+        push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.base + g * (S * S), tb.rcp);
       else
-        push_quad<S, l>(q % nq1, q / nq1, coarse, same, tb.rcp);
+        push_quad<S, l>(q % nq1, q / nq1, coarse, lv + level_offset<S>(l), tb.rcp);
     }
     __syncthreads();
-    push_levels<S, l - 1>(tb, base);
+    push_levels<S, l - 1>(tb);
   }
 }
 
-// LDS of one ISM's pull-push.
+// LDS of one workgroup's pull-push: kGroupMax<S> tiles (4096 texels) and
+// their pull levels.
 template <int S, int L>
 struct TileLds {
-  uint32_t base[S * S];
-  Texel lv[level_offset<S>(L + 1)];
+  static constexpr int kLv = level_offset<S>(L + 1);
+  uint32_t base[kGroupMax<S> * S * S];
+  Texel lv[kGroupMax<S> * kLv];
   float rtab[256], mtab[256], rcp[17];
 };
 
-// Pull-push of ISM `ism` of an atlas, by the whole workgroup; ends with the
-// unorm16 tile written (the caller synchronises before reusing the LDS).
+// Pull-push of ISMs ism0 .. ism0 + groups - 1 of an atlas, by the whole
+// workgroup; ends with the unorm16 tiles written (the caller synchronises
+// before reusing the LDS).
 template <int S, int L>
 __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t* __restrict__ atlas,
                                               int atlas_side, float z_far, const PullConsts& pc,
-                                              uint16_t* __restrict__ out, int ism) {
+                                              uint16_t* __restrict__ out, int ism0, int groups_) {
   const int tid = threadIdx.x;
+  const int groups = group_count<S>(groups_);
   const int n1d = atlas_side / S;
-  const int ism_x = ism % n1d;
-  const int ism_y = ism / n1d;
-  const size_t a0 = (size_t)ism_y * S * atlas_side + (size_t)ism_x * S;
+  const int n1d_log2 = 31 - __builtin_clz(n1d);
+  constexpr int n4 = S * S / 4;  // 16-byte words per tile
+  auto origin = [&](int g) {
+    const int ism = ism0 + g;
+    return (size_t)(ism >> n1d_log2) * S * atlas_side + (size_t)(ism & (n1d - 1)) * S;
+  };
 
   // ---- radius-byte tables (same expressions as the per-texel decode)
   for (int b = tid; b < 256; b += kThreads) {
@@ -536,30 +566,36 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
     sm.mtab[b] = r * 2.0f / z_far * 0.6f;
   }
   if (tid < 17) sm.rcp[tid] = kRcp[tid];
-  // ---- load the ISM's atlas tile (S*S u32) with 16-byte loads
-  for (int i = tid; i < S * S / 4; i += kThreads) {
-    const int row = (i * 4) / S, col = (i * 4) % S;
-    const uint4 v = gload4(reinterpret_cast<const uint4*>(atlas + a0 + (size_t)row * atlas_side + col));
-    *reinterpret_cast<uint4*>(&sm.base[row * S + col]) = v;
+  // ---- load the group's atlas tiles (S*S u32 each) with 16-byte loads
+  for (int i = tid; i < groups * n4; i += kThreads) {
+    const int g = i / n4, w = i - g * n4;
+    const int row = (w * 4) / S, col = (w * 4) % S;
+    const uint4 v = gload4(reinterpret_cast<const uint4*>(atlas + origin(g) + (size_t)row * atlas_side + col));
+    *reinterpret_cast<uint4*>(&sm.base[g * S * S + row * S + col]) = v;
   }
   __syncthreads();
-  Tables tb{sm.base, sm.lv, sm.rtab, sm.mtab, sm.rcp, {}, z_far, ism_x, ism_y};
+  Tables tb{sm.base, sm.lv, TileLds<S, L>::kLv, sm.rtab, sm.mtab, sm.rcp, {}, z_far, ism0, n1d_log2, groups};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
   tb.empty = decode_texel(0xFFFFFFFFu, sm.rtab[255], sm.mtab[255], z_far, S);
 
   pull_levels<S, 1, L>(tb, pc);
   if constexpr (L == 1) {
     // no level-1 push stores the bounds the level-0 push reads
-    for (int t = tid; t < (S / 2) * (S / 2); t += kThreads) sm.lv[t].radius = sqrt_gt_bound_f(sm.lv[t].radius);
+    constexpr int n1 = (S / 2) * (S / 2);
+    for (int t = tid; t < groups * n1; t += kThreads) {
+      Texel& x = sm.lv[(t / n1) * TileLds<S, L>::kLv + t % n1];
+      x.radius = sqrt_gt_bound_f(x.radius);
+    }
     __syncthreads();
   }
-  push_levels<S, L - 1>(tb, sm.base);
+  push_levels<S, L - 1>(tb);
 
-  // ---- write the unorm16 tile (held in the low halves of the tile slots)
-  for (int i = tid; i < S * S / 4; i += kThreads) {
-    const int row = (i * 4) / S, col = (i * 4) % S;
-    const uint4 v = *reinterpret_cast<const uint4*>(&sm.base[row * S + col]);
-    gstore(reinterpret_cast<uint64_t*>(out + a0 + (size_t)row * atlas_side + col),
+  // ---- write the unorm16 tiles (held in the low halves of the tile slots)
+  for (int i = tid; i < groups * n4; i += kThreads) {
+    const int g = i / n4, w = i - g * n4;
+    const int row = (w * 4) / S, col = (w * 4) % S;
+    const uint4 v = *reinterpret_cast<const uint4*>(&sm.base[g * S * S + row * S + col]);
+    gstore(reinterpret_cast<uint64_t*>(out + origin(g) + (size_t)row * atlas_side + col),
            (uint64_t)(v.x | (v.y << 16)) | ((uint64_t)(v.z | (v.w << 16)) << 32));
   }
 }
@@ -567,28 +603,29 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
 template <int S, int L>
 __global__ __launch_bounds__(kThreads) void pullpush_kernel(const uint32_t* __restrict__ atlas,
                                                             int atlas_side, float z_far, PullConsts pc,
-                                                            uint16_t* __restrict__ out) {
+                                                            uint16_t* __restrict__ out, int groups) {
   __shared__ TileLds<S, L> sm;
-  pullpush_tile<S, L>(sm, atlas, atlas_side, z_far, pc, out, blockIdx.x);
+  pullpush_tile<S, L>(sm, atlas, atlas_side, z_far, pc, out, blockIdx.x * groups, groups);
 }
 
-// Batched: one workgroup per (frame, ISM) item of the frames that share the
-// ISM side S and the level count L. ISMs are independent, so no persistent
-// loop is needed: the hardware keeps as many workgroups resident as LDS and
-// registers allow. (A persistent loop over items doubled the register count,
-// 105 against 57 at S = 32, halving the occupancy of small ISMs.)
+// Batched: one workgroup per (frame, ISM group) item of the frames that share
+// the ISM side S and the level count L. ISMs are independent, so no
+// persistent loop is needed: the hardware keeps as many workgroups resident
+// as LDS and registers allow. (A persistent loop over items doubled the
+// register count, 105 against 57 at S = 32.)
 template <int S, int L>
 __global__ __launch_bounds__(kThreads) void batch_pullpush_kernel(const BatchFrame* __restrict__ frames,
                                                                   int n, uint32_t total) {
   __shared__ TileLds<S, L> sm;
   const uint32_t item = blockIdx.x;
   if (item >= total) return;
-  const BatchFrame& f = frames[batch_find<&BatchFrame::first_ism>(frames, n, item)];
+  const BatchFrame& f = frames[batch_find<&BatchFrame::first_pp>(frames, n, item)];
   PullConsts pc;
   pc.e1 = Texel{__uint_as_float(f.pp_e1[0]), __uint_as_float(f.pp_e1[1]), __uint_as_float(f.pp_e1[2]),
                 f.pp_e1[3]};
+  const int groups = pullpush_group(S, f.n1d * f.n1d);
   pullpush_tile<S, L>(sm, f.atlas, f.atlas_side, f.z_far, pc, const_cast<uint16_t*>(f.g.ism),
-                      (int)(item - f.first_ism));
+                      (int)(item - f.first_pp) * groups, groups);
 }
 
 // Host-side constant: pull level 1 of an all-empty 2x2 block, computed with
@@ -616,8 +653,9 @@ template <int S, int L>
 hipError_t launch_sl(hipStream_t s, const uint32_t* atlas, int atlas_side, float z_far,
                      const PullConsts& pc, uint16_t* out) {
   const int n1d = atlas_side / S;
-  hipLaunchKernelGGL((pullpush_kernel<S, L>), dim3(n1d * n1d), dim3(kThreads), 0, s, atlas, atlas_side,
-                     z_far, pc, out);
+  const int groups = pullpush_group(S, n1d * n1d);
+  hipLaunchKernelGGL((pullpush_kernel<S, L>), dim3(n1d * n1d / groups), dim3(kThreads), 0, s, atlas, atlas_side,
+                     z_far, pc, out, groups);
   return hipGetLastError();
 }
 
