@@ -92,7 +92,7 @@ class Context:
             raise MlgiError(rc, f"{what}: {msg.decode() if msg else ''}")
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):  # None also when __init__ raised
             for ptr in list(self._live):
                 lib.mlgi_free(self.h, C.c_void_p(ptr))
             self._live.clear()
@@ -351,7 +351,7 @@ class DeviceScene:
         return inv
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):  # None also when __init__ raised
             lib.mlgi_scene_device_destroy(self.h)
             self.h = None
 
@@ -571,7 +571,7 @@ class Batch:
         return FrameGraph(self.ctx, g)
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):  # None also when __init__ raised
             lib.mlgi_batch_destroy(self.h)
             self.h = None
 
@@ -637,7 +637,7 @@ class Server:
         return stats, lat
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):  # None also when __init__ raised
             lib.mlgi_server_destroy(self.h)
             self.h = None
 
@@ -655,7 +655,7 @@ class FrameGraph:
         self.ctx.check(lib.mlgi_graph_launch(self.ctx.h, self.h), "mlgi_graph_launch")
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):  # None also when __init__ raised
             lib.mlgi_graph_destroy(self.h)
             self.h = None
 

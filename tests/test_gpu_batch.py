@@ -94,6 +94,31 @@ def test_batch_matches_single_frames(ctx, frames):
     b.close()
 
 
+# ISM side 16, three levels, 4 and 256 ISMs: one pull-push launch whose
+# workgroups take 4 ISMs in the first frame and 16 in the second.
+GROUP_SHAPES = [
+    (48, 32, 4, 2, 32, 8, mlgi.PRECISION_MIXED, 23),
+    (64, 48, 256, 4, 256, 8, mlgi.PRECISION_MIXED, 29),
+]
+
+
+def test_batch_pullpush_groups_of_different_sizes(ctx):
+    fr = [make_frame(ctx, s) for s in GROUP_SHAPES]
+    assert [f[0].ism_side for f in fr] == [16, 16]
+    ref = single_path_outputs(ctx, fr)
+    for wl, inp, st, rsm, pts, gb in fr:
+        clobber(st, wl)
+    b = mlgi.Batch(ctx, [(st, rsm, pts, gb) for wl, inp, st, rsm, pts, gb in fr])
+    b.run()
+    ctx.sync()
+    for i, (wl, inp, st, rsm, pts, gb) in enumerate(fr):
+        got = outputs(st)
+        for name, x, y in zip(("vpl", "atlas", "ism"), got[:3], ref[i][:3]):
+            assert np.array_equal(x, y), (i, name, GROUP_SHAPES[i])
+        assert np.array_equal(got[2], oracle.pullpush(got[1], 16, 3, 50.0)), i
+    b.close()
+
+
 def test_batch_of_one_and_graph_replays(ctx, frames):
     big = mlgi.Batch(ctx, [(st, rsm, pts, gb) for wl, inp, st, rsm, pts, gb in frames])
     big.run()
