@@ -8,19 +8,21 @@ on inputs already resident in HBM -- VPL extraction from the RSM, ISM atlas
 clear + point splat, pull-push, interleaved final gathering -- at 1920x1080,
 4,096 VPLs (128^2 RSM), 1M splat points into 4,096 64^2 ISMs of a 4096^2
 atlas, 4 pull-push levels, 8x8 interleave, 40k-triangle procedural room.
-Three forms run on the same inputs:
+Four forms run on the same inputs:
   fp32          the plain FP32 form through the single-frame entry point
-                (mlgi_frame)
+                (mlgi_frame, GIStage::process's path)
+  mixed_single  the mixed-precision form through the same entry point
   fp32_batched  the FP32 form through the persistent batched kernels
                 (mlgi_batch_run of the one frame)
   mixed         the mixed-precision form through the batched kernels
 The timed region is exactly K frames of a form bracketed by a device
 synchronize on both sides; ms_per_step is wall time / K; HIP events on the
 context stream bracket every stage of every timed frame. `value` is the
-mixed form's ms/frame when it runs. Batched forms are also replayed from a
-captured HIP graph (graph_ms_per_frame). mixed_speedup_vs_fp32 compares
-mixed with the plain fp32 path; mixed_speedup_same_path isolates precision
-(fp32_batched / mixed).
+mixed form through the single-frame entry point (the drop-in's path for one
+frame); with --form mixed, the batched mixed form. Batched forms are also
+replayed from a captured HIP graph (graph_ms_per_frame). Precision is
+compared within a path: mixed_speedup_vs_fp32 = fp32 / mixed_single,
+mixed_speedup_same_path = fp32_batched / mixed (and its graph replays).
 
 roofline: for the kernel with the largest mean duration in the timed frames:
 its VALU issue rate ("valu": SQ_INSTS_VALU per launch from the committed
@@ -85,14 +87,14 @@ def upload_frame(ctx, inp):
 
 
 def run_form(ctx, wl, inp, form: str, steps: int, warmup: int):
-    """fp32: plain single-frame path; fp32_batched / mixed: batched path
-    (batch of one) in FP32 / mixed precision."""
+    """fp32 / mixed_single: single-frame path in FP32 / mixed precision;
+    fp32_batched / mixed: batched path (batch of one)."""
     import mlgi
 
-    prec = mlgi.PRECISION_MIXED if form == "mixed" else mlgi.PRECISION_FP32
+    prec = mlgi.PRECISION_MIXED if form in ("mixed", "mixed_single") else mlgi.PRECISION_FP32
     st = mlgi.GIStage(ctx, wl.params(precision=prec), wl.n_vpl, wl.width, wl.height)
     rsm, pts, gb = upload_frame(ctx, inp)
-    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb)]) if form != "fp32" else None
+    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb)]) if form in ("fp32_batched", "mixed") else None
     step = batch.run if batch else (lambda: st.process(rsm, pts, gb))
     for _ in range(warmup):
         step()
@@ -137,8 +139,11 @@ def valu_roof(kernel_ms, sq, peak_fma_s):
             "peak_source": "measured FMA probe" if peak_fma_s else "spec"}
 
 
-def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None):
+def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None, single=False):
+    """Per-kernel rooflines of one form's timed frames; `single` selects the
+    single-frame kernels' counters (profile keys with a _single suffix)."""
     sq = sq or {}
+    sfx = "_single" if single else ""
     a2 = wl.atlas_side * wl.atlas_side
     out = {}
     t = stages["pullpush_ms"] * 1e-3
@@ -147,7 +152,7 @@ def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None):
         out["pullpush"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": 6 * a2,
                            "kernel_ms": round(stages["pullpush_ms"], 4),
-                           "valu": valu_roof(stages["pullpush_ms"], sq.get("pullpush"), peak_fma_s)}
+                           "valu": valu_roof(stages["pullpush_ms"], sq.get("pullpush" + sfx), peak_fma_s)}
     t = stages["ism_splat_ms"] * 1e-3
     if t > 0:
         # ISM-major splat: point records read once, the atlas written once
@@ -164,7 +169,7 @@ def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None):
                                "unit": "TFLOP/s", "frac": round(tf / FP32_VECTOR_PEAK_TFLOPS, 4),
                                "lit_pairs": lit, "flops_per_lit_pair": FLOPS_PER_LIT_PAIR,
                                "kernel_ms": round(stages["final_gather_ms"], 4),
-                               "valu": valu_roof(stages["final_gather_ms"], sq.get("final_gather"), peak_fma_s)}
+                               "valu": valu_roof(stages["final_gather_ms"], sq.get("final_gather" + sfx), peak_fma_s)}
     return out
 
 
@@ -212,28 +217,29 @@ def frame_bench(a, ctx, wl):
     t0 = time.perf_counter()
     inp = W.build_inputs(wl)
     log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s")
-    forms = {"fp32": ["fp32"], "mixed": ["mixed"], "both": ["fp32", "fp32_batched", "mixed"]}[a.form]
+    forms = {"fp32": ["fp32"], "mixed": ["mixed"], "both": ["fp32", "mixed_single", "fp32_batched", "mixed"]}[a.form]
     res = {}
     for form in forms:
         log(f"[bench] form {form}: warmup {a.warmup}, steps {a.steps}")
         res[form] = run_form(ctx, wl, inp, form, a.steps, a.warmup)
         log(f"[bench] {form}: {res[form]['wall_ms']:.4f} ms/frame, stages {res[form]['stages']}")
-    head = "mixed" if "mixed" in res else "fp32"
+    head = next(f for f in ("mixed_single", "mixed", "fp32") if f in res)
     h = res[head]
+    single = head in ("mixed_single", "fp32")
     peak_fma_s = ctx.probe_valu()
     log(f"[bench] FP32 FMA probe: {peak_fma_s * 2 / 1e12:.1f} TFLOP/s")
     sq = None
     sq_path = ROOT / "profiles" / f"sq_counters_{wl.name}.json"
     if sq_path.exists():
         sq = {k: v for k, v in json.loads(sq_path.read_text()).items() if not k.startswith("_")}
-    kernels = kernel_rooflines(wl, h["stages"], h["lit"], sq, peak_fma_s)
+    kernels = kernel_rooflines(wl, h["stages"], h["lit"], sq, peak_fma_s, single)
     dominant = max(kernels, key=lambda k: kernels[k]["kernel_ms"])
     roof = dict(kernels[dominant])
     roof["kernel"] = dominant
     roof["traffic"] = None
     tpath = ROOT / "profiles" / f"traffic_{wl.name}.json"
     if tpath.exists():
-        tr = json.loads(tpath.read_text()).get(dominant)
+        tr = json.loads(tpath.read_text()).get(dominant + ("_single" if single else ""))
         if tr:
             roof["traffic"] = tr["bytes_per_launch"]
     forms_out = {}
@@ -253,7 +259,7 @@ def frame_bench(a, ctx, wl):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16x2+fp32" if head == "mixed" else "fp32",
+        "dtype": "fp16x2+fp32" if head.startswith("mixed") else "fp32",
         "data": DATA,
         "config": config_of(wl, head),
         "forms": forms_out,
@@ -261,9 +267,9 @@ def frame_bench(a, ctx, wl):
         "kernels": kernels,
     }
     out["fp32_fma_probe_tflops"] = round(peak_fma_s * 2 / 1e12, 2)
-    if "fp32" in res and "mixed" in res:
-        # path and precision together: single-frame FP32 vs batched mixed
-        out["mixed_speedup_vs_fp32"] = round(res["fp32"]["wall_ms"] / res["mixed"]["wall_ms"], 4)
+    if "fp32" in res and "mixed_single" in res:
+        # precision on the single-frame path
+        out["mixed_speedup_vs_fp32"] = round(res["fp32"]["wall_ms"] / res["mixed_single"]["wall_ms"], 4)
     if "fp32_batched" in res and "mixed" in res:
         # precision alone: both through the batched kernels
         out["mixed_speedup_same_path"] = round(res["fp32_batched"]["wall_ms"] / res["mixed"]["wall_ms"], 4)
