@@ -248,6 +248,7 @@ struct VplTest {
 };
 
 struct LitConsts {
+  __amdgpu_buffer_rsrc_t ism;  // the unorm16 ISM atlas as a raw buffer
   float hs;   // ((p + 1) / 2) * S = p * S/2 + S/2
   int amask;  // atlas side - 1
   int alog;   // log2(atlas side)
@@ -255,8 +256,13 @@ struct LitConsts {
   float k2;   // smoothstep slope / 65535
 };
 
+// The ISM is fetched through a raw buffer resource (base and range in scalar
+// registers, a 32-bit byte offset per lane; texel < 2^28), which saves the
+// 64-bit address arithmetic of a global load per lit pair. Resource word 3
+// 0x00020000 selects the 32-bit data format raw buffer loads use on CDNA.
 __device__ __forceinline__ LitConsts lit_consts(const GatherArgs& a) {
-  return LitConsts{0.5f * (float)a.ism_side, a.atlas_side - 1, 31 - __builtin_clz(a.atlas_side),
+  return LitConsts{__builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(a.ism), 0, 0x7FFFFFFF, 0x00020000),
+                   0.5f * (float)a.ism_side, a.atlas_side - 1, 31 - __builtin_clz(a.atlas_side),
                    a.shadow_k1, a.shadow_k2};
 }
 
@@ -329,7 +335,7 @@ __device__ __forceinline__ float pair_weight(const GatherArgs& a, float px, floa
                                              float pdot, bool shadow, const LitConsts& lc) {
   const PairSetup s = pair_setup<kMixed>(a, px, py, pz, x, y, depth, vt, v, vi, ndot, pdot, shadow, lc);
   if (!shadow) return s.g;
-  return pair_finish(s, gload(a.ism + s.texel), lc);
+  return pair_finish(s, __builtin_amdgcn_raw_buffer_load_b16(lc.ism, s.texel << 1, 0, 0), lc);
 }
 
 // The VPL's flux as floats (MIXED: packed halves, (r, g) in cr, (b, 0) in cg).
@@ -488,6 +494,40 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
   }
 }
 
+// Sums of one batched item's pixel over its class's VPLs, with the
+// single-frame kernel's arithmetic and order (MIXED: per-chunk sums combined
+// with Kahan compensation).
+template <bool kMixed, bool kShadow>
+__device__ __forceinline__ void batch_item_sums(const GatherArgs& a, const Pixel& p, const WaveBounds& wb,
+                                                int first, const LitConsts& lc, float& sr, float& sg, float& sb) {
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+  float tr = 0.0f, tg = 0.0f, tb = 0.0f, er = 0.0f, eg = 0.0f, eb = 0.0f;
+  for (int chunk = 0; chunk < a.per_class; chunk += 64) {
+    const int count = min(64, a.per_class - chunk);
+    const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
+    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb);
+    while (mask) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      const VplLds v = uniform_vpl<kMixed>(a.vpl, first + chunk + j);
+      const VplTest vt{v.px, v.py, v.pz, v.plane, v.nx, v.ny, v.nz};
+      const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
+      const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
+      if (ndot > 0.0f && pdot < 0.0f)
+        accumulate_pair<kMixed>(a, p, vt, v, first + chunk + j, ndot, pdot, kShadow, lc, ar, ag, ab);
+    }
+    if (kMixed) {
+      kahan_add(tr, er, ar);
+      kahan_add(tg, eg, ag);
+      kahan_add(tb, eb, ab);
+      ar = ag = ab = 0.0f;
+    }
+  }
+  sr = kMixed ? tr : ar;
+  sg = kMixed ? tg : ag;
+  sb = kMixed ? tb : ab;
+}
+
 // Batched gather: persistent wave64s over the (frame, class, strip) items of
 // all frames of one precision. A strip is 2^w x 64/2^w pixels of one
 // interleave class (w per frame, picked by the autotuner). Waves share no
@@ -515,26 +555,22 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
 // latencies, measured 9% slower at 4K and 15% slower for FP32: the records
 // stay live across the pixel reconstruction and the bound reductions.)
 template <bool kMixed>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
                                                            uint32_t total) {
-  const int lane = threadIdx.x & 63;
-  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // Items are dealt to the XCDs in chunks of kChunk consecutive items (the
-  // interleave classes of neighbouring strips, which read the same G-buffer
-  // lines), chunk c to XCD c mod 8, and each XCD's waves take its items in
-  // turn. Every XCD thus works all over the screen: strips of very different
-  // cost (lit or not) balance over the chip, where one contiguous eighth per
-  // XCD left the XCDs covering the lit region working alone at the end.
+  const int lane = threadIdx.x;
+  // One single-wave workgroup per item, dispatched by the hardware as waves
+  // retire (a persistent grid walking the items ran the same instructions
+  // 20% slower). Items are dealt to the XCDs in chunks of kChunk consecutive
+  // items (the interleave classes of neighbouring strips, which read the
+  // same G-buffer lines), chunk c to XCD c mod 8: workgroup b runs on XCD
+  // b mod 8 and takes that XCD's (b / 8)-th item. Every XCD works all over
+  // the screen, so strips of very different cost (lit or not) balance over
+  // the chip.
   constexpr uint32_t kChunk = 64;
-  const uint32_t slots = (gridDim.x / kBatchXcds) * 4;
-  const uint32_t slot = (blockIdx.x / kBatchXcds) * 4 + wid;
-  const uint32_t xcd = blockIdx.x % kBatchXcds;
-  const uint32_t n_chunks = (total + kChunk - 1) / kChunk;
-  for (uint32_t i = slot;; i += slots) {
-    const uint32_t chunk = (i / kChunk) * kBatchXcds + xcd;
-    if (chunk >= n_chunks) break;
-    const uint32_t item = chunk * kChunk + i % kChunk;
-    if (item >= total) continue;  // the tail of the last chunk
+  const uint32_t xcd = blockIdx.x % kBatchXcds, i = blockIdx.x / kBatchXcds;
+  const uint32_t item = ((i / kChunk) * kBatchXcds + xcd) * kChunk + i % kChunk;
+  if (item >= total) return;  // the tail of the last chunks
+  {
     const BatchFrame& f = frames[batch_find<&BatchFrame::first_wave>(frames, n, item)];
     const GatherArgs a = f.g;  // by value: the fields live in registers, not re-read per VPL
     const uint32_t local = item - f.first_wave;
@@ -548,38 +584,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void b
     const Pixel p = load_pixel(a, (sx << wlog) + lx, sy * (64 >> wlog) + ly, kx, ky);
     WaveBounds wb;
     const bool live = wave_bounds(p, wb);
-    if (!live) continue;  // wave-uniform: nothing of this strip is on screen
+    if (!live) return;  // wave-uniform: nothing of this strip is on screen
     const int first = k * a.per_class;
     const LitConsts lc = lit_consts(a);
-    const bool shadow = a.shadowing != 0;
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-    float tr = 0.0f, tg = 0.0f, tb = 0.0f, er = 0.0f, eg = 0.0f, eb = 0.0f;
-    for (int chunk = 0; chunk < a.per_class; chunk += 64) {
-      const int count = min(64, a.per_class - chunk);
-      const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-      unsigned long long mask = cull_chunk<kMixed>(crec, count, wb);
-      while (mask) {
-        const int j = __builtin_ctzll(mask);
-        mask &= mask - 1ull;
-        const VplLds v = uniform_vpl<kMixed>(a.vpl, first + chunk + j);
-        const VplTest vt{v.px, v.py, v.pz, v.plane, v.nx, v.ny, v.nz};
-        const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
-        const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
-        if (ndot > 0.0f && pdot < 0.0f)
-          accumulate_pair<kMixed>(a, p, vt, v, first + chunk + j, ndot, pdot, shadow, lc, ar, ag, ab);
-      }
-      if (kMixed) {
-        kahan_add(tr, er, ar);
-        kahan_add(tg, eg, ag);
-        kahan_add(tb, eb, ab);
-        ar = ag = ab = 0.0f;
-      }
-    }
-    if (kMixed) {
-      ar = tr;
-      ag = tg;
-      ab = tb;
-    }
+    // the frame's shadowing flag picks one of two instances of the VPL loop
+    // (a scalar branch), so that the lit path holds no per-lane copy of it
+    if (__builtin_amdgcn_readfirstlane(a.shadowing))
+      batch_item_sums<kMixed, true>(a, p, wb, first, lc, ar, ag, ab);
+    else
+      batch_item_sums<kMixed, false>(a, p, wb, first, lc, ar, ag, ab);
     if (p.valid) {
       float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
       gstore(o, ar * a.out_scale);
@@ -660,11 +674,13 @@ hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision
 hipError_t launch_batch_gather(hipStream_t s, const BatchFrame* frames, int n, uint32_t total,
                                int precision) {
   if (total == 0) return hipSuccess;
-  const unsigned grid = persistent_grid((total + 3) / 4, 8);
+  // every item's workgroup: whole chunks on every XCD
+  const uint64_t per_round = (uint64_t)kBatchXcds * 64;
+  const unsigned grid = (unsigned)((total + per_round - 1) / per_round * per_round);
   if (precision == 1)
-    hipLaunchKernelGGL(batch_gather_kernel<true>, dim3(grid), dim3(256), 0, s, frames, n, total);
+    hipLaunchKernelGGL(batch_gather_kernel<true>, dim3(grid), dim3(64), 0, s, frames, n, total);
   else
-    hipLaunchKernelGGL(batch_gather_kernel<false>, dim3(grid), dim3(256), 0, s, frames, n, total);
+    hipLaunchKernelGGL(batch_gather_kernel<false>, dim3(grid), dim3(64), 0, s, frames, n, total);
   return hipGetLastError();
 }
 
