@@ -82,6 +82,19 @@ __device__ __forceinline__ XcdRange xcd_range(uint32_t total, uint32_t slot, uin
   return XcdRange{begin, end, begin + slot, slots_per_xcd};
 }
 
+// Item of workgroup blockIdx.x in a launch of one workgroup per item over a
+// grid of xcd_items_grid(items) workgroups: XCD b % 8 takes one contiguous
+// eighth of the items, as xcd_range does for a persistent grid, and the
+// hardware dispatches each item's workgroup as others retire. Items at or
+// past `items` have no work.
+__device__ __forceinline__ uint32_t xcd_item(uint32_t items) {
+  const uint32_t per_xcd = (items + kBatchXcds - 1) / kBatchXcds;
+  return (blockIdx.x % kBatchXcds) * per_xcd + blockIdx.x / kBatchXcds;
+}
+inline unsigned xcd_items_grid(uint32_t items) {
+  return (unsigned)((items + kBatchXcds - 1) / kBatchXcds * kBatchXcds);
+}
+
 // Persistent grid size: `per_cu` workgroups per CU on 256 CUs, at most one
 // per item, rounded up to a multiple of the XCD count.
 inline unsigned persistent_grid(uint32_t items, unsigned per_cu) {

@@ -408,26 +408,23 @@ __device__ __forceinline__ TileJob tile_job(const BatchFrame& f) {
                  f.ism_side, f.atlas_side, f.z_far, f.atlas, f.mask, f.spill_state, f.spill};
 }
 
-// Single-frame kernels. Grids are multiples of 8; workgroup k serves XCD
-// k % 8, which walks one contiguous eighth of the items.
+// Single-frame kernels: one workgroup per item (xcd_item), XCD x taking one
+// contiguous eighth of the items (a persistent grid walking them measured
+// slower: the hardware balances retiring workgroups better).
 __global__ __launch_bounds__(256) void ism_mask_kernel(TileJob j, int n_bins) {
   __shared__ Candidate cand[kMaxTest];
-  const XcdRange r = xcd_range((uint32_t)n_bins, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
-  for (uint32_t b = r.start; b < r.end; b += r.stride) {
-    mask_bin(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, const_cast<uint16_t*>(j.mask),
-             j.spill_state, (int)b);
-    __syncthreads();
-  }
+  const uint32_t b = xcd_item((uint32_t)n_bins);
+  if (b >= (uint32_t)n_bins) return;
+  mask_bin(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, const_cast<uint16_t*>(j.mask),
+           j.spill_state, (int)b);
 }
 
 template <class QueueT>
 __global__ __launch_bounds__(256) void ism_tiles_kernel(TileJob j, int n_tiles) {
   QueueT* queues = reinterpret_cast<QueueT*>(tile_lds + j.ism_side * j.ism_side);
-  const XcdRange r = xcd_range((uint32_t)n_tiles, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
-  for (uint32_t v = r.start; v < r.end; v += r.stride) {
-    splat_tile(j, tile_lds, queues, (int)v);
-    __syncthreads();
-  }
+  const uint32_t v = xcd_item((uint32_t)n_tiles);
+  if (v >= (uint32_t)n_tiles) return;
+  splat_tile(j, tile_lds, queues, (int)v);
 }
 
 __global__ __launch_bounds__(256) void ism_spill_kernel(TileJob j, int n_bins) {
@@ -438,30 +435,27 @@ __global__ __launch_bounds__(256) void ism_spill_kernel(TileJob j, int n_bins) {
   }
 }
 
-// Batched kernels: persistent workgroups over the (frame, item) space of a
-// stage; frame f's items start at its first_bin / first_tile.
+// Batched kernels: one workgroup per (frame, item) of a stage's item space
+// (the spill pass stays persistent); frame f's items start at its first_bin
+// / first_tile.
 __global__ __launch_bounds__(256) void batch_mask_kernel(const BatchFrame* __restrict__ frames, int n,
                                                          uint32_t total) {
   __shared__ Candidate cand[kMaxTest];
-  const XcdRange r = xcd_range(total, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
-  for (uint32_t item = r.start; item < r.end; item += r.stride) {
-    const BatchFrame& f = frames[batch_find<&BatchFrame::first_bin>(frames, n, item)];
-    mask_bin(cand, reinterpret_cast<const float4*>(f.points), f.bin_offsets, f.n_points, f.g.vpl, f.g.n_vpl,
-             f.n1d, f.mask, f.spill_state, (int)(item - f.first_bin));
-    __syncthreads();
-  }
+  const uint32_t item = xcd_item(total);
+  if (item >= total) return;
+  const BatchFrame& f = frames[batch_find<&BatchFrame::first_bin>(frames, n, item)];
+  mask_bin(cand, reinterpret_cast<const float4*>(f.points), f.bin_offsets, f.n_points, f.g.vpl, f.g.n_vpl,
+           f.n1d, f.mask, f.spill_state, (int)(item - f.first_bin));
 }
 
 template <class QueueT>
 __global__ __launch_bounds__(256) void batch_tiles_kernel(const BatchFrame* __restrict__ frames, int n,
                                                           uint32_t total, int max_side) {
   QueueT* queues = reinterpret_cast<QueueT*>(tile_lds + max_side * max_side);
-  const XcdRange r = xcd_range(total, blockIdx.x / kBatchXcds, gridDim.x / kBatchXcds);
-  for (uint32_t item = r.start; item < r.end; item += r.stride) {
-    const BatchFrame& f = frames[batch_find<&BatchFrame::first_tile>(frames, n, item)];
-    splat_tile(tile_job(f), tile_lds, queues, (int)(item - f.first_tile));
-    __syncthreads();
-  }
+  const uint32_t item = xcd_item(total);
+  if (item >= total) return;
+  const BatchFrame& f = frames[batch_find<&BatchFrame::first_tile>(frames, n, item)];
+  splat_tile(tile_job(f), tile_lds, queues, (int)(item - f.first_tile));
 }
 
 __global__ __launch_bounds__(256) void batch_spill_kernel(const BatchFrame* __restrict__ frames, int n,
@@ -474,7 +468,6 @@ __global__ __launch_bounds__(256) void batch_spill_kernel(const BatchFrame* __re
   }
 }
 
-unsigned xcd_grid(uint32_t items, unsigned per_cu) { return persistent_grid(items, per_cu); }
 
 }  // namespace
 
@@ -528,12 +521,12 @@ hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint
                              atlas, w);
   const bool any = n_bins > 0 && n_points > 0;
   if (any) {
-    hipLaunchKernelGGL(ism_mask_kernel, dim3(xcd_grid((uint32_t)n_bins, 8)), dim3(256), 0, s, j, n_bins);
+    hipLaunchKernelGGL(ism_mask_kernel, dim3(xcd_items_grid((uint32_t)n_bins)), dim3(256), 0, s, j, n_bins);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   const int n_tiles = n1d * n1d;
-  const dim3 grid(xcd_grid((uint32_t)n_tiles, 16));
+  const dim3 grid(xcd_items_grid((uint32_t)n_tiles));
   if (ism_side > 32)
     hipLaunchKernelGGL(ism_tiles_kernel<uint16_t>, grid, dim3(256), tile_lds_bytes<uint16_t>(ism_side), s, j, n_tiles);
   else
@@ -546,14 +539,14 @@ hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint
 
 hipError_t launch_batch_mask(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_mask_kernel, dim3(xcd_grid(total, 8)), dim3(256), 0, s, frames, n, total);
+  hipLaunchKernelGGL(batch_mask_kernel, dim3(xcd_items_grid(total)), dim3(256), 0, s, frames, n, total);
   return hipGetLastError();
 }
 
 hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, uint32_t total, int max_side) {
   if (total == 0) return hipSuccess;
   if (max_side < 4 || max_side > kTileMax) return hipErrorInvalidValue;
-  const dim3 grid(xcd_grid(total, 16));
+  const dim3 grid(xcd_items_grid(total));
   if (max_side > 32)
     hipLaunchKernelGGL(batch_tiles_kernel<uint16_t>, grid, dim3(256), tile_lds_bytes<uint16_t>(max_side), s, frames, n,
                        total, max_side);
