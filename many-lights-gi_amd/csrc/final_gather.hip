@@ -566,7 +566,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   // b mod 8 and takes that XCD's (b / 8)-th item. Every XCD works all over
   // the screen, so strips of very different cost (lit or not) balance over
   // the chip.
-  constexpr uint32_t kChunk = 64;
+  constexpr uint32_t kChunk = 1024;
   const uint32_t xcd = blockIdx.x % kBatchXcds, i = blockIdx.x / kBatchXcds;
   const uint32_t item = ((i / kChunk) * kBatchXcds + xcd) * kChunk + i % kChunk;
   if (item >= total) return;  // the tail of the last chunks
@@ -675,7 +675,7 @@ hipError_t launch_batch_gather(hipStream_t s, const BatchFrame* frames, int n, u
                                int precision) {
   if (total == 0) return hipSuccess;
   // every item's workgroup: whole chunks on every XCD
-  const uint64_t per_round = (uint64_t)kBatchXcds * 64;
+  const uint64_t per_round = (uint64_t)kBatchXcds * 1024;
   const unsigned grid = (unsigned)((total + per_round - 1) / per_round * per_round);
   if (precision == 1)
     hipLaunchKernelGGL(batch_gather_kernel<true>, dim3(grid), dim3(64), 0, s, frames, n, total);
