@@ -561,11 +561,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
   // One single-wave workgroup per item, dispatched by the hardware as waves
   // retire (a persistent grid walking the items ran the same instructions
   // 20% slower). Items are dealt to the XCDs in chunks of kChunk consecutive
-  // items (the interleave classes of neighbouring strips, which read the
-  // same G-buffer lines), chunk c to XCD c mod 8: workgroup b runs on XCD
-  // b mod 8 and takes that XCD's (b / 8)-th item. Every XCD works all over
-  // the screen, so strips of very different cost (lit or not) balance over
-  // the chip.
+  // items (bands of neighbouring strips with all their interleave classes,
+  // which share G-buffer lines), chunk c to XCD c mod 8: workgroup b runs on
+  // XCD b mod 8 and takes that XCD's (b / 8)-th item. Every XCD works in
+  // several bands across the screen, so strips of very different cost (lit
+  // or not) balance over the chip. Measured at 1080p / 4K: 1024-item chunks
+  // 66 / 345 us, 64-item chunks 74 / 346 us, one contiguous eighth per XCD
+  // 71 / 359 us.
   constexpr uint32_t kChunk = 1024;
   const uint32_t xcd = blockIdx.x % kBatchXcds, i = blockIdx.x / kBatchXcds;
   const uint32_t item = ((i / kChunk) * kBatchXcds + xcd) * kChunk + i % kChunk;
