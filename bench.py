@@ -12,7 +12,7 @@ Four forms run on the same inputs:
   fp32          the plain FP32 form through the single-frame entry point
                 (mlgi_frame, GIStage::process's path)
   mixed_single  the mixed-precision form through the same entry point
-  fp32_batched  the FP32 form through the persistent batched kernels
+  fp32_batched  the FP32 form through the batched kernels
                 (mlgi_batch_run of the one frame)
   mixed         the mixed-precision form through the batched kernels
 The timed region is exactly K frames of a form bracketed by a device
@@ -46,8 +46,8 @@ Other workloads (one JSON line each, not the default):
             160k triangles, mixed batched form, light pose changed every
             frame (device RSM) and the frame replayed from a HIP graph
   batch64   64 independent frames of the 40k scene (resolutions 128^2..1024^2,
-            256..4,096 VPLs, random light poses), autotuned, one persistent
-            batch replayed from a HIP graph; ms per batch and per frame
+            256..4,096 VPLs, random light poses), autotuned, one batch
+            replayed from a HIP graph; ms per batch and per frame
   serve     the request server over 3 resident scenes (4k/40k/160k triangles),
             Poisson arrivals at --rate req/s for --seconds, HBM-budget
             admission; req/s and p50/p99 latency
@@ -328,7 +328,7 @@ def stress_bench(a, ctx, wl):
 
 def batch_bench(a, ctx):
     """64 frames of the 40k scene with random shapes and light poses, autotuned,
-    one persistent batch replayed from a HIP graph."""
+    one batch replayed from a HIP graph."""
     import numpy as np
     import mlgi
     from mlgi import workloads as W
@@ -377,7 +377,7 @@ def batch_bench(a, ctx):
     g.close()
     batch.close()
     dev.close()
-    return {"metric": "GI ms per batch of 64 mixed-shape frames (mixed, persistent batched, graph replay)",
+    return {"metric": "GI ms per batch of 64 mixed-shape frames (mixed, batched kernels, graph replay)",
             "value": round(ms, 4), "unit": "ms/batch", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
             "ms_per_step": round(ms, 4), "ms_per_frame": round(ms / 64, 4), "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp16x2+fp32", "data": DATA,

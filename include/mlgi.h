@@ -271,8 +271,8 @@ void mlgi_graph_destroy(mlgi_graph* graph);
 /* ---------------------------------------------------------------- batches
  * No counterpart in the reference (it renders one view per frame): many
  * independent GI frames (views or light poses, each with its own shapes and
- * parameters) run as one set of persistent launches, one per stage (pull-push
- * and gather once per ISM shape / precision group). Each frame's results are
+ * parameters) run as one set of launches, one per stage over the work items
+ * of all frames (pull-push and gather once per ISM shape / precision group). Each frame's results are
  * bit-identical to running it alone through the batch path. */
 typedef struct mlgi_frame_desc {
   mlgi_rsm rsm;

@@ -1,4 +1,5 @@
-// Device helpers of the persistent batched kernels.
+// Device helpers of the batched kernels (one launch per stage over all frames
+// of a batch).
 #pragma once
 
 #include "kernels.hpp"

@@ -528,8 +528,8 @@ __device__ __forceinline__ void batch_item_sums(const GatherArgs& a, const Pixel
   sb = kMixed ? tb : ab;
 }
 
-// Batched gather: persistent wave64s over the (frame, class, strip) items of
-// all frames of one precision. A strip is 2^w x 64/2^w pixels of one
+// Batched gather: one wave64 workgroup per (frame, class, strip) item, over
+// all frames of one precision in one launch. A strip is 2^w x 64/2^w pixels of one
 // interleave class (w per frame, picked by the autotuner). Waves share no
 // memory and never synchronise, so frames of different shapes share
 // workgroups freely. The per-pixel arithmetic and summation order are the

@@ -128,7 +128,7 @@ hipError_t launch_raycast(hipStream_t st, const mlgi_scene_impl::SceneView& s,
                           float* c);
 
 // ------------------------------------------------------------------ batches
-// One frame of a batch as the persistent batched kernels see it. The table
+// One frame of a batch as the batched kernels see it. The table
 // lives in device memory; every stage's work items of all frames form one
 // index space, and first_* is the frame's first item in it (frames of a
 // pull-push or gather group are contiguous and counted within their group).

@@ -1,5 +1,5 @@
 // GI request server (include/mlgi.h, "serving"): resident scenes, an HBM
-// budget arena, continuous batching over the persistent batched kernels.
+// budget arena, continuous batching over the batched kernels.
 //
 // One thread drives one context stream. Each loop iteration
 //   1. moves requests whose arrival time has passed into the FIFO queue,

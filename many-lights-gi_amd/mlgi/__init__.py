@@ -538,7 +538,7 @@ def deferred_composite(ctx: "Context", depth, normal, diffuse, gi, vsm, out, *, 
 
 
 class Batch:
-    """Many GI frames as one set of persistent batched launches.
+    """Many GI frames as one set of batched launches, one per stage.
 
     `frames` is a list of (GIStage, rsm, points, gbuffer): each stage owns its
     parameters and its VPL / atlas / ISM / image buffers.
