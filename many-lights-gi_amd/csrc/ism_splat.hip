@@ -236,22 +236,26 @@ __device__ __forceinline__ void mask_bin(Candidate* cand, const float4* __restri
 
 constexpr int kTileMax = 64;                   // largest ISM side of the ISM-major form
 constexpr int kBinsPerWave = kMaxTest / 4;     // the 16 candidate bins over 4 waves
-constexpr int kStep = 256;                     // points per bin per step
-constexpr int kQueue = kBinsPerWave * kStep;   // hit indices of one step
+// Points per bin per step, and the hit entries one step can queue per wave.
+template <class QueueT>
+constexpr int kStep = sizeof(QueueT) == 2 ? 128 : 256;
+template <class QueueT>
+constexpr int kQueue = kBinsPerWave * kStep<QueueT>;
 
 // LDS of the tile pass, sized per launch for the largest ISM side S of its
 // items: the S x S tile, then the four waves' point queues. Up to S = 32 the
-// queue entries are point indices (16 KiB; 20 KiB in all lets eight
-// workgroups share a CU, the wave limit); at S = 64 they are 16-bit codes of
-// the step's bin slot t and offset o, t * kStep + o (8 KiB; 24 KiB in all,
-// six workgroups where 32 KiB allowed five). The codes cost a decode per
-// point, which measured slower where occupancy does not rise.
+// queue entries are point indices, 256 points per bin per step (16 KiB; 20
+// KiB in all lets eight workgroups share a CU, the wave limit); at S = 64
+// they are 16-bit codes of the step's bin slot t and offset o,
+// t * kStep + o, 128 points per bin per step (4 KiB; 20 KiB in all, eight
+// workgroups per CU). The codes cost a decode per point, which measured
+// slower where occupancy does not rise.
 extern __shared__ __align__(16) uint32_t tile_lds[];
-static_assert(kBinsPerWave * kStep <= 65536, "queue codes are 16-bit");
+static_assert(kQueue<uint16_t> <= 65536, "queue codes are 16-bit");
 
 template <class QueueT>
 size_t tile_lds_bytes(int max_side) {
-  return (size_t)max_side * max_side * sizeof(uint32_t) + 4 * (size_t)kQueue * sizeof(QueueT);
+  return (size_t)max_side * max_side * sizeof(uint32_t) + 4 * (size_t)kQueue<QueueT> * sizeof(QueueT);
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -292,10 +296,11 @@ __device__ __forceinline__ void splat_queued(const TileJob& j, uint32_t* tile, c
 }
 
 // Tile pass of ISM v (all 256 lanes; ends after the tile is written). `tile`
-// holds at least S * S words, `queues` 4 * kQueue entries.
+// holds at least S * S words, `queues` 4 * kQueue<QueueT> entries.
 template <class QueueT>
 __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, QueueT* queues, int v) {
   constexpr bool kCoded = sizeof(QueueT) == 2;
+  constexpr int kSt = kStep<QueueT>, kLoads = kSt / 64;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int S = j.ism_side;
@@ -312,11 +317,11 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
     const float side = (float)j.atlas_side;
     const uint32_t ox = (uint32_t)(tile_x * S), oy = (uint32_t)(tile_y * S);
     // Wave w owns bins i = w, w + 4, w + 8, w + 12. Per step it reads the
-    // masks of up to kStep points of each of its four bins (16 loads in
-    // flight per lane), queues the hits' point indices in LDS (ballot +
+    // masks of up to kSt points of each of its four bins (4 * kSt / 64
+    // loads in flight per lane), queues the hits' point indices in LDS (ballot +
     // mbcnt), then splats the queued points 64 at a time with every lane
     // busy, the next 64 records requested while the current ones project.
-    QueueT* q = queues + wave * kQueue;
+    QueueT* q = queues + wave * kQueue<QueueT>;
     uint32_t beg[kBinsPerWave], cnt[kBinsPerWave];
     uint32_t longest = 0;
 #pragma unroll
@@ -328,12 +333,12 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
       cnt[t] = end - beg[t];
       longest = max(longest, cnt[t]);
     }
-    for (uint32_t s0 = 0; s0 < longest; s0 += kStep) {
-      uint32_t m[kBinsPerWave][4];
+    for (uint32_t s0 = 0; s0 < longest; s0 += kSt) {
+      uint32_t m[kBinsPerWave][kLoads];
 #pragma unroll
       for (int t = 0; t < kBinsPerWave; ++t)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kLoads; ++k) {
           const uint32_t o = s0 + (uint32_t)(k * 64 + lane);
           m[t][k] = o < cnt[t] ? (uint32_t)gload(j.mask + beg[t] + o) : 0u;
         }
@@ -341,14 +346,14 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
 #pragma unroll
       for (int t = 0; t < kBinsPerWave; ++t)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kLoads; ++k) {
           const bool hit = (m[t][k] >> (wave + 4 * t)) & 1u;
           const uint64_t bal = __ballot(hit);
           if (hit) {
             const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
             const uint32_t o = (uint32_t)(k * 64 + lane);
-            q[slot] = (QueueT)(kCoded ? (uint32_t)t * kStep + o : beg[t] + s0 + o);
+            q[slot] = (QueueT)(kCoded ? (uint32_t)t * kSt + o : beg[t] + s0 + o);
           }
           qn += __builtin_popcountll(bal);
         }
@@ -359,7 +364,7 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
         auto point_of = [&](int e) {
           const uint32_t x = q[min(e, qn - 1)];
           if (!kCoded) return x;
-          const uint32_t t = x / kStep, o = x % kStep;
+          const uint32_t t = x / kSt, o = x % kSt;
           const uint32_t b = t == 0 ? beg[0] : t == 1 ? beg[1] : t == 2 ? beg[2] : beg[3];
           return b + s0 + o;
         };
