@@ -165,6 +165,13 @@ const char* mlgi_last_error(const mlgi_context* ctx);
 int mlgi_sync(mlgi_context* ctx);
 /* hipStream_t of the context, as an opaque pointer. */
 void* mlgi_stream(mlgi_context* ctx);
+/* Orders ctx's stream after everything enqueued so far on `after`'s stream
+ * (an event recorded on one and waited on by the other; no host thread
+ * blocks). Both contexts on one device, neither in use by another thread
+ * meanwhile. No counterpart in the reference (it renders on one GL context):
+ * lets two contexts pipeline work, e.g. the next frame's RSM render beside
+ * the current frame's GI. */
+int mlgi_context_wait(mlgi_context* ctx, mlgi_context* after);
 
 /* Defaults of GIStage::initialize (GIStage.cpp:215-232) plus the atlas and
  * pull-push constants of ImperfectShadowmap (2048 side, 6 levels). */

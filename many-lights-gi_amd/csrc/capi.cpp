@@ -187,6 +187,7 @@ void mlgi_context_destroy(mlgi_context* c) {
   for (auto e : c->pool) (void)hipEventDestroy(e);
   (void)hipFree(c->counter);
   if (c->splat_ws) (void)hipFree(c->splat_ws);
+  if (c->order_event) (void)hipEventDestroy(c->order_event);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -199,6 +200,24 @@ int mlgi_sync(mlgi_context* c) {
 }
 
 void* mlgi_stream(mlgi_context* c) { return c ? (void*)c->stream : nullptr; }
+
+int mlgi_context_wait(mlgi_context* c, mlgi_context* after) {
+  if (!c || !after) return MLGI_ERR_INVALID_ARGUMENT;
+  if (c == after) return MLGI_OK;  // one stream is already in order
+  if (c->device != after->device)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "mlgi_context_wait: contexts on devices %d and %d", c->device,
+                after->device);
+  if (int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+  if (!after->order_event) {
+    if (int rc = hip_check(c, hipEventCreateWithFlags(&after->order_event, hipEventDisableTiming),
+                           "hipEventCreateWithFlags"))
+      return rc;
+  }
+  // A wait applies to the event's latest record at the time of the wait, so
+  // one event per context serves every call.
+  if (int rc = hip_check(c, hipEventRecord(after->order_event, after->stream), "hipEventRecord")) return rc;
+  return hip_check(c, hipStreamWaitEvent(c->stream, after->order_event, 0), "hipStreamWaitEvent");
+}
 
 void mlgi_params_default(mlgi_params* p) {
   if (!p) return;

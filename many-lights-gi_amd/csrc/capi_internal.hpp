@@ -32,6 +32,8 @@ struct mlgi_context {
   // work area of the single-frame ISM-major splat (grown on demand)
   void* splat_ws = nullptr;
   size_t splat_ws_bytes = 0;
+  // recorded by mlgi_context_wait when another context waits for this one
+  hipEvent_t order_event = nullptr;
 };
 
 namespace mlgi_capi {

@@ -84,6 +84,7 @@ class Context:
         if rc:
             raise MlgiError(rc, f"mlgi_context_create(device={device}) failed")
         self.h = h
+        self.device = device
         self._live: set[int] = set()
 
     def check(self, rc: int, what: str = "") -> None:
@@ -123,6 +124,11 @@ class Context:
 
     def sync(self) -> None:
         self.check(lib.mlgi_sync(self.h), "mlgi_sync")
+
+    def wait_for(self, other: "Context") -> None:
+        """Order this context's stream after the work enqueued so far on
+        `other`'s (mlgi_context_wait; the host does not block)."""
+        self.check(lib.mlgi_context_wait(self.h, other.h), "mlgi_context_wait")
 
     def probe_valu(self) -> float:
         """Measured FP32 vector FMA rate (lane FMAs per second)."""
@@ -577,6 +583,83 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+class RsmPipeline:
+    """Frames whose light moves every frame, with each frame's RSM ray-cast on
+    the device. The RSM of frame i + 1 renders on a second context (its own
+    stream) while frame i's GI runs on `ctx`, into the other of two RSM
+    buffers; mlgi_context_wait orders the streams: frame i's GI after its RSM
+    render, the render of frame i + 1 after frame i - 1's GI (the last reader
+    of its buffer). The work and the results are those of rendering and then
+    running each frame in turn. The stage's frame runs as the batched kernels
+    replayed from a HIP graph, one graph per RSM buffer.
+
+    `light_of(i)` is frame i's light Camera. After `frame()` returned i and
+    `ctx` has synchronised, frame i's results are in the stage's buffers.
+    """
+
+    def __init__(self, ctx: Context, scene: Scene, stage: "GIStage", points: N.Points, gbuffer: N.GBuffer,
+                 rsm_size: int, light_of):
+        self.ctx, self.light_of = ctx, light_of
+        self.render_ctx, self.dev, self.batches, self.graphs = None, None, [], []
+        try:  # on failure, release what was made here before the error leaves
+            self.render_ctx = Context(ctx.device)
+            self.dev = scene.upload(self.render_ctx)
+            r = rsm_size
+            self.bufs = [(ctx.empty((r, r, 3), np.float32), ctx.empty((r, r, 3), np.float32),
+                          ctx.empty((r, r), np.float32)) for _ in range(2)]
+            eye = np.eye(4, dtype=np.float32).ravel()  # replaced per frame by update_light
+            for b in self.bufs:
+                self.batches.append(Batch(ctx, [(stage, rsm_desc(*b, eye), points, gbuffer)]))
+            for b in self.batches:
+                self.graphs.append(b.capture())
+            self.next = 0
+            self._inv = self._render(0)
+        except BaseException:
+            self.close()
+            raise
+
+    def _render(self, i: int):
+        dif, nrm, dep = self.bufs[i % 2]
+        return self.dev.render_rsm(self.light_of(i), dif, nrm, dep)
+
+    def frame(self) -> int:
+        """Enqueue the next frame; returns its index."""
+        i = self.next
+        self.ctx.wait_for(self.render_ctx)   # frame i's RSM render, the last work on the render stream
+        self.render_ctx.wait_for(self.ctx)   # frame i - 1's GI, the last work on ctx
+        inv_next = self._render(i + 1)
+        self.batches[i % 2].update_light(0, self._inv)
+        self.graphs[i % 2].launch()
+        self._inv, self.next = inv_next, i + 1
+        return i
+
+    def sync(self) -> None:
+        self.ctx.sync()
+        if self.render_ctx is not None:
+            self.render_ctx.sync()
+
+    def close(self) -> None:
+        if getattr(self, "render_ctx", None) is None:
+            return
+        try:
+            self.sync()
+        finally:
+            for g in self.graphs:
+                g.close()
+            for b in self.batches:
+                b.close()
+            if self.dev is not None:
+                self.dev.close()
+            self.render_ctx.close()
+            self.graphs, self.batches, self.dev, self.render_ctx = [], [], None, None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def frame_desc(stage: "GIStage", rsm: N.Rsm, points: N.Points, gbuffer: N.GBuffer) -> N.FrameDesc:

@@ -181,6 +181,7 @@ SIGNATURES = {
     "mlgi_last_error": (C.c_char_p, [vp]),
     "mlgi_sync": (C.c_int, [vp]),
     "mlgi_stream": (vp, [vp]),
+    "mlgi_context_wait": (C.c_int, [vp, vp]),
     "mlgi_params_default": (None, [P(Params)]),
     "mlgi_malloc": (C.c_int, [vp, C.c_size_t, P(vp)]),
     "mlgi_free": (C.c_int, [vp, vp]),
