@@ -39,6 +39,7 @@ int fail(mlgi_context* c, int code, const char* fmt, ...) {
 }
 
 int hip_check(mlgi_context* c, hipError_t e, const char* what) {
+  if (c) c->tail = nullptr;  // the call may have enqueued work after the tail event
   if (e == hipSuccess) return MLGI_OK;
   return fail(c, e == hipErrorOutOfMemory ? MLGI_ERR_OUT_OF_MEMORY : MLGI_ERR_HIP, "%s: %s", what,
               hipGetErrorString(e));
@@ -60,6 +61,30 @@ hipEvent_t take_event(mlgi_context* c) {
   hipEvent_t e = nullptr;
   if (hipEventCreate(&e) != hipSuccess) return nullptr;
   return e;
+}
+
+hipEvent_t tail_event(mlgi_context* c) {
+  if (c->tail) return c->tail;
+  hipEvent_t e = take_event(c);
+  if (!e) return nullptr;
+  if (hipEventRecord(e, c->stream) != hipSuccess) {
+    c->pool.push_back(e);
+    return nullptr;
+  }
+  c->tail = e;
+  return e;
+}
+
+// The events of the recorded spans, each once (adjacent spans share one).
+static std::vector<hipEvent_t> span_events(const mlgi_context* c) {
+  std::vector<hipEvent_t> v;
+  for (const auto& s : c->spans) {
+    v.push_back(s.a);
+    v.push_back(s.b);
+  }
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  return v;
 }
 
 
@@ -180,10 +205,7 @@ void mlgi_context_destroy(mlgi_context* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->shuffle) (void)hipFree(kv.second);
-  for (auto& s : c->spans) {
-    (void)hipEventDestroy(s.a);
-    (void)hipEventDestroy(s.b);
-  }
+  for (auto e : span_events(c)) (void)hipEventDestroy(e);
   for (auto e : c->pool) (void)hipEventDestroy(e);
   (void)hipFree(c->counter);
   if (c->splat_ws) (void)hipFree(c->splat_ws);
@@ -701,11 +723,9 @@ int mlgi_profile_enable(mlgi_context* c, int32_t on) {
 int mlgi_profile_reset(mlgi_context* c) {
   if (!c) return MLGI_ERR_INVALID_ARGUMENT;
   (void)hipStreamSynchronize(c->stream);
-  for (auto& s : c->spans) {
-    c->pool.push_back(s.a);
-    c->pool.push_back(s.b);
-  }
+  for (auto e : span_events(c)) c->pool.push_back(e);
   c->spans.clear();
+  c->tail = nullptr;
   c->frames = 0;
   return MLGI_OK;
 }

@@ -25,6 +25,10 @@ struct mlgi_context {
     hipEvent_t a, b;
   };
   std::vector<Span> spans;
+  // Event last recorded at the stream's tail by a Scope, while nothing has
+  // been enqueued after it (see Scope); depth of the open scopes.
+  hipEvent_t tail = nullptr;
+  int scope_depth = 0;
   std::vector<hipEvent_t> pool;
   int frames = 0;
   // autotuner cache: shape key -> (tile_w, tile_h)
@@ -55,26 +59,34 @@ int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
                       const mlgi_ism_depth* ism, const mlgi_params* p, mlgi::GatherArgs* a);
 void pick_tile(const mlgi_params* p, int* tw, int* th);
 
-struct Scope {  // records a timed span on the context stream when profiling
+// An event at the stream's tail: c->tail when nothing was enqueued since it
+// was recorded, else a newly recorded one (nullptr if recording fails).
+hipEvent_t tail_event(mlgi_context* c);
+
+// Records a timed span on the context stream when profiling. Adjacent
+// boundaries share one event (a stage's end is the next stage's start, an
+// outer scope's start its first inner scope's start): every event record
+// costs the stream a few microseconds between kernels. c->tail is cleared by
+// hip_check, through which every enqueue of the library goes, and when the
+// outermost scope ends, so a shared event never spans work enqueued in
+// between, nor anything a caller enqueues between two calls.
+struct Scope {
   mlgi_context* c;
   int stage;
+  bool on;
   hipEvent_t a = nullptr;
-  Scope(mlgi_context* ctx, int st) : c(ctx), stage(st) {
-    if (c->profile) {
-      a = take_event(c);
-      if (a) (void)hipEventRecord(a, c->stream);
-    }
+  Scope(mlgi_context* ctx, int st) : c(ctx), stage(st), on(ctx->profile) {
+    if (!on) return;
+    ++c->scope_depth;
+    a = tail_event(c);
   }
   ~Scope() {
-    if (c->profile && a) {
-      hipEvent_t b = take_event(c);
-      if (b) {
-        (void)hipEventRecord(b, c->stream);
-        c->spans.push_back({stage, a, b});
-      } else {
-        c->pool.push_back(a);
-      }
+    if (!on) return;
+    if (a) {
+      hipEvent_t b = tail_event(c);
+      if (b) c->spans.push_back({stage, a, b});
     }
+    if (--c->scope_depth == 0) c->tail = nullptr;
   }
 };
 
