@@ -556,21 +556,19 @@ __device__ __forceinline__ void batch_item_sums(const GatherArgs& a, const Pixel
 // stay live across the pixel reconstruction and the bound reductions.)
 template <bool kMixed>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
-                                                           uint32_t total) {
+                                                           uint32_t total, int chunk_log2) {
   const int lane = threadIdx.x;
   // One single-wave workgroup per item, dispatched by the hardware as waves
   // retire (a persistent grid walking the items ran the same instructions
-  // 20% slower). Items are dealt to the XCDs in chunks of kChunk consecutive
-  // items (bands of neighbouring strips with all their interleave classes,
-  // which share G-buffer lines), chunk c to XCD c mod 8: workgroup b runs on
-  // XCD b mod 8 and takes that XCD's (b / 8)-th item. Every XCD works in
-  // several bands across the screen, so strips of very different cost (lit
-  // or not) balance over the chip. Measured at 1080p / 4K: 1024-item chunks
-  // 66 / 345 us, 64-item chunks 74 / 346 us, one contiguous eighth per XCD
-  // 71 / 359 us.
-  constexpr uint32_t kChunk = 1024;
+  // 20% slower). Items are dealt to the XCDs in chunks of 2^chunk_log2
+  // consecutive items (bands of neighbouring strips with all their
+  // interleave classes, which share G-buffer lines), chunk c to XCD c mod 8:
+  // workgroup b runs on XCD b mod 8 and takes that XCD's (b / 8)-th item.
+  // Every XCD works in several bands across the screen, so strips of very
+  // different cost (lit or not) balance over the chip.
   const uint32_t xcd = blockIdx.x % kBatchXcds, i = blockIdx.x / kBatchXcds;
-  const uint32_t item = ((i / kChunk) * kBatchXcds + xcd) * kChunk + i % kChunk;
+  const uint32_t chunk_mask = (1u << chunk_log2) - 1u;
+  const uint32_t item = (((i >> chunk_log2) * kBatchXcds + xcd) << chunk_log2) | (i & chunk_mask);
   if (item >= total) return;  // the tail of the last chunks
   {
     const BatchFrame& f = frames[batch_find<&BatchFrame::first_wave>(frames, n, item)];
@@ -676,13 +674,20 @@ hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision
 hipError_t launch_batch_gather(hipStream_t s, const BatchFrame* frames, int n, uint32_t total,
                                int precision) {
   if (total == 0) return hipSuccess;
+  // Chunks of 1024 items, halved until there are at least two per XCD: a
+  // 256^2 frame (1,024 strips) in one 1024-item chunk would run on one XCD.
+  // Measured at 1080p / 4K (32,400 / 129,600 items, so 1024 there): 1024-item
+  // chunks 66 / 345 us, 64-item chunks 74 / 346 us, one contiguous eighth per
+  // XCD 71 / 359 us.
+  int chunk_log2 = 10;
+  while (chunk_log2 > 0 && total < 2u * kBatchXcds * (1u << chunk_log2)) --chunk_log2;
   // every item's workgroup: whole chunks on every XCD
-  const uint64_t per_round = (uint64_t)kBatchXcds * 1024;
+  const uint64_t per_round = (uint64_t)kBatchXcds << chunk_log2;
   const unsigned grid = (unsigned)((total + per_round - 1) / per_round * per_round);
   if (precision == 1)
-    hipLaunchKernelGGL(batch_gather_kernel<true>, dim3(grid), dim3(64), 0, s, frames, n, total);
+    hipLaunchKernelGGL(batch_gather_kernel<true>, dim3(grid), dim3(64), 0, s, frames, n, total, chunk_log2);
   else
-    hipLaunchKernelGGL(batch_gather_kernel<false>, dim3(grid), dim3(64), 0, s, frames, n, total);
+    hipLaunchKernelGGL(batch_gather_kernel<false>, dim3(grid), dim3(64), 0, s, frames, n, total, chunk_log2);
   return hipGetLastError();
 }
 
