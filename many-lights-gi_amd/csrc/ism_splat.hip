@@ -236,17 +236,18 @@ __device__ __forceinline__ void mask_bin(Candidate* cand, const float4* __restri
 
 constexpr int kTileMax = 64;                   // largest ISM side of the ISM-major form
 constexpr int kBinsPerWave = kMaxTest / 4;     // the 16 candidate bins over 4 waves
-// Points per bin per step, and the hit entries one step can queue per wave.
+// Points per bin per step, and the queue entries per wave: a step's hits,
+// plus (point-index queues) fewer than 64 kept from the previous step.
 template <class QueueT>
-constexpr int kStep = sizeof(QueueT) == 2 ? 128 : 256;
+constexpr int kStep = sizeof(QueueT) == 2 ? 128 : 192;
 template <class QueueT>
-constexpr int kQueue = kBinsPerWave * kStep<QueueT>;
+constexpr int kQueue = kBinsPerWave * kStep<QueueT> + (sizeof(QueueT) == 2 ? 0 : 64);
 
 // LDS of the tile pass, sized per launch for the largest ISM side S of its
 // items: the S x S tile, then the four waves' point queues. Up to S = 32 the
-// queue entries are point indices, 256 points per bin per step (16 KiB; 20
-// KiB in all lets eight workgroups share a CU, the wave limit); at S = 64
-// they are 16-bit codes of the step's bin slot t and offset o,
+// queue entries are point indices, 192 points per bin per step (13 KiB; 17
+// KiB in all with a 32^2 tile, eight workgroups per CU, the wave limit); at
+// S = 64 they are 16-bit codes of the step's bin slot t and offset o,
 // t * kStep + o, 128 points per bin per step (4 KiB; 20 KiB in all, eight
 // workgroups per CU). The codes cost a decode per point, which measured
 // slower where occupancy does not rise.
@@ -333,6 +334,12 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
       cnt[t] = end - beg[t];
       longest = max(longest, cnt[t]);
     }
+    // Point indices stay valid from one step to the next, so a point-index
+    // queue keeps the hits of a step's last, partial round of 64 for the next
+    // step (a partial round costs as much as a full one) and only the last
+    // step drains it; a coded queue, whose entries are relative to the step,
+    // is drained every step.
+    int carry = 0;  // entries q[0, carry) kept from the previous step (< 64); wave-uniform
     for (uint32_t s0 = 0; s0 < longest; s0 += kSt) {
       uint32_t m[kBinsPerWave][kLoads];
 #pragma unroll
@@ -342,7 +349,7 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
           const uint32_t o = s0 + (uint32_t)(k * 64 + lane);
           m[t][k] = o < cnt[t] ? (uint32_t)gload(j.mask + beg[t] + o) : 0u;
         }
-      int qn = 0;  // wave-uniform
+      int qn = carry;  // wave-uniform
 #pragma unroll
       for (int t = 0; t < kBinsPerWave; ++t)
 #pragma unroll
@@ -358,24 +365,36 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
           qn += __builtin_popcountll(bal);
         }
       wave_sync();
-      if (qn > 0) {
-        // entry -> point index beg[t] + s0 + o; lanes past the queue end
-        // read its last entry (a valid point)
+      // entries splatted this step: all of them, or the whole rounds of 64
+      // of a point-index queue before its last step
+      const int np = (kCoded || s0 + kSt >= longest) ? qn : (qn & ~63);
+      if (np > 0) {
+        // entry -> point index; lanes past the projected entries read the
+        // last of them (a valid point)
         auto point_of = [&](int e) {
-          const uint32_t x = q[min(e, qn - 1)];
+          const uint32_t x = q[min(e, np - 1)];
           if (!kCoded) return x;
           const uint32_t t = x / kSt, o = x % kSt;
           const uint32_t b = t == 0 ? beg[0] : t == 1 ? beg[1] : t == 2 ? beg[2] : beg[3];
           return b + s0 + o;
         };
         float4 rec = gload4(j.points + point_of(lane));
-        for (int c0 = 0; c0 < qn; c0 += 64) {
+        for (int c0 = 0; c0 < np; c0 += 64) {
           const float4 next = gload4(j.points + point_of(c0 + 64 + lane));
-          if (c0 + lane < qn) splat_queued(j, tile, c, rec, ox, oy, inv_n1d, side);
+          if (c0 + lane < np) splat_queued(j, tile, c, rec, ox, oy, inv_n1d, side);
           rec = next;
         }
       }
-      wave_sync();  // the queue is read before the next step refills it
+      wave_sync();  // the queue is read before it is rewritten
+      if (!kCoded) {
+        // move the remaining entries q[np, qn) (fewer than 64) to the front
+        const int rem = qn - np;
+        const QueueT v = lane < rem ? q[np + lane] : QueueT(0);
+        wave_sync();
+        if (lane < rem) q[lane] = v;
+        wave_sync();
+        carry = rem;
+      }
     }
   }
   __syncthreads();
