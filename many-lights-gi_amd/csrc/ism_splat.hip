@@ -236,23 +236,29 @@ __device__ __forceinline__ void mask_bin(Candidate* cand, const float4* __restri
 
 constexpr int kTileMax = 64;                   // largest ISM side of the ISM-major form
 constexpr int kBinsPerWave = kMaxTest / 4;     // the 16 candidate bins over 4 waves
-// Points per bin per step, and the queue entries per wave: a step's hits,
-// plus (point-index queues) fewer than 64 kept from the previous step.
+// Points per bin per step, and the queue entries per wave: a step's hits
+// plus fewer than 64 kept from earlier steps.
 template <class QueueT>
 constexpr int kStep = sizeof(QueueT) == 2 ? 128 : 192;
 template <class QueueT>
-constexpr int kQueue = kBinsPerWave * kStep<QueueT> + (sizeof(QueueT) == 2 ? 0 : 64);
+constexpr int kQueue = kBinsPerWave * kStep<QueueT> + 64;
+
+// 16-bit queue codes: bin slot t (2 bits) and the point's offset in the bin
+// relative to the current step's first point, biased by kCodeBias (14 bits).
+// Kept entries are re-coded (offset - kStep) at each step; draining the
+// queue every kMaxAge steps keeps every offset >= -kCodeBias.
+constexpr uint32_t kCodeBias = 8192;
+constexpr uint32_t kMaxAge = kCodeBias / kStep<uint16_t>;
+static_assert(kStep<uint16_t> < kCodeBias && kBinsPerWave == 4, "codes hold 2 + 14 bits");
 
 // LDS of the tile pass, sized per launch for the largest ISM side S of its
 // items: the S x S tile, then the four waves' point queues. Up to S = 32 the
 // queue entries are point indices, 192 points per bin per step (13 KiB; 17
 // KiB in all with a 32^2 tile, eight workgroups per CU, the wave limit); at
-// S = 64 they are 16-bit codes of the step's bin slot t and offset o,
-// t * kStep + o, 128 points per bin per step (4 KiB; 20 KiB in all, eight
-// workgroups per CU). The codes cost a decode per point, which measured
-// slower where occupancy does not rise.
+// S = 64 they are 16-bit codes, 128 points per bin per step (4.5 KiB; 20.5
+// KiB in all, seven workgroups per CU). The codes cost a decode per point,
+// which measured slower where occupancy does not rise.
 extern __shared__ __align__(16) uint32_t tile_lds[];
-static_assert(kQueue<uint16_t> <= 65536, "queue codes are 16-bit");
 
 template <class QueueT>
 size_t tile_lds_bytes(int max_side) {
@@ -334,12 +340,10 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
       cnt[t] = end - beg[t];
       longest = max(longest, cnt[t]);
     }
-    // Point indices stay valid from one step to the next, so a point-index
-    // queue keeps the hits of a step's last, partial round of 64 for the next
-    // step (a partial round costs as much as a full one) and only the last
-    // step drains it; a coded queue, whose entries are relative to the step,
-    // is drained every step.
-    int carry = 0;  // entries q[0, carry) kept from the previous step (< 64); wave-uniform
+    // The queue keeps the hits of a step's last, partial round of 64 for the
+    // next step (a partial round costs as much as a full one); the last step
+    // drains it, and so does every kMaxAge-th step of a coded queue.
+    int carry = 0;  // entries q[0, carry) kept from earlier steps (< 64); wave-uniform
     for (uint32_t s0 = 0; s0 < longest; s0 += kSt) {
       uint32_t m[kBinsPerWave][kLoads];
 #pragma unroll
@@ -360,23 +364,24 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
             const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
             const uint32_t o = (uint32_t)(k * 64 + lane);
-            q[slot] = (QueueT)(kCoded ? (uint32_t)t * kSt + o : beg[t] + s0 + o);
+            q[slot] = (QueueT)(kCoded ? ((uint32_t)t << 14) | (o + kCodeBias) : beg[t] + s0 + o);
           }
           qn += __builtin_popcountll(bal);
         }
       wave_sync();
-      // entries splatted this step: all of them, or the whole rounds of 64
-      // of a point-index queue before its last step
-      const int np = (kCoded || s0 + kSt >= longest) ? qn : (qn & ~63);
+      // entries splatted this step: the whole rounds of 64, or all of them
+      // when the queue drains
+      const bool drain = s0 + kSt >= longest || (kCoded && (s0 / kSt) % kMaxAge == kMaxAge - 1);
+      const int np = drain ? qn : (qn & ~63);
       if (np > 0) {
         // entry -> point index; lanes past the projected entries read the
         // last of them (a valid point)
         auto point_of = [&](int e) {
           const uint32_t x = q[min(e, np - 1)];
           if (!kCoded) return x;
-          const uint32_t t = x / kSt, o = x % kSt;
+          const uint32_t t = x >> 14;
           const uint32_t b = t == 0 ? beg[0] : t == 1 ? beg[1] : t == 2 ? beg[2] : beg[3];
-          return b + s0 + o;
+          return b + s0 + (x & 0x3FFFu) - kCodeBias;  // offset >= -s0
         };
         float4 rec = gload4(j.points + point_of(lane));
         for (int c0 = 0; c0 < np; c0 += 64) {
@@ -386,10 +391,12 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
         }
       }
       wave_sync();  // the queue is read before it is rewritten
-      if (!kCoded) {
-        // move the remaining entries q[np, qn) (fewer than 64) to the front
+      {
+        // move the remaining entries q[np, qn) (fewer than 64) to the front,
+        // codes re-coded relative to the next step
         const int rem = qn - np;
-        const QueueT v = lane < rem ? q[np + lane] : QueueT(0);
+        QueueT v = 0;
+        if (lane < rem) v = kCoded ? (QueueT)(q[np + lane] - kSt) : q[np + lane];
         wave_sync();
         if (lane < rem) q[lane] = v;
         wave_sync();
