@@ -80,6 +80,10 @@ typedef struct mlgi_params {
   int32_t splat_block;            /* 0 = ISM-major splat (LDS tiles, no clear);
                                      64/128/256 = point-major splat (global atomics
                                      into a cleared atlas) with that workgroup size */
+  int32_t disable_culling;        /* 0 = per-wave VPL culling (default); 1 = every VPL
+                                     of the pixel's class is evaluated, as the
+                                     reference's full light lists: the same sums,
+                                     bit for bit (a verification switch) */
 } mlgi_params;
 
 /* RSM inputs (RasterizationStage outputs bound in VPLProcessor.cpp:84-86).
@@ -207,8 +211,14 @@ int mlgi_ism_splat(mlgi_context* ctx, const mlgi_points* pts, const mlgi_vpls* v
  * spill list (then the point-major splat ran over the tiles). No reference
  * counterpart. */
 int mlgi_splat_spill_stats(mlgi_context* ctx, uint32_t* spills, int32_t* overflowed);
+/* Number of (point, ISM) splats of the context's last ISM-major splat run
+ * through mlgi_ism_splat (the LDS atomic mins of its tile pass, at most 4
+ * per point); waits for the stream. A measurement aid for the splat's
+ * atomic rate; no reference counterpart. */
+int mlgi_splat_count(mlgi_context* ctx, uint64_t* splats);
 /* Pull-push hole filling, one workgroup per ISM, R16 result. Bit-exact.
- * atlas->ism_side (ismPixelSize) must be one of 8, 16, 32, 64. */
+ * atlas->ism_side (ismPixelSize) must be one of 8, 16, 32, 64; the atlas
+ * must be 16-byte and the output 8-byte aligned (as mlgi_malloc returns). */
 int mlgi_ism_pullpush(mlgi_context* ctx, const mlgi_atlas* atlas, const mlgi_params* p,
                       mlgi_ism_depth* out);
 /* Splat + pull-push (ImperfectShadowmap::process). */
@@ -266,7 +276,10 @@ int mlgi_deferred_composite(mlgi_context* ctx, const mlgi_composite_inputs* in,
 typedef struct mlgi_graph mlgi_graph;
 /* Capture one mlgi_frame with these descriptors into an instantiated HIP
  * graph (kernel nodes only). Replays read the buffers' current contents, so
- * new RSM / G-buffer data can be uploaded between replays. */
+ * new RSM / G-buffer data can be uploaded between replays. The graph owns
+ * its splat work area (2 bytes per point + 128 KiB, allocated here, freed by
+ * mlgi_graph_destroy), so eager work on the context between replays, of any
+ * size, never touches memory a replay uses. */
 int mlgi_graph_capture_frame(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_points* pts,
                              const mlgi_gbuffer* gb, const mlgi_params* p, mlgi_vpls* vpls,
                              mlgi_atlas* atlas, mlgi_ism_depth* ism, mlgi_image* out,

@@ -394,13 +394,24 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
   if (p->splat_block == 0 && mlgi::ism_major_supported(s) && ((uintptr_t)atlas->texels & 15u) == 0) {
     // ISM-major form: writes every texel, no clear
     const uint32_t np = pts->count > 0 ? (uint32_t)pts->count : 0u;
-    if ((rc = ensure_splat_work(c, mlgi::splat_work_bytes(np)))) return rc;
+    const size_t need = mlgi::splat_work_bytes(np);
+    void* ws = nullptr;
+    if (c->capture_ws) {
+      if (c->capture_ws_bytes < need)
+        return fail(c, MLGI_ERR_INVALID_ARGUMENT, "captured splat needs %zu work bytes, the graph holds %zu", need,
+                    c->capture_ws_bytes);
+      ws = c->capture_ws;
+    } else {
+      if ((rc = ensure_splat_work(c, need))) return rc;
+      ws = c->splat_ws;
+      c->splat_ws_points = np;
+    }
     Scope sc(c, kSplat);
     return hip_check(c,
                      mlgi::launch_ism_splat_tiled(c->stream, pts->records, pts->bin_offsets,
                                                   pts->count > 0 ? pts->n_bins : 0, pts->count, vpls->records,
                                                   vpls->count, n1d, s, atlas->side, p->z_far, atlas->texels,
-                                                  mlgi::splat_work(c->splat_ws, np)),
+                                                  mlgi::splat_work(ws, np)),
                      "ism_splat_tiled");
   }
   {
@@ -469,6 +480,25 @@ int mlgi_splat_spill_stats(mlgi_context* c, uint32_t* spills, int32_t* overflowe
   return MLGI_OK;
 }
 
+int mlgi_splat_count(mlgi_context* c, uint64_t* splats) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!splats) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "splats is NULL");
+  *splats = 0;
+  if (!c->splat_ws || c->splat_ws_points < 0)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "no ISM-major splat has run through mlgi_ism_splat on this context");
+  const uint32_t np = (uint32_t)c->splat_ws_points;
+  int rc = hip_check(c, hipMemsetAsync(c->counter, 0, sizeof(unsigned long long), c->stream), "hipMemsetAsync");
+  if (!rc)
+    rc = hip_check(c, mlgi::launch_count_splats(c->stream, mlgi::splat_work(c->splat_ws, np), np, c->counter),
+                   "count_splats");
+  unsigned long long v = 0;
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(&v, c->counter, sizeof v, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (!rc) *splats = v;
+  return rc;
+}
+
 int mlgi_ism_pullpush(mlgi_context* c, const mlgi_atlas* atlas, const mlgi_params* p,
                       mlgi_ism_depth* out) {
   if (!c) return MLGI_ERR_INVALID_ARGUMENT;
@@ -488,6 +518,9 @@ int mlgi_ism_pullpush(mlgi_context* c, const mlgi_atlas* atlas, const mlgi_param
   if (p->pull_push_levels < 1 || p->pull_push_levels > log2i(s))
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "pull-push levels %d out of range 1..%d",
                 p->pull_push_levels, log2i(s));
+  // the kernel reads the atlas in 16-byte and writes the ISM in 8-byte units
+  if (((uintptr_t)atlas->texels & 15u) != 0 || ((uintptr_t)out->texels & 7u) != 0)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas must be 16-byte and ISM 8-byte aligned");
   out->ism_side = s;
   Scope sc(c, kPullPush);
   return hip_check(c,
@@ -561,6 +594,7 @@ int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
   a->clamp_value = p->vpl_clamping_value;
   a->out_scale = p->gi_intensity_factor / (float)(vpl_end - p->vpl_start_index) * (float)classes;
   a->shadowing = p->enable_shadowing ? 1 : 0;
+  a->cull = p->disable_culling ? 0 : 1;
   return MLGI_OK;
 }
 
@@ -628,6 +662,7 @@ int mlgi_frame(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_points* pts,
 struct mlgi_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  void* splat_ws = nullptr;  // a captured frame's own splat work area
 };
 
 extern "C" {
@@ -642,33 +677,48 @@ int mlgi_graph_capture_frame(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_po
   int rc = check_vpls(c, vpls);
   if (rc) return rc;
   if (!p) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "params is NULL");
-  // everything that allocates or copies happens before the capture
+  if (!pts || pts->count < 0) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "points is NULL or negative");
+  // everything that allocates or copies happens before the capture: the
+  // shuffle table, and the graph's own splat work area (the context's
+  // shared one may be reallocated by a later eager splat, so a graph never
+  // refers to it)
   const int32_t* sh = nullptr;
   if (p->shuffle_lights && (rc = get_shuffle(c, vpls->count, &sh))) return rc;
-  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"))) return rc;
+  auto* out_graph = new mlgi_graph();
+  const size_t ws_bytes = mlgi::splat_work_bytes((uint32_t)pts->count);
+  if ((rc = hip_check(c, hipMalloc(&out_graph->splat_ws, ws_bytes), "hipMalloc(graph splat work)"))) {
+    mlgi_graph_destroy(out_graph);
+    return rc;
+  }
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"))) {
+    mlgi_graph_destroy(out_graph);
+    return rc;
+  }
   const bool prof = c->profile;
   c->profile = false;  // no event nodes in the graph
   rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
                  "hipStreamBeginCapture");
   if (rc) {
     c->profile = prof;
+    mlgi_graph_destroy(out_graph);
     return rc;
   }
+  c->capture_ws = out_graph->splat_ws;
+  c->capture_ws_bytes = ws_bytes;
   const int frame_rc = mlgi_frame(c, rsm, pts, gb, p, vpls, atlas, ism, out);
+  c->capture_ws = nullptr;
+  c->capture_ws_bytes = 0;
   hipGraph_t g = nullptr;
   const hipError_t end = hipStreamEndCapture(c->stream, &g);
   c->profile = prof;
+  out_graph->graph = g;
   if (frame_rc) {
-    if (g) (void)hipGraphDestroy(g);
+    mlgi_graph_destroy(out_graph);
     return frame_rc;
   }
-  if ((rc = hip_check(c, end, "hipStreamEndCapture"))) return rc;
-  auto* out_graph = new mlgi_graph();
-  out_graph->graph = g;
-  if ((rc = hip_check(c, hipGraphInstantiate(&out_graph->exec, g, nullptr, nullptr, 0),
-                      "hipGraphInstantiate"))) {
-    (void)hipGraphDestroy(g);
-    delete out_graph;
+  if ((rc = hip_check(c, end, "hipStreamEndCapture")) ||
+      (rc = hip_check(c, hipGraphInstantiate(&out_graph->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate"))) {
+    mlgi_graph_destroy(out_graph);
     return rc;
   }
   *graph = out_graph;
@@ -688,6 +738,8 @@ void mlgi_graph_destroy(mlgi_graph* graph) {
   if (!graph) return;
   if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
   if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  // hipFree waits for the device's queued work, so no replay still uses it
+  if (graph->splat_ws) (void)hipFree(graph->splat_ws);
   delete graph;
 }
 
@@ -785,6 +837,10 @@ int batch_frame(mlgi_context* c, const mlgi_frame_desc& d, mlgi::BatchFrame* f) 
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "point bins (%d) must equal the VPL count (%d)", pts.n_bins,
                 d.vpls.count);
   if (!d.atlas.texels || !d.ism.texels) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas / ism is NULL");
+  // the batched tile pass writes and the pull-push reads the atlas in 16-byte
+  // units; the pull-push writes the ISM in 8-byte units
+  if (((uintptr_t)d.atlas.texels & 15u) != 0 || ((uintptr_t)d.ism.texels & 7u) != 0)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas must be 16-byte and ISM 8-byte aligned");
   int n1d, s;
   if ((rc = atlas_geometry(c, d.atlas.side, d.vpls.count, &n1d, &s))) return rc;
   if ((d.atlas.ism_side != 0 && d.atlas.ism_side != s) || d.ism.side != d.atlas.side ||

@@ -36,6 +36,12 @@ struct mlgi_context {
   // work area of the single-frame ISM-major splat (grown on demand)
   void* splat_ws = nullptr;
   size_t splat_ws_bytes = 0;
+  int64_t splat_ws_points = -1;  // points of the last ISM-major splat in splat_ws (-1: none)
+  // While mlgi_graph_capture_frame records a frame: that graph's own splat
+  // work area, used instead of splat_ws (which a later eager splat may
+  // reallocate while the graph still refers to it).
+  void* capture_ws = nullptr;
+  size_t capture_ws_bytes = 0;
   // recorded by mlgi_context_wait when another context waits for this one
   hipEvent_t order_event = nullptr;
 };

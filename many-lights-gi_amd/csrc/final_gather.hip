@@ -196,9 +196,12 @@ __device__ __forceinline__ bool cull_keep(const VplLds& v, const WaveBounds& w) 
 }
 
 // Survivor mask of the `count` VPLs of a chunk, lane j testing VPL j, from
-// the chunk's records in global memory (flux as floats).
+// the chunk's records in global memory (flux as floats). With culling off
+// (a wave-uniform flag) every VPL of the chunk survives.
 template <bool kMixed>
-__device__ __forceinline__ unsigned long long cull_chunk(const float* crec, int count, const WaveBounds& w) {
+__device__ __forceinline__ unsigned long long cull_chunk(const float* crec, int count, const WaveBounds& w,
+                                                         int cull) {
+  if (!cull) return count >= 64 ? ~0ull : ((1ull << count) - 1ull);
   const int lane = threadIdx.x & 63;
   bool keep = false;
   if (lane < count) keep = cull_keep<kMixed>(load_vpl(crec + (size_t)lane * kVplFloats), w);
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
     const int count = min(64, a.per_class - chunk);
     const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
     if (!wave_live) continue;
-    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb);
+    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb, a.cull);
     while (mask) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
@@ -505,7 +508,7 @@ __device__ __forceinline__ void batch_item_sums(const GatherArgs& a, const Pixel
   for (int chunk = 0; chunk < a.per_class; chunk += 64) {
     const int count = min(64, a.per_class - chunk);
     const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb);
+    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb, a.cull);
     while (mask) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;

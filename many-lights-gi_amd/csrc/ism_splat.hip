@@ -31,6 +31,8 @@
 //     result is the same.
 // Tiles are mapped to XCDs in contiguous ranges, so the bins that
 // neighbouring ISMs share are read through one L2.
+#include <algorithm>
+
 #include "batch_device.hpp"
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
@@ -565,6 +567,29 @@ hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !any) return e;
   hipLaunchKernelGGL(ism_spill_kernel, dim3(n_bins < 2048 ? n_bins : 2048), dim3(256), 0, s, j, n_bins);
+  return hipGetLastError();
+}
+
+namespace {
+// Sum over points of the number of set bits in each point's candidate mask:
+// the (point, ISM) splats of the frame, one LDS atomic min each in the tile
+// pass. One wave-level reduction and one global atomic per workgroup.
+__global__ __launch_bounds__(256) void count_splats_kernel(const uint16_t* __restrict__ mask, uint32_t n,
+                                                           unsigned long long* counter) {
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
+    c += (uint32_t)__builtin_popcount((uint32_t)gload(mask + i));
+  unsigned long long v = c;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) __hip_atomic_fetch_add(counter, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+}  // namespace
+
+hipError_t launch_count_splats(hipStream_t s, const SplatWork& w, uint32_t n_points, unsigned long long* counter) {
+  if (n_points == 0) return hipSuccess;
+  const uint32_t blocks = std::min<uint32_t>((n_points + 255) / 256, 2048u);
+  hipLaunchKernelGGL(count_splats_kernel, dim3(blocks), dim3(256), 0, s, w.mask, n_points, counter);
   return hipGetLastError();
 }
 

@@ -44,6 +44,9 @@ struct SplatWork {
 bool ism_major_supported(int ism_side);
 size_t splat_work_bytes(uint32_t n_points);
 SplatWork splat_work(void* base, uint32_t n_points);
+// Adds to *counter the number of set candidate bits over the first n_points
+// masks of a work area: the splats (LDS atomic mins) of that ISM-major splat.
+hipError_t launch_count_splats(hipStream_t s, const SplatWork& w, uint32_t n_points, unsigned long long* counter);
 hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint32_t* bin_offsets, int n_bins,
                                   int n_points, const float* vpl_records, int n_vpl, int n1d, int ism_side,
                                   int atlas_side, float z_far, uint32_t* atlas, const SplatWork& w);
@@ -74,6 +77,7 @@ struct GatherArgs {
   float clamp_value;
   float out_scale;   // giIntensityFactor / vplCount * I^2
   int shadowing;
+  int cull;          // 1: per-wave VPL culling; 0: every VPL of the class is evaluated
   float* out;
 };
 hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision, int tile_w,

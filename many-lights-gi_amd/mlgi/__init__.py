@@ -445,6 +445,12 @@ class ImperfectShadowmap:
         self.ctx.check(lib.mlgi_splat_spill_stats(self.ctx.h, C.byref(n), C.byref(o)), "mlgi_splat_spill_stats")
         return int(n.value), bool(o.value)
 
+    def splat_count(self) -> int:
+        """(point, ISM) splats of the context's last ISM-major splat (mlgi_splat_count)."""
+        n = C.c_uint64(0)
+        self.ctx.check(lib.mlgi_splat_count(self.ctx.h, C.byref(n)), "mlgi_splat_count")
+        return int(n.value)
+
     def pullpush(self, params: Params) -> None:
         self.ctx.check(lib.mlgi_ism_pullpush(self.ctx.h, C.byref(self.atlas_desc()), C.byref(params),
                                              C.byref(self.depth_desc())), "mlgi_ism_pullpush")

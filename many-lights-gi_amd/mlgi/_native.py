@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("tile_h", C.c_int32),
         ("gather_fetch", C.c_int32),
         ("splat_block", C.c_int32),
+        ("disable_culling", C.c_int32),
     ]
 
 
@@ -191,6 +192,7 @@ SIGNATURES = {
     "mlgi_vpl_extract": (C.c_int, [vp, P(Rsm), P(Params), P(Vpls)]),
     "mlgi_ism_splat": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas)]),
     "mlgi_splat_spill_stats": (C.c_int, [vp, P(C.c_uint32), P(C.c_int32)]),
+    "mlgi_splat_count": (C.c_int, [vp, P(C.c_uint64)]),
     "mlgi_probe_valu": (C.c_int, [vp, P(C.c_double)]),
     "mlgi_deferred_composite": (C.c_int, [vp, P(CompositeInputs), P(CompositeParams), P(Image)]),
     "mlgi_ism_pullpush": (C.c_int, [vp, P(Atlas), P(Params), P(IsmDepth)]),

@@ -59,6 +59,8 @@ _sigs = {
     "oracle_ism_splat_f64": (None, [vp, vp, i32, vp, vp, i32, i32, f32, vp]),
     "oracle_pullpush_f32": (None, [vp, i32, i32, i32, f32, vp]),
     "oracle_pullpush_f64": (None, [vp, i32, i32, i32, f32, vp, vp]),
+    "oracle_ism_splat_mt_f32": (None, [vp, vp, i32, vp, vp, i32, i32, f32, vp, i32]),
+    "oracle_pullpush_mt_f32": (None, [vp, i32, i32, i32, f32, vp, i32]),
     "oracle_final_gather_f64": (None, [vp, vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, vp, i32, f32,
                                        f32, f32, i32, i32, i32, vp, vp]),
     "oracle_final_gather_f32": (None, [vp, vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, vp, i32, f32,
@@ -164,23 +166,32 @@ def vpl_extract(diffuse, normal, depth, inv_bvp, light_intensity, shuffle, n, dt
     return pos, nrm, pk, col
 
 
-def ism_splat(points, bin_offsets, vpl_pos, vpl_nrm, atlas_side, z_far, dtype=np.float32):
+def ism_splat(points, bin_offsets, vpl_pos, vpl_nrm, atlas_side, z_far, dtype=np.float32, threads=1):
+    """threads > 1 (FP32 only): bins on that many host threads, same bits."""
     pts = _c(points, np.float32)
     off = _c(bin_offsets, np.uint32)
     vpos, vnrm = _c(vpl_pos, np.float32), _c(vpl_nrm, np.float32)
     atlas = np.empty((atlas_side, atlas_side), np.uint32)
+    if threads > 1 and dtype == np.float32:
+        _lib.oracle_ism_splat_mt_f32(_p(pts), _p(off), len(off) - 1, _p(vpos), _p(vnrm), len(vpos), atlas_side,
+                                     z_far, _p(atlas), threads)
+        return atlas
     fn = _lib.oracle_ism_splat_f32 if dtype == np.float32 else _lib.oracle_ism_splat_f64
     fn(_p(pts), _p(off), len(off) - 1, _p(vpos), _p(vnrm), len(vpos), atlas_side, z_far, _p(atlas))
     return atlas
 
 
-def pullpush(atlas, ism_side, levels, z_far, dtype=np.float32):
-    """Returns the R16 result (and, for FP64, the unquantised depth)."""
+def pullpush(atlas, ism_side, levels, z_far, dtype=np.float32, threads=1):
+    """Returns the R16 result (and, for FP64, the unquantised depth).
+    threads > 1 (FP32 only): rows on that many host threads, same bits."""
     a = _c(atlas, np.uint32)
     side = a.shape[0]
     out = np.empty((side, side), np.uint16)
     if dtype == np.float32:
-        _lib.oracle_pullpush_f32(_p(a), side, ism_side, levels, z_far, _p(out))
+        if threads > 1:
+            _lib.oracle_pullpush_mt_f32(_p(a), side, ism_side, levels, z_far, _p(out), threads)
+        else:
+            _lib.oracle_pullpush_f32(_p(a), side, ism_side, levels, z_far, _p(out))
         return out
     dep = np.empty((side, side), np.float64)
     _lib.oracle_pullpush_f64(_p(a), side, ism_side, levels, z_far, _p(out), _p(dep))

@@ -64,6 +64,31 @@ template <class T> inline uint32_t to_uint(T v) {
   return (uint32_t)v;
 }
 
+// Runs f(row) for every row in [0, n) on `threads` threads (rows taken from
+// a shared counter). Rows must be independent.
+template <class F> void parallel_rows(int n, int threads, F f) {
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    for (int r = next.fetch_add(1); r < n; r = next.fetch_add(1)) f(r);
+  };
+  const int nt = std::max(1, std::min(threads, n));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+}
+
+// dst = min(dst, v), atomically when other threads update the same array.
+inline void store_min(uint32_t* dst, uint32_t v, bool atomic) {
+  if (!atomic) {
+    if (v < *dst) *dst = v;
+    return;
+  }
+  uint32_t cur = __atomic_load_n(dst, __ATOMIC_RELAXED);
+  while (v < cur && !__atomic_compare_exchange_n(dst, &cur, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+}
+
 // ---------------------------------------------------------------- packing
 // floatpacking.glsl:26-38 pack3UNToFloat, kept as uint32 bits.
 template <class T> uint32_t pack3un(T x, T y, T z) {
@@ -257,14 +282,17 @@ void vpl_extract(const float* diffuse, const float* normal, const float* depth, 
 
 // ------------------------------------------------------------- ISM splat
 // ism.comp:54-139 (usePushPull branch, scaleISMs = pointsOnlyIntoScaledISMs = false)
+// The atlas is a per-texel minimum, so the bins may run on `threads`
+// threads in any order (atomic min) with the same result.
 template <class T>
 void ism_splat(const float* points, const uint32_t* bin_off, int n_bins, const float* vpos,
-               const float* vnrm, int n_vpl, int atlas_side, float z_far, uint32_t* atlas) {
+               const float* vnrm, int n_vpl, int atlas_side, float z_far, uint32_t* atlas, int threads = 1) {
   const int max_test = 16, max_collect = 4;
   std::fill(atlas, atlas + (size_t)atlas_side * atlas_side, 0xFFFFFFFFu);
   const int n1d = ism_indices_1d(n_vpl);
   const T zf = T(z_far);
-  for (int b = 0; b < n_bins; ++b) {
+  const bool atomic = threads > 1;
+  parallel_rows(n_bins, threads, [&](int b) {
     int ids[max_test];
     V3<T> cp[max_test], cn[max_test];
     for (int i = 0; i < max_test; ++i) {
@@ -304,11 +332,10 @@ void ism_splat(const float* points, const uint32_t* bin_off, int n_bins, const f
         // ivec2(v.xy): truncation toward zero
         long long tx = (long long)std::trunc(vx), ty = (long long)std::trunc(vy);
         if (tx < 0 || ty < 0 || tx >= atlas_side || ty >= atlas_side) continue;
-        uint32_t& dst = atlas[(size_t)ty * atlas_side + tx];
-        if (val < dst) dst = val;
+        store_min(&atlas[(size_t)ty * atlas_side + tx], val, atomic);
       }
     }
-  }
+  });
 }
 
 // ------------------------------------------------------------- pull-push
@@ -320,12 +347,12 @@ template <class T> struct Texel {
 // pull.comp:20-126, one level. lvl = level written (1..L).
 template <class T>
 void pull_level(const uint32_t* atlas, const std::vector<Texel<T>>* prev, std::vector<Texel<T>>& out,
-                int out_side, int lvl, int ism_side, T z_far) {
+                int out_side, int lvl, int ism_side, T z_far, int threads) {
   static const int offs[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
   const int in_side = out_side * 2;
   const T scale = std::ldexp(T(1.0), -lvl);
   out.assign((size_t)out_side * out_side, Texel<T>{T(1), T(0), T(0), 0u});
-  for (int oy = 0; oy < out_side; ++oy) {
+  parallel_rows(out_side, threads, [&](int oy) {
     for (int ox = 0; ox < out_side; ++ox) {
       T ds[4], mds[4], rs[4], dx[4], dy[4];
       bool valid[4];
@@ -391,7 +418,7 @@ void pull_level(const uint32_t* atlas, const std::vector<Texel<T>>* prev, std::v
         o.disp = pack_half2<T>(xacc / fn, yacc / fn);
       }
     }
-  }
+  });
 }
 
 inline int log2i(int v) {
@@ -402,10 +429,12 @@ inline int log2i(int v) {
 
 // push.comp:44-165, one level. lvl = level written (L-1..0).
 // same: pull level lvl (lvl >= 1); atlas for lvl == 0. coarse: level lvl+1.
+// Invocation row gy writes fine rows 2 gy - 1 and 2 gy only, so the rows of
+// invocations run on `threads` threads.
 template <class T>
 void push_level(const uint32_t* atlas, const std::vector<Texel<T>>* same,
                 const std::vector<Texel<T>>& coarse, std::vector<Texel<T>>* out,
-                uint16_t* out_r16, T* out_depth, int side, int lvl, int ism_side) {
+                uint16_t* out_r16, T* out_depth, int side, int lvl, int ism_side, int threads) {
   static const int offs[4][2] = {{0, 0}, {0, 1}, {1, 1}, {1, 0}};
   static const int wx[4] = {9, 3, 1, 3};
   const int cside = side / 2;
@@ -413,7 +442,7 @@ void push_level(const uint32_t* atlas, const std::vector<Texel<T>>* same,
   const T scale = std::ldexp(T(1.0), -lvl);
   if (out) out->assign((size_t)side * side, Texel<T>{T(1), T(0), T(0), 0u});
   const int groups = side / 2 + 1;  // invocations per axis that touch [0, side)
-  for (int gy = 0; gy < groups; ++gy) {
+  parallel_rows(groups, threads, [&](int gy) {
     for (int gx = 0; gx < groups; ++gx) {
       int clx = gx - 1, cly = gy - 1;
       T cd[4], cmd[4], cr[4], cdx[4], cdy[4];
@@ -505,23 +534,23 @@ void push_level(const uint32_t* atlas, const std::vector<Texel<T>>* same,
         }
       }
     }
-  }
+  });
 }
 
 template <class T>
 void pullpush(const uint32_t* atlas, int atlas_side, int ism_side, int levels, float z_far,
-              uint16_t* out_r16, T* out_depth) {
+              uint16_t* out_r16, T* out_depth, int threads = 1) {
   std::vector<std::vector<Texel<T>>> pull(levels + 1);
   for (int l = 1; l <= levels; ++l)
     pull_level<T>(atlas, l > 1 ? &pull[l - 1] : nullptr, pull[l], atlas_side >> l, l, ism_side,
-                  T(z_far));
+                  T(z_far), threads);
   std::vector<Texel<T>> coarse = pull[levels], cur;
   for (int l = levels - 1; l >= 0; --l) {
     if (l > 0) {
-      push_level<T>(atlas, &pull[l], coarse, &cur, nullptr, nullptr, atlas_side >> l, l, ism_side);
+      push_level<T>(atlas, &pull[l], coarse, &cur, nullptr, nullptr, atlas_side >> l, l, ism_side, threads);
       coarse.swap(cur);
     } else {
-      push_level<T>(atlas, nullptr, coarse, nullptr, out_r16, out_depth, atlas_side, 0, ism_side);
+      push_level<T>(atlas, nullptr, coarse, nullptr, out_r16, out_depth, atlas_side, 0, ism_side, threads);
     }
   }
 }
@@ -862,6 +891,13 @@ void oracle_ism_splat_f64(const float* p, const uint32_t* bo, int nb, const floa
 
 void oracle_pullpush_f32(const uint32_t* a, int side, int s, int l, float zf, uint16_t* o) {
   pullpush<float>(a, side, s, l, zf, o, nullptr);
+}
+void oracle_ism_splat_mt_f32(const float* p, const uint32_t* bo, int nb, const float* vp, const float* vn, int nv,
+                             int side, float zf, uint32_t* atlas, int threads) {
+  ism_splat<float>(p, bo, nb, vp, vn, nv, side, zf, atlas, threads);
+}
+void oracle_pullpush_mt_f32(const uint32_t* a, int side, int s, int l, float zf, uint16_t* o, int threads) {
+  pullpush<float>(a, side, s, l, zf, o, nullptr, threads);
 }
 void oracle_pullpush_f64(const uint32_t* a, int side, int s, int l, float zf, uint16_t* o,
                          double* od) {

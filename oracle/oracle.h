@@ -91,6 +91,15 @@ void oracle_pullpush_f32(const uint32_t* atlas, int atlas_side, int ism_side, in
 void oracle_pullpush_f64(const uint32_t* atlas, int atlas_side, int ism_side, int levels,
                          float z_far, uint16_t* out_r16, double* out_depth);
 
+/* The FP32 splat and pull-push on `threads` host threads (bins / rows in
+ * parallel; the same bits as the single-threaded functions), for the CPU
+ * baseline's timing. */
+void oracle_ism_splat_mt_f32(const float* points, const uint32_t* bin_offsets, int n_bins,
+                             const float* vpl_pos, const float* vpl_nrm, int n_vpl,
+                             int atlas_side, float z_far, uint32_t* atlas, int threads);
+void oracle_pullpush_mt_f32(const uint32_t* atlas, int atlas_side, int ism_side, int levels,
+                            float z_far, uint16_t* out_r16, int threads);
+
 /* ---- final gathering (final_gathering.comp:63-135), brute-force class lists ----
  * depth: w*h window depth; normal: w*h*3 encoded n*0.5+0.5.
  * vp_inv: column-major inverse(P*V) of the camera.

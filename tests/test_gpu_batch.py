@@ -1,14 +1,15 @@
-"""GPU: persistent batched frames.
+"""GPU: batched frames.
 
 A batch of frames with different G-buffer sizes, VPL counts, ISM sides,
 interleaves, gather strips, light poses and both precisions runs as one set
-of persistent launches. Every frame's VPL records, atlas and unorm16 ISM
-must be bit-identical to the same frame run alone through the single-frame
-path, and its irradiance within rel-L2 1e-5 of it (the batched gather sums
-compacted lit pairs: the same terms in the same order, but each product
-rounded before its add); every output must be bit-identical to the frame
-run as a batch of one, to a second run, and to replays of the captured batch
-graph; one frame is also checked against the oracle.
+of launches, one per stage (one workgroup per work item of all frames).
+Every frame's VPL records, atlas and unorm16 ISM must be bit-identical to
+the same frame run alone through the single-frame path, and its irradiance
+within rel-L2 1e-5 of it (the batched gather runs one wave per strip, the
+single-frame kernel one workgroup per tile, so the FP32 forms may contract
+differently); every output must be bit-identical to the frame run as a
+batch of one, to a second run, and to replays of the captured batch graph;
+one frame is also checked against the oracle.
 """
 import numpy as np
 import pytest

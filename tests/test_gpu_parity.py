@@ -54,6 +54,16 @@ class Frame:
         r = self.rec
         return r[:, 0:3], r[:, 4:7], r[:, 16:19]
 
+    def reference(self):
+        """FP64 irradiance and lit-pair count of the GPU's VPLs and ISM (computed once)."""
+        if getattr(self, "_ref", None) is None:
+            pos, nrm, col = self.vpl
+            p = self.params
+            self._ref = oracle.final_gather(self.inp.gb_depth, self.inp.gb_normal, self.inp.vp_inv, pos, nrm, col,
+                                            self.r16, self.wl.atlas_side, p.z_far, p.gi_intensity_factor,
+                                            p.vpl_clamping_value, self.wl.interleave)
+        return self._ref
+
 
 @pytest.fixture(scope="module", params=["tiny", "small", "1080p"])
 def frame(request, ctx):
@@ -99,11 +109,7 @@ def test_pullpush_bit_exact(frame):
 
 
 def test_final_gather_fp32_vs_fp64(frame):
-    pos, nrm, col = frame.vpl
-    p = frame.params
-    ref, lit = oracle.final_gather(frame.inp.gb_depth, frame.inp.gb_normal, frame.inp.vp_inv, pos, nrm, col,
-                                   frame.r16, frame.wl.atlas_side, p.z_far, p.gi_intensity_factor,
-                                   p.vpl_clamping_value, frame.wl.interleave)
+    ref, lit = frame.reference()
     assert lit > 0
     err = rel_l2(frame.img, ref)
     print(f"[fg] {frame.wl.name} rel-L2 {err:.3e} lit pairs {lit}")
@@ -111,8 +117,6 @@ def test_final_gather_fp32_vs_fp64(frame):
 
 
 def test_final_gather_mixed_vs_fp64(frame, ctx):
-    pos, nrm, col = frame.vpl
-    p = frame.params
     st = frame.stage
     st.params.precision = mlgi.PRECISION_MIXED
     try:
@@ -121,9 +125,7 @@ def test_final_gather_mixed_vs_fp64(frame, ctx):
         img = st.gi_buffer.numpy()
     finally:
         st.params.precision = mlgi.PRECISION_FP32
-    ref, _ = oracle.final_gather(frame.inp.gb_depth, frame.inp.gb_normal, frame.inp.vp_inv, pos, nrm, col,
-                                 frame.r16, frame.wl.atlas_side, p.z_far, p.gi_intensity_factor,
-                                 p.vpl_clamping_value, frame.wl.interleave)
+    ref, _ = frame.reference()
     err = rel_l2(img, ref)
     per_channel = [rel_l2(img[..., c], ref[..., c]) for c in range(3)]
     pix = np.linalg.norm(img - ref, axis=-1) / np.maximum(np.linalg.norm(ref, axis=-1), 1e-30)
@@ -133,6 +135,35 @@ def test_final_gather_mixed_vs_fp64(frame, ctx):
     assert err <= REL_L2_MIXED
     assert max(per_channel) <= REL_L2_MIXED
     assert not np.array_equal(img, frame.img)  # the mixed kernel did run
+
+
+@pytest.mark.parametrize("name,prec,bar", [("fp32", mlgi.PRECISION_FP32, REL_L2_FP32),
+                                            ("mixed", mlgi.PRECISION_MIXED, REL_L2_MIXED)])
+def test_batched_path_vs_fp64(frame, ctx, name, prec, bar):
+    """The batched kernels (the benchmark's headline path) on the one frame:
+    VPLs, atlas and ISM bit-identical to the stage-wise path, irradiance
+    within the precision's bar of the FP64 oracle."""
+    st = frame.stage
+    st.params.precision = prec
+    try:
+        b = mlgi.Batch(ctx, [(st, frame.rsm, frame.pts, frame.gb)])
+        st.gi_buffer.copy_from(np.full((frame.wl.height, frame.wl.width, 3), -1.0, np.float32))
+        b.run()
+        ctx.sync()
+        b.close()
+        rec, atlas, r16, img = (st.vpl_processor.host_records(), st.ism.softrender.numpy(),
+                                st.ism.pushpull_result.numpy(), st.gi_buffer.numpy())
+    finally:
+        st.params.precision = mlgi.PRECISION_FP32
+    assert np.array_equal(rec, frame.rec)
+    assert np.array_equal(atlas, frame.atlas)
+    assert np.array_equal(r16, frame.r16)
+    ref, _ = frame.reference()
+    err = rel_l2(img, ref)
+    per_channel = [rel_l2(img[..., c], ref[..., c]) for c in range(3)]
+    print(f"[fg batched {name}] {frame.wl.name} rel-L2 {err:.3e} per channel "
+          f"{['%.3e' % e for e in per_channel]}")
+    assert err <= bar and max(per_channel) <= bar
 
 
 def test_frame_matches_stages(frame, ctx):

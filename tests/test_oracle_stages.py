@@ -267,3 +267,22 @@ def test_gather_fp32_twin_vs_fp64():
     b, lb = oracle.final_gather(depth, nrm, m, vpos, vn, col, ism, 64, ZF, 3000.0, 1e-3, 2, dtype=np.float32)
     assert la == lb > 0
     assert np.linalg.norm(b - a) / np.linalg.norm(a) < 1e-5
+
+
+def test_threaded_splat_and_pullpush_match_single_thread():
+    # the CPU baseline runs these on many threads; the atlas is a minimum and
+    # the pull-push rows are independent, so the bits must not change
+    rng = np.random.default_rng(5)
+    n = 256
+    vpos = np.column_stack([rng.uniform(1, 9, n), rng.uniform(1, 9, n), rng.uniform(1, 9, n)]).astype(np.float32)
+    vn = np.array([quantised_normal(v / np.linalg.norm(v)) for v in rng.normal(size=(n, 3))], np.float32)
+    pts = [pack_point(rng.uniform(1, 9, 3), rng.normal(size=3) / 1.8, rng.uniform(0.05, 0.5)) for _ in range(30000)]
+    pts = np.array(pts, np.float32)
+    bins = rng.integers(0, n, len(pts))
+    order = np.argsort(bins, kind="stable")
+    off = np.concatenate([[0], np.cumsum(np.bincount(bins, minlength=n))]).astype(np.uint32)
+    a1 = oracle.ism_splat(pts[order], off, vpos, vn, 512, ZF)
+    a8 = oracle.ism_splat(pts[order], off, vpos, vn, 512, ZF, threads=8)
+    assert (a1 != 0xFFFFFFFF).sum() > 1000
+    assert np.array_equal(a1, a8)
+    assert np.array_equal(oracle.pullpush(a1, 32, 4, ZF), oracle.pullpush(a1, 32, 4, ZF, threads=8))
