@@ -206,6 +206,22 @@ int mlgi_vpl_extract(mlgi_context* ctx, const mlgi_rsm* rsm, const mlgi_params* 
  * demand (growing it drains the context stream). */
 int mlgi_ism_splat(mlgi_context* ctx, const mlgi_points* pts, const mlgi_vpls* vpls,
                    const mlgi_params* p, mlgi_atlas* atlas);
+/* mlgi_ism_splat that allocates nothing: no hipMalloc, hipFree, stream
+ * synchronisation or profiling event. No counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * kernels and bit-exact atlas as mlgi_ism_splat; the splat's work area
+ * (candidate masks, spill list and counters) is a static array in device
+ * memory, one per GPU, held until the process ends (16.1 MiB), sized for at
+ * most MLGI_NOALLOC_MAX_POINTS points: a larger point set is rejected with
+ * MLGI_ERR_INVALID_ARGUMENT. Every call on a GPU shares that GPU's array;
+ * the kernels reset what must start at zero (spill count and overflow flag)
+ * and write every mask before reading it, so consecutive calls need no
+ * clearing, but calls that overlap on one GPU (two contexts or streams of
+ * the same device) race and must be ordered by the caller. */
+#define MLGI_NOALLOC_MAX_POINTS 8388608
+int mlgi_ism_splat_noalloc(mlgi_context* ctx, const mlgi_points* pts, const mlgi_vpls* vpls,
+                           const mlgi_params* p, mlgi_atlas* atlas);
 /* Diagnostics of the context's last ISM-major splat (waits for the stream):
  * splats that landed outside their ISM's tile, and whether they exceeded the
  * spill list (then the point-major splat ran over the tiles). No reference

@@ -24,6 +24,8 @@
 
 using namespace mlgi_capi;
 
+static_assert(MLGI_NOALLOC_MAX_POINTS == mlgi::kScratchMaxPoints, "the ABI's limit is the static area's size");
+
 namespace mlgi_capi {
 
 int fail(mlgi_context* c, int code, const char* fmt, ...) {
@@ -195,6 +197,14 @@ int mlgi_context_create(int32_t device, mlgi_context** out) {
     (void)hipStreamDestroy(c->stream);
     delete c;
     return MLGI_ERR_OUT_OF_MEMORY;
+  }
+  // the device's static splat work area: its address on this device, with
+  // the device current (resolved once per device)
+  if (mlgi::splat_scratch_init(device, &c->scratch) != hipSuccess) {
+    (void)hipFree(c->counter);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MLGI_ERR_HIP;
   }
   *out = c;
   return MLGI_OK;
@@ -370,8 +380,18 @@ int mlgi_vpl_extract(mlgi_context* c, const mlgi_rsm* rsm, const mlgi_params* p,
                    "vpl_extract");
 }
 
-int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls,
-                   const mlgi_params* p, mlgi_atlas* atlas) {
+}  // extern "C"
+
+namespace mlgi_capi {
+
+// Work area of an ISM-major splat.
+enum SplatArea {
+  kAreaContext,  // the context's area, grown on demand (or a captured graph's own)
+  kAreaStatic    // the device's static area; no allocation, no profiling events
+};
+
+int ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls, const mlgi_params* p,
+              mlgi_atlas* atlas, SplatArea area) {
   if (!c) return MLGI_ERR_INVALID_ARGUMENT;
   int rc = check_params(c, p);
   if (rc || (rc = check_vpls(c, vpls))) return rc;
@@ -396,7 +416,9 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
     const uint32_t np = pts->count > 0 ? (uint32_t)pts->count : 0u;
     const size_t need = mlgi::splat_work_bytes(np);
     void* ws = nullptr;
-    if (c->capture_ws) {
+    if (area == kAreaStatic) {
+      ws = c->scratch;  // sized for kScratchMaxPoints (checked by the caller)
+    } else if (c->capture_ws) {
       if (c->capture_ws_bytes < need)
         return fail(c, MLGI_ERR_INVALID_ARGUMENT, "captured splat needs %zu work bytes, the graph holds %zu", need,
                     c->capture_ws_bytes);
@@ -406,7 +428,7 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
       ws = c->splat_ws;
       c->splat_ws_points = np;
     }
-    Scope sc(c, kSplat);
+    Scope sc(c, area == kAreaStatic ? -1 : kSplat);
     return hip_check(c,
                      mlgi::launch_ism_splat_tiled(c->stream, pts->records, pts->bin_offsets,
                                                   pts->count > 0 ? pts->n_bins : 0, pts->count, vpls->records,
@@ -415,18 +437,38 @@ int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpl
                      "ism_splat_tiled");
   }
   {
-    Scope sc(c, kClear);
+    Scope sc(c, area == kAreaStatic ? -1 : kClear);
     if ((rc = hip_check(c, mlgi::launch_atlas_clear(c->stream, atlas->texels, texels), "atlas_clear")))
       return rc;
   }
   if (pts->count == 0) return MLGI_OK;
-  Scope sc(c, kSplat);
+  Scope sc(c, area == kAreaStatic ? -1 : kSplat);
   return hip_check(c,
                    mlgi::launch_ism_splat(c->stream, pts->records, pts->bin_offsets, pts->n_bins,
                                           pts->count, vpls->records, vpls->count, n1d, atlas->side,
                                           p->z_far, atlas->texels,
                                           p->splat_block > 0 ? p->splat_block : 256),
                    "ism_splat");
+}
+
+}  // namespace mlgi_capi
+
+extern "C" {
+
+int mlgi_ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls, const mlgi_params* p,
+                   mlgi_atlas* atlas) {
+  return ism_splat(c, pts, vpls, p, atlas, kAreaContext);
+}
+
+int mlgi_ism_splat_noalloc(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls, const mlgi_params* p,
+                           mlgi_atlas* atlas) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (pts && (int64_t)pts->count > (int64_t)MLGI_NOALLOC_MAX_POINTS)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT,
+                "mlgi_ism_splat_noalloc: %d points exceed the static work area's maximum of %d", pts->count,
+                (int)MLGI_NOALLOC_MAX_POINTS);
+  if (!c->scratch) return fail(c, MLGI_ERR_HIP, "mlgi_ism_splat_noalloc: no static work area on this device");
+  return ism_splat(c, pts, vpls, p, atlas, kAreaStatic);
 }
 
 int mlgi_deferred_composite(mlgi_context* c, const mlgi_composite_inputs* in, const mlgi_composite_params* p,

@@ -42,6 +42,9 @@ struct mlgi_context {
   // reallocate while the graph still refers to it).
   void* capture_ws = nullptr;
   size_t capture_ws_bytes = 0;
+  // The device's static splat work area (mlgi_ism_splat_noalloc), resolved
+  // at context creation; shared by every context on the device.
+  void* scratch = nullptr;
   // recorded by mlgi_context_wait when another context waits for this one
   hipEvent_t order_event = nullptr;
 };
@@ -75,13 +78,14 @@ hipEvent_t tail_event(mlgi_context* c);
 // costs the stream a few microseconds between kernels. c->tail is cleared by
 // hip_check, through which every enqueue of the library goes, and when the
 // outermost scope ends, so a shared event never spans work enqueued in
-// between, nor anything a caller enqueues between two calls.
+// between, nor anything a caller enqueues between two calls. A negative
+// stage records nothing (callers that must create no events).
 struct Scope {
   mlgi_context* c;
   int stage;
   bool on;
   hipEvent_t a = nullptr;
-  Scope(mlgi_context* ctx, int st) : c(ctx), stage(st), on(ctx->profile) {
+  Scope(mlgi_context* ctx, int st) : c(ctx), stage(st), on(ctx->profile && st >= 0) {
     if (!on) return;
     ++c->scope_depth;
     a = tail_event(c);

@@ -32,6 +32,7 @@
 // Tiles are mapped to XCDs in contiguous ranges, so the bins that
 // neighbouring ISMs share are read through one L2.
 #include <algorithm>
+#include <mutex>
 
 #include "batch_device.hpp"
 #include "kernels.hpp"
@@ -508,6 +509,38 @@ bool ism_major_supported(int ism_side) { return ism_side >= 4 && ism_side <= kTi
 
 size_t splat_work_bytes(uint32_t n_points) {
   return 256 + (size_t)kSpillCap * sizeof(uint2) + (((size_t)n_points * 2 + 255) & ~(size_t)255);
+}
+
+// Static splat work area of the allocation-free splat entry point: one per
+// GPU, loaded with the code object and held until the process ends, sized
+// for kScratchMaxPoints points. Host code never passes `scratch` itself (its
+// host-side name is a stand-in, not a device address): splat_scratch_init
+// resolves each device's address once, and that address is what kernels get.
+constexpr size_t kScratchBytes = 256 + (size_t)kSpillCap * sizeof(uint2) + (size_t)kScratchMaxPoints * 2;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[kScratchDevices];  // per device, resolved once
+}  // namespace
+
+static_assert(kScratchBytes >= 256 + (size_t)kSpillCap * sizeof(uint2) + (((size_t)kScratchMaxPoints * 2 + 255) &
+                                                                           ~(size_t)255),
+              "the static area holds the work area of kScratchMaxPoints points");
+
+hipError_t splat_scratch_init(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // the current device's copy
+    if (e != hipSuccess) return e;
+    scratch_ptr[device] = p;
+  }
+  *out = scratch_ptr[device];
+  return hipSuccess;
 }
 
 SplatWork splat_work(void* base, uint32_t n_points) {

@@ -44,6 +44,11 @@ struct SplatWork {
 bool ism_major_supported(int ism_side);
 size_t splat_work_bytes(uint32_t n_points);
 SplatWork splat_work(void* base, uint32_t n_points);
+// The static work area of the allocation-free splat (ism_splat.hip), sized
+// for kScratchMaxPoints points. splat_scratch_init, called with `device`
+// current, returns that device's address of it (resolved once per device).
+constexpr uint32_t kScratchMaxPoints = 1u << 23;
+hipError_t splat_scratch_init(int device, void** out);
 // Adds to *counter the number of set candidate bits over the first n_points
 // masks of a work area: the splats (LDS atomic mins) of that ISM-major splat.
 hipError_t launch_count_splats(hipStream_t s, const SplatWork& w, uint32_t n_points, unsigned long long* counter);

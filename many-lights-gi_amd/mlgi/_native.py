@@ -23,6 +23,7 @@ PRECISION_FP32 = 0
 PRECISION_MIXED = 1
 
 VPL_RECORD_FLOATS = 32
+NOALLOC_MAX_POINTS = 8388608
 
 
 class Params(C.Structure):
@@ -191,6 +192,7 @@ SIGNATURES = {
     "mlgi_memcpy_dtod": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "mlgi_vpl_extract": (C.c_int, [vp, P(Rsm), P(Params), P(Vpls)]),
     "mlgi_ism_splat": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas)]),
+    "mlgi_ism_splat_noalloc": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas)]),
     "mlgi_splat_spill_stats": (C.c_int, [vp, P(C.c_uint32), P(C.c_int32)]),
     "mlgi_splat_count": (C.c_int, [vp, P(C.c_uint64)]),
     "mlgi_probe_valu": (C.c_int, [vp, P(C.c_double)]),
