@@ -30,8 +30,20 @@
 // The radius tests compare a correctly rounded square root with a float,
 // sqrt_rn(x) > R. Because sqrt(x) of a float x can never equal the midpoint m
 // between R and the next float (m has 25 significant bits, m^2 then has more
-// than 48, x has 24), that is exactly x > m^2, evaluated in FP64 where m and
-// m^2 are exact: the same decision as the FP32 twin for a fraction of the cost.
+// than 48, x has 24), that is exactly x > m^2, and for a float x exactly
+// x > RD(m^2), which rd_bound forms in FP32: the same decision as the FP32
+// twin without a square root, and no FP64 anywhere.
+//
+// Most texels of an atlas are empty: at 1080p 77% of the 2x2 blocks the
+// first pull level reads are all empty, and such a block's result is the
+// constant E_1. That level therefore runs in two passes: a cheap pass writes
+// E_1 for the empty blocks and appends the others to a list in LDS (ballot +
+// mbcnt), and the full arithmetic runs over the list, every wave iteration
+// full of real work. The list lives in the LDS of the coarser levels, which
+// are not written yet, so the workgroup's LDS does not grow. (The push
+// levels gain nothing from the same split: once level 1 is pushed, its
+// filled holes reach 88-90% of the finer quads, which then need the full
+// arithmetic anyway.)
 #include "batch_device.hpp"
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
@@ -121,26 +133,19 @@ __host__ __device__ inline void unpack_half2_hd(uint32_t u, float& x, float& y) 
   y = f16_to_f32((uint16_t)(u >> 16));
 }
 
-// m^2 for the test sqrt_rn(x) > R, R >= 0 finite (see the file header).
-__host__ __device__ inline double sqrt_gt_bound(float R) {
-  union {
-    float f;
-    uint32_t u;
-  } n;
-  n.f = R;
-  n.u += 1u;  // next float above R
-  const double m = ((double)R + (double)n.f) * 0.5;
-  return m * m;
-}
-
-// The same test against a float: for a float x and a real b, x > b exactly
-// when x > RD(b), the largest float <= b (a float above RD(b) is at least the
-// next float, which lies above b unless b is itself a float, RD(b) = b).
-__host__ __device__ inline float sqrt_gt_bound_f(float R) {
-  const double b = sqrt_gt_bound(R);
-  float f = (float)b;
-  if ((double)f > b) f = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) - 1u);  // f > 0 here
-  return f;
+// RD(m^2), m the midpoint between R >= 0 and the next float: the bound of
+// the exact test sqrt_rn(x) > R  <=>  x > RD(m^2) for a float x (see the file
+// header). With u = ulp(R), m^2 = R next(R) + u^2 / 4, where R next(R) is an
+// integer multiple of u^2 with 47 or 48 significant bits, so u^2 / 4 lies
+// below one ulp of it and RD(m^2) = RD(R next(R)): the rounded product, moved
+// down one float when its FMA error shows it was rounded up. For R = 0 the
+// product and the bound are 0. (Checked against the exact FP64 m^2 for every
+// float of [2^-16, 2^5) on the host.)
+__host__ __device__ inline float rd_bound(float R) {
+  const float n = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, R) + 1u);
+  const float p = R * n;
+  const float e = __builtin_fmaf(R, n, -p);
+  return e < 0.0f ? __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, p) - 1u) : p;
 }
 
 // x / b correctly rounded from y = RN(1 / b): q = RN(x y) lies within an ulp
@@ -186,8 +191,9 @@ __host__ __device__ inline Decoded decode_texel(uint32_t s, float r, float madd,
 __host__ __device__ inline Texel pull_combine(const float* ds, const float* mds, const float* rs,
                                               const float* dx, const float* dy, bool* valid) {
   // Branch-free selection: fminf(d, minimum) == d is d <= minimum (no NaNs
-  // here), and a term of an invalid input is +0, which leaves the sums'
-  // bits unchanged (none of them can be -0).
+  // here). A sum takes each term as fma(v, term, acc) with v = 1 or 0: the
+  // term itself, or +-0, which leaves the sum's bits unchanged (a sum starts
+  // at +0 and is never -0: an exact cancellation rounds to +0).
   float minimum = __builtin_inff();
   float max_depth = 0.0f;
   bool any = false;
@@ -203,10 +209,11 @@ __host__ __device__ inline Texel pull_combine(const float* ds, const float* mds,
   uint32_t nv = 0;
   for (int i = 0; i < 4; ++i) {
     const bool v = valid[i] && !(ds[i] > max_depth);
-    dacc = dacc + (v ? ds[i] : 0.0f);
-    xacc = xacc + (v ? dx[i] : 0.0f);
-    yacc = yacc + (v ? dy[i] : 0.0f);
-    racc = racc + (v ? rs[i] : 0.0f);
+    const float vf = v ? 1.0f : 0.0f;
+    dacc = __builtin_fmaf(vf, ds[i], dacc);
+    xacc = __builtin_fmaf(vf, dx[i], xacc);
+    yacc = __builtin_fmaf(vf, dy[i], yacc);
+    racc = __builtin_fmaf(vf, rs[i], racc);
     mvm = (v && mvm < mds[i]) ? mds[i] : mvm;
     nv += v ? 1u : 0u;
   }
@@ -279,7 +286,7 @@ __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, in
       rs[i] = tx.radius;
       // sqrt(dx^2 + dy^2) <= radius * scale, decided exactly without the root
       const float x = dx[i] * dx[i] + dy[i] * dy[i];
-      valid[i] = ds[i] != 1.0f && !((double)x > sqrt_gt_bound(rs[i] * scale));
+      valid[i] = ds[i] != 1.0f && !(x > rd_bound(rs[i] * scale));
     }
   }
   return pull_combine(ds, mds, rs, dx, dy, valid);
@@ -303,8 +310,7 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
   // offsets {0,0},{0,1},{1,1},{1,0} for both neighbours and fine pixels
   constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
   bool usable[4];
-  float cd[4], cmd[4], cr[4], cdx2[4], cdy2[4];
-  double bound[4];
+  float cd[4], cmd[4], cr[4], cdx2[4], cdy2[4], bound[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
@@ -320,7 +326,7 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
     cdx2[i] = dxh * 2.0f;
     cdy2[i] = dyh * 2.0f;
     usable[i] = inside && c.depth != 1.0f;
-    bound[i] = sqrt_gt_bound(c.radius * scale);
+    bound[i] = rd_bound(c.radius * scale);
   }
 #pragma unroll
   for (int op = 0; op < 4; ++op) {
@@ -334,7 +340,7 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
       ddx[i] = ((float)(2 * ox[i] - ox[op]) - 0.5f) + cdx2[i];
       ddy[i] = ((float)(2 * oy[i] - oy[op]) - 0.5f) + cdy2[i];
       const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
-      valid[i] = usable[i] && !((double)x > bound[i]);  // sqrt(x) <= radius * 2^-level
+      valid[i] = usable[i] && !(x > bound[i]);  // sqrt(x) <= radius * 2^-level
     }
     float minimum = 9001.0f, max_depth = 0.0f;
 #pragma unroll
@@ -353,17 +359,20 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
       occluded = occluded || (keep[i] && res.depth > cmd[i]);
     }
     if (any && (res.depth == 1.0f || occluded)) {
+      // kept terms as fma(k, term, acc), k = 1 or 0 (pull_combine); each
+      // weighted term w v is rounded first, as the reference's products are
       float wacc = 0.0f, dacc = 0.0f, macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
         const float w = (wt == 0) ? 9.0f : (wt == 2 ? 1.0f : 3.0f);
-        wacc = wacc + (keep[i] ? w : 0.0f);
-        dacc = dacc + (keep[i] ? cd[i] * w : 0.0f);
-        macc = macc + (keep[i] ? cmd[i] * w : 0.0f);
-        racc = racc + (keep[i] ? cr[i] * w : 0.0f);
-        xacc = xacc + (keep[i] ? ddx[i] * w : 0.0f);
-        yacc = yacc + (keep[i] ? ddy[i] * w : 0.0f);
+        const float k = keep[i] ? 1.0f : 0.0f;
+        wacc = __builtin_fmaf(k, w, wacc);
+        dacc = __builtin_fmaf(k, cd[i] * w, dacc);
+        macc = __builtin_fmaf(k, cmd[i] * w, macc);
+        racc = __builtin_fmaf(k, cr[i] * w, racc);
+        xacc = __builtin_fmaf(k, ddx[i] * w, xacc);
+        yacc = __builtin_fmaf(k, ddy[i] * w, yacc);
       }
       const float y = rcp[(int)wacc];
       res.depth = div_by_recip(dacc, wacc, y);
@@ -373,7 +382,7 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
     }
     // level 1 is read once more, by the depth-only level-0 push, which needs
     // its radius only as the bound of the distance test
-    if constexpr (l == 1) res.radius = sqrt_gt_bound_f(res.radius);
+    if constexpr (l == 1) res.radius = rd_bound(res.radius);
     same[t] = res;
   }
 }
@@ -381,7 +390,7 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
 // Level-0 push (push.comp:44-165 at level 0, depth only): quad (gx, gy) writes
 // the unorm16 depth of the fine pixels 2g - 1 + {0,1}^2 from the coarse
 // neighbours g - 1 + {0,1}^2 of level 1, whose radius slots hold
-// sqrt_gt_bound_f(radius) (level 0 scales the radius by 2^0). The weights of
+// rd_bound(radius) (level 0 scales the radius by 2^0). The weights of
 // push_quad become per-neighbour flags: w_i != 0 is `valid`, the weight that
 // survives the max-depth test is `keep`, and a fill needs an occluding kept
 // neighbour, so `wacc > 0 && occluded` is just `occluded`. Sums over the kept
@@ -449,7 +458,7 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
 }
 
 template <int S>
-constexpr int level_offset(int l) {  // texels of pull levels 1..l-1
+constexpr int level_offset(int l) {  // texels of pull levels 1..l-1 of one ISM
   int o = 0;
   for (int k = 1; k < l; ++k) o += (S >> k) * (S >> k);
   return o;
@@ -463,15 +472,34 @@ constexpr int level_offset(int l) {  // texels of pull levels 1..l-1
 template <int S>
 constexpr int kGroupMax = (64 / S) * (64 / S);
 
+// Pull levels in LDS, level-major: level l of every ISM of the group, then
+// level l + 1 (so the levels from l on form one contiguous range). Level l of
+// group member g starts at lv_offset<S>(l) + g (S >> l)^2.
+template <int S>
+constexpr int lv_offset(int l) {
+  return kGroupMax<S> * level_offset<S>(l);
+}
+
+// The two-pass form of a level needs a list of one 16-bit entry per output
+// (pull) or per quad (push) of the group, placed in the levels from `from`
+// on, which that level neither reads nor writes. Used where the list fits.
+template <int S, int L>
+constexpr bool list_fits(int entries, int from) {
+  return from <= L && entries * 2 <= (lv_offset<S>(L + 1) - lv_offset<S>(from)) * 16;
+}
+template <int S, int L>
+constexpr bool kCompactPull1 = list_fits<S, L>(kGroupMax<S> * (S / 2) * (S / 2), 2);
+
+
 // One workgroup's group of ISMs ism0 .. ism0 + groups - 1: tile g at
-// base + g * S * S, its pull levels at lv + g * lv_stride.
+// base + g * S * S, its pull levels in lv (lv_offset).
 struct Tables {
   uint32_t* base;
   Texel* lv;
-  int lv_stride;
   const float* rtab;
   const float* mtab;
   const float* rcp;
+  int* counts;  // list length of the two-pass pull level 1
   Decoded empty;
   float z_far;
   int ism0, n1d_log2, groups;
@@ -484,51 +512,73 @@ __device__ __forceinline__ int group_count(int groups) {
   return kGroupMax<S> == 1 ? 1 : groups;
 }
 
+// Appends v to an LDS list for the lanes with `take` set. Called by every
+// active lane of a wave together; in the strided loops below a wave's lane 0
+// holds its lowest index, so it is active whenever any lane is.
+__device__ __forceinline__ void list_append(uint16_t* list, int* count, bool take, int v) {
+  const unsigned long long m = __ballot(take);
+  if (m == 0ull) return;
+  int first = 0;
+  if ((threadIdx.x & 63) == 0) first = atomicAdd(count, __popcll(m));
+  first = __shfl(first, 0, 64);
+  const uint32_t below =
+      __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  if (take) list[first + (int)below] = (uint16_t)v;
+}
+
 template <int S, int l, int L>
 __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& pc) {
   if constexpr (l <= L) {
     constexpr int so = S >> l, n = so * so;
-    for (int t = threadIdx.x; t < group_count<S>(tb.groups) * n; t += kThreads) {
+    const int total = group_count<S>(tb.groups) * n;
+    auto full = [&](int t) {
       const int g = t / n, o = t - g * n;
-      const int ox = o % so, oy = o / so;
-      const uint32_t* base = tb.base + g * (S * S);
-      Texel* lv = tb.lv + g * tb.lv_stride;
-      Texel* dst = lv + level_offset<S>(l);
       const int ism = tb.ism0 + g;
       const int ism_x = ism & ((1 << tb.n1d_log2) - 1), ism_y = ism >> tb.n1d_log2;
-      if constexpr (l == 1) {
-        // a wave whose 2x2 blocks are all empty writes E_1
+      tb.lv[lv_offset<S>(l) + t] = pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.base + g * (S * S),
+                                                  tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.rtab, tb.mtab,
+                                                  tb.empty, tb.z_far);
+    };
+    if constexpr (l == 1 && kCompactPull1<S, L>) {
+      // pass 1: all-empty 2x2 blocks are E_1, the others go to the list
+      uint16_t* list = reinterpret_cast<uint16_t*>(tb.lv + lv_offset<S>(2));
+      for (int t = threadIdx.x; t < total; t += kThreads) {
+        const int g = t / n, o = t - g * n;
+        const int ox = o % so, oy = o / so;
+        const uint32_t* base = tb.base + g * (S * S);
         const uint2 a = *reinterpret_cast<const uint2*>(&base[(2 * oy) * S + 2 * ox]);
         const uint2 b = *reinterpret_cast<const uint2*>(&base[(2 * oy + 1) * S + 2 * ox]);
-        const bool all_empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
-        if (__ballot(!all_empty) == 0ull) {
-          dst[o] = pc.e1;
-          continue;
-        }
+        const bool empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
+        if (empty) tb.lv[lv_offset<S>(1) + t] = pc.e1;
+        list_append(list, &tb.counts[0], !empty, t);
       }
-      dst[o] = pull_one<S, l>(ox, oy, ism_x, ism_y, base, lv + level_offset<S>(l - 1), tb.rtab, tb.mtab,
-                              tb.empty, tb.z_far);
+      __syncthreads();
+      const int count = tb.counts[0];
+      for (int i = threadIdx.x; i < count; i += kThreads) full(list[i]);
+    } else {
+      for (int t = threadIdx.x; t < total; t += kThreads) full(t);
     }
     __syncthreads();
     pull_levels<S, l + 1, L>(tb, pc);
   }
 }
 
-template <int S, int l>
+template <int S, int l, int L>
 __device__ __forceinline__ void push_levels(const Tables& tb) {
   if constexpr (l >= 0) {
-    constexpr int so = S >> l, nq1 = so / 2 + 1, nq = nq1 * nq1;
-    for (int t = threadIdx.x; t < group_count<S>(tb.groups) * nq; t += kThreads) {
+    constexpr int so = S >> l, sc = so / 2, nq1 = so / 2 + 1, nq = nq1 * nq1;
+    const int total = group_count<S>(tb.groups) * nq;
+    auto full = [&](int t) {
       const int g = t / nq, q = t - g * nq;
-      Texel* lv = tb.lv + g * tb.lv_stride;
-      const Texel* coarse = lv + level_offset<S>(l + 1);
+      const Texel* coarse = tb.lv + lv_offset<S>(l + 1) + g * (sc * sc);
       if constexpr (l == 0)
         push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.base + g * (S * S), tb.rcp);
       else
-        push_quad<S, l>(q % nq1, q / nq1, coarse, lv + level_offset<S>(l), tb.rcp);
-    }
+        push_quad<S, l>(q % nq1, q / nq1, coarse, tb.lv + lv_offset<S>(l) + g * (so * so), tb.rcp);
+    };
+    for (int t = threadIdx.x; t < total; t += kThreads) full(t);
     __syncthreads();
-    push_levels<S, l - 1>(tb);
+    push_levels<S, l - 1, L>(tb);
   }
 }
 
@@ -540,6 +590,7 @@ struct TileLds {
   uint32_t base[kGroupMax<S> * S * S];
   Texel lv[kGroupMax<S> * kLv];
   float rtab[256], mtab[256], rcp[17];
+  int counts[1];  // list length of the two-pass pull level 1
 };
 
 // Pull-push of ISMs ism0 .. ism0 + groups - 1 of an atlas, by the whole
@@ -566,6 +617,7 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
     sm.mtab[b] = r * 2.0f / z_far * 0.6f;
   }
   if (tid < 17) sm.rcp[tid] = kRcp[tid];
+  if (tid == 0) sm.counts[0] = 0;
   // ---- load the group's atlas tiles (S*S u32 each) with 16-byte loads
   for (int i = tid; i < groups * n4; i += kThreads) {
     const int g = i / n4, w = i - g * n4;
@@ -574,7 +626,7 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
     *reinterpret_cast<uint4*>(&sm.base[g * S * S + row * S + col]) = v;
   }
   __syncthreads();
-  Tables tb{sm.base, sm.lv, TileLds<S, L>::kLv, sm.rtab, sm.mtab, sm.rcp, {}, z_far, ism0, n1d_log2, groups};
+  Tables tb{sm.base, sm.lv, sm.rtab, sm.mtab, sm.rcp, sm.counts, {}, z_far, ism0, n1d_log2, groups};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
   tb.empty = decode_texel(0xFFFFFFFFu, sm.rtab[255], sm.mtab[255], z_far, S);
 
@@ -583,12 +635,12 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
     // no level-1 push stores the bounds the level-0 push reads
     constexpr int n1 = (S / 2) * (S / 2);
     for (int t = tid; t < groups * n1; t += kThreads) {
-      Texel& x = sm.lv[(t / n1) * TileLds<S, L>::kLv + t % n1];
-      x.radius = sqrt_gt_bound_f(x.radius);
+      Texel& x = sm.lv[t];
+      x.radius = rd_bound(x.radius);
     }
     __syncthreads();
   }
-  push_levels<S, L - 1>(tb);
+  push_levels<S, L - 1, L>(tb);
 
   // ---- write the unorm16 tiles (held in the low halves of the tile slots)
   for (int i = tid; i < groups * n4; i += kThreads) {

@@ -67,3 +67,55 @@ long check_sqrt_bound(long n, uint64_t seed, int midpoint) {
   }
   return bad;
 }
+
+/* RD(m^2) in FP32 alone: m^2 = R * next(R) + u^2 / 4 (u = ulp(R)), and the
+ * added u^2 / 4 is below one ulp of the product, so RD(m^2) = RD(R next(R)):
+ * the product rounded to nearest, moved down one float when its FMA error
+ * shows it was rounded up. */
+static float rd_bound(float R) {
+  const float n = u2f(f2u(R) + 1u);
+  volatile float p = R * n;
+  const float e = fmaf(R, n, -p);
+  return e < 0.0f ? u2f(f2u(p) - 1u) : p;
+}
+
+/* RD(m^2) from the exact FP64 m^2. */
+static float rd_bound_f64(float R) {
+  const double b = sqrt_gt_bound(R, 1);
+  float f = (float)b;
+  if ((double)f > b) f = u2f(f2u(f) - 1u);
+  return f;
+}
+
+/* Mismatches of rd_bound against rd_bound_f64 for every float R in the
+ * binades [2^e0, 2^e1), plus 0 and 4096 denormals. */
+long check_rd_bound(int e0, int e1) {
+  long bad = 0;
+  for (int e = e0; e < e1; ++e)
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+      const float R = u2f(((uint32_t)(e + 127) << 23) | m);
+      if (f2u(rd_bound(R)) != f2u(rd_bound_f64(R))) ++bad;
+    }
+  for (uint32_t d = 0; d < 4096; ++d) {
+    const float R = u2f(d * 2047u);
+    if (f2u(rd_bound(R)) != f2u(rd_bound_f64(R))) ++bad;
+  }
+  return bad;
+}
+
+/* Mismatches of x > rd_bound(R) against sqrtf(x) > R, as check_sqrt_bound. */
+long check_rd_bound_sqrt(long n, uint64_t seed) {
+  long bad = 0;
+  for (long i = 0; i < n; ++i) {
+    const uint32_t e = 119u + (uint32_t)(splitmix(&seed) % 13u);
+    const float R = u2f((e << 23) | (uint32_t)(splitmix(&seed) & 0x7FFFFFu));
+    const float b = rd_bound(R);
+    const uint32_t c = f2u(R * R);
+    for (int k = -33; k <= 32; ++k) {
+      const float x = k < 32 ? u2f(c + (uint32_t)k) : R * R * (float)(splitmix(&seed) % 1000u) / 500.0f;
+      volatile float sx = sqrtf(x);
+      if ((sx > R) != (x > b)) ++bad;
+    }
+  }
+  return bad;
+}

@@ -5,8 +5,9 @@ cheaper forms that give the same bits:
   * x / w for the push and pull weight sums (w = 1..16) and x / 3.14 in the
     radius decode become RN(q + r y) with y = RN(1/w), q = RN(x y) and the
     exact FMA remainder r (Markstein's theorem);
-  * sqrt(x) > R in the radius tests becomes (double)x > m^2, m the midpoint
-    between R and the next float.
+  * sqrt(x) > R in the radius tests becomes x > RD(m^2), m the midpoint
+    between R and the next float, with RD(m^2) = RD(R next(R)) formed in
+    FP32 from the rounded product and its FMA error (no FP64 on the GPU).
 Both are checked here against IEEE float division and sqrtf: the division
 over every significand of two binades per divisor and sign (scaling by a
 power of two leaves all three roundings unchanged), the comparison on random
@@ -32,6 +33,10 @@ def lib(tmp_path_factory):
     L.check_div_by_recip.restype = C.c_long
     L.check_sqrt_bound.argtypes = [C.c_long, C.c_uint64, C.c_int]
     L.check_sqrt_bound.restype = C.c_long
+    L.check_rd_bound.argtypes = [C.c_int, C.c_int]
+    L.check_rd_bound.restype = C.c_long
+    L.check_rd_bound_sqrt.argtypes = [C.c_long, C.c_uint64]
+    L.check_rd_bound_sqrt.restype = C.c_long
     return L
 
 
@@ -42,6 +47,17 @@ def test_division_by_reciprocal_is_exact(lib, w):
 
 def test_sqrt_comparison_without_root(lib):
     assert lib.check_sqrt_bound(200_000, 12345, 1) == 0
+
+
+def test_fp32_bound_equals_the_fp64_midpoint_bound(lib):
+    # the kernel's radii are at most 15 px and shrink with the level scale
+    # and with small ISMs; checked over every float of [2^-16, 2^5), 0 and
+    # denormals (the identity itself holds for every normal R)
+    assert lib.check_rd_bound(-16, 5) == 0
+
+
+def test_sqrt_comparison_with_the_fp32_bound(lib):
+    assert lib.check_rd_bound_sqrt(200_000, 54321) == 0
 
 
 def test_planted_mistakes_are_caught(lib):
