@@ -34,6 +34,7 @@
 // classes of one screen tile, which read the same G-buffer cache lines, land
 // on one XCD and its L2.
 #include "batch_device.hpp"
+#include "float_float.hpp"
 #include "kernels.hpp"
 
 namespace mlgi {
@@ -208,41 +209,23 @@ __device__ __forceinline__ unsigned long long cull_chunk(const float* crec, int 
   return __ballot(keep);
 }
 
-// NEAREST texel of the pair re-decided in FP64 (full rate on CDNA): the
-// FP32 local coordinate lies within kTie of a texel edge, closer than its
-// rounding error can be trusted. Rebuilds the pixel position, the lookAtRH
-// basis and the projection from the FP32 inputs in double.
+// NEAREST texel of the pair re-decided exactly: the FP32 texel coordinate
+// lies within kTie of a texel edge, closer than its rounding error can be
+// trusted. The FP32 estimate names the nearest edge; which side of it the
+// coordinate lies on is decided in float-float arithmetic (FP32 operations
+// only, ~44 bits; float_float.hpp) from the FP32 inputs, as an FP64
+// evaluation would decide except within ~1e-11 texel of the edge.
 constexpr float kTie = 2.5e-4f;
 
-// Out of line: the FP64 temporaries would otherwise raise the register
+// Out of line: the float-float temporaries would otherwise raise the register
 // count of the whole kernel for a path taken by well under 1% of pairs.
 // Returns the texel as (ty << 16) | tx.
 __device__ __noinline__ int refine_texel(Mat4 m4, int width, int height, int ism_side, int x, int y,
                                          float depth, float vpx, float vpy, float vpz, float vnx,
-                                         float vny, float vnz) {
-  const float* m = m4.m;
-  const double c0 = (double)x / (double)width * 2.0 - 1.0;
-  const double c1 = (double)y / (double)height * 2.0 - 1.0;
-  const double c2 = (double)depth * 2.0 - 1.0;
-  double r[4];
-  for (int row = 0; row < 4; ++row)
-    r[row] = (double)m[row] * c0 + (double)m[4 + row] * c1 + ((double)m[8 + row] * c2 + (double)m[12 + row]);
-  const double dx = r[0] / r[3] - (double)vpx;
-  const double dy = r[1] / r[3] - (double)vpy;
-  const double dz = r[2] / r[3] - (double)vpz;
-  const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-  const double nx = vnx, ny = vny, nz = vnz;
-  const double upz = (double)0.01f;  // lookAtRH up = (0, 1, 0.01)
-  const double qx = ny * upz - nz, qy = -nx * upz, qz = nx;
-  const double ql = sqrt(qx * qx + qy * qy + qz * qz);
-  const double sx = qx / ql, sy = qy / ql, sz = qz / ql;
-  const double ux = sy * nz - sz * ny, uy = sz * nx - sx * nz, uz = sx * ny - sy * nx;
-  const double den = 1.0 + (nx * dx + ny * dy + nz * dz) / dist;
-  const double lx = (sx * dx + sy * dy + sz * dz) / dist / den;
-  const double ly = (ux * dx + uy * dy + uz * dz) / dist / den;
-  const double S = (double)ism_side;
-  const int tx = (int)floor((lx + 1.0) * 0.5 * S);
-  const int ty = (int)floor((ly + 1.0) * 0.5 * S);
+                                         float vny, float vnz, float cx, float cy) {
+  const float vp[3] = {vpx, vpy, vpz}, vn[3] = {vnx, vny, vnz};
+  int tx, ty;
+  ff::texel_exact(m4.m, width, height, x, y, depth, vp, vn, ism_side, cx, cy, &tx, &ty);
   return ((ty & 0xFFFF) << 16) | (tx & 0xFFFF);
 }
 
@@ -269,11 +252,24 @@ __device__ __forceinline__ LitConsts lit_consts(const GatherArgs& a) {
                    a.shadow_k1, a.shadow_k2};
 }
 
+// Atlas texel of ISM texel (tx, ty) in the tile at (ox, oy), wrapped as
+// GL_REPEAT wraps the atlas.
+__device__ __forceinline__ uint32_t atlas_texel(const LitConsts& lc, int ox, int oy, int tx, int ty) {
+  return (uint32_t)((((oy + ty) & lc.amask) << lc.alog) | ((ox + tx) & lc.amask));
+}
+
+// The smoothstep argument of a lit pair at ISM texel (tx, ty), clamped to
+// [0, 1]: the pair's shadow weight is a function of this value alone.
+__device__ __forceinline__ float shadow_arg(const LitConsts& lc, int ox, int oy, int tx, int ty, float sbase) {
+  const uint16_t occ = __builtin_amdgcn_raw_buffer_load_b16(lc.ism, atlas_texel(lc, ox, oy, tx, ty) << 1, 0, 0);
+  return __builtin_amdgcn_fmed3f(__builtin_fmaf((float)occ, lc.k2, sbase), 0.0f, 1.0f);
+}
+
 // One lit pair (n_v . d > 0, n_p . d < 0) up to its ISM fetch: the geometry
 // term with its clamp (final_gathering.comp:100-117), the texel of the
 // pair's ISM (:118-127) and the distance part of the smoothstep argument.
 // (px, py, pz) is the pixel's position; x, y, depth identify the pixel for
-// the FP64 tie check.
+// the exact texel decision.
 struct PairSetup {
   float g;         // clamped geometry term
   float sbase;     // 1 - dist / (bias zFar): the smoothstep argument without occ
@@ -298,6 +294,9 @@ __device__ __forceinline__ PairSetup pair_setup(const GatherArgs& a, float px, f
     const float ud = __builtin_fmaf(v.ux, dx, __builtin_fmaf(v.uy, dy, v.uz * dz));
     const float cx = __builtin_fmaf(sd, rh, lc.hs);
     const float cy = __builtin_fmaf(ud, rh, lc.hs);
+    const int ox = (vi & (a.n1d - 1)) * a.ism_side;  // the VPL's ISM tile
+    const int oy = (vi >> a.n1d_log2) * a.ism_side;
+    r.sbase = __builtin_fmaf(-dist, lc.k1, 1.0f);
     int tx, ty;
     if (kMixed) {
       tx = (int)floorf(cx);
@@ -306,19 +305,25 @@ __device__ __forceinline__ PairSetup pair_setup(const GatherArgs& a, float px, f
       const float fx = __builtin_amdgcn_fractf(cx), fy = __builtin_amdgcn_fractf(cy);
       tx = (int)(cx - fx);
       ty = (int)(cy - fy);
-      if (fmaxf(fabsf(fx - 0.5f), fabsf(fy - 0.5f)) > 0.5f - kTie) {
-        const int t2 = refine_texel(a.vp_inv, a.width, a.height, a.ism_side, x, y, depth, vt.px, vt.py, vt.pz,
-                                    vt.nx, vt.ny, vt.nz);
-        tx = (int)(int16_t)(t2 & 0xFFFF);
-        ty = t2 >> 16;
+      const bool near_x = fabsf(fx - 0.5f) > 0.5f - kTie, near_y = fabsf(fy - 0.5f) > 0.5f - kTie;
+      if (near_x || near_y) {
+        // The exact texel is one of the (up to) four on either side of the
+        // near edges. When all of them give the same clamped smoothstep
+        // argument the pair's weight does not depend on the choice.
+        const int ax = near_x ? (fx < 0.5f ? -1 : 1) : 0, ay = near_y ? (fy < 0.5f ? -1 : 1) : 0;
+        const float t0 = shadow_arg(lc, ox, oy, tx, ty, r.sbase);
+        const bool same = shadow_arg(lc, ox, oy, tx + ax, ty, r.sbase) == t0 &&
+                          shadow_arg(lc, ox, oy, tx, ty + ay, r.sbase) == t0 &&
+                          shadow_arg(lc, ox, oy, tx + ax, ty + ay, r.sbase) == t0;
+        if (!same) {
+          const int t2 = refine_texel(a.vp_inv, a.width, a.height, a.ism_side, x, y, depth, vt.px, vt.py, vt.pz,
+                                      vt.nx, vt.ny, vt.nz, cx, cy);
+          tx = (int)(int16_t)(t2 & 0xFFFF);
+          ty = t2 >> 16;
+        }
       }
     }
-    const int tile_y = vi >> a.n1d_log2;
-    const int tile_xi = vi & (a.n1d - 1);
-    const int gx = (tile_xi * a.ism_side + tx) & lc.amask;  // GL_REPEAT wrap
-    const int gy = (tile_y * a.ism_side + ty) & lc.amask;
-    r.texel = (uint32_t)((gy << lc.alog) | gx);
-    r.sbase = __builtin_fmaf(-dist, lc.k1, 1.0f);
+    r.texel = atlas_texel(lc, ox, oy, tx, ty);
   }
   return r;
 }
@@ -327,7 +332,7 @@ __device__ __forceinline__ PairSetup pair_setup(const GatherArgs& a, float px, f
 // occ)) with t = 1 + (occ - dist/zFar) / bias (final_gathering.comp:128-131).
 __device__ __forceinline__ float pair_finish(const PairSetup& s, uint16_t occ16, const LitConsts& lc) {
   float t = __builtin_fmaf((float)occ16, lc.k2, s.sbase);
-  t = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
+  t = __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);  // shadow_arg's value
   return s.g * (t * t * __builtin_fmaf(-2.0f, t, 3.0f));
 }
 
@@ -421,7 +426,8 @@ enum FetchVariant { kFetchWhole = 0, kFetchTestFirst = 1 };
 
 // Precision forms.
 //   FP32  the reference's arithmetic in FP32, NEAREST texel decisions that
-//         FP32 cannot resolve re-decided in FP64 (rel-L2 <= 1e-4).
+//         FP32 cannot resolve re-decided in float-float arithmetic
+//         (rel-L2 <= 1e-4; no FP64 anywhere).
 //   MIXED flux staged and multiplied as packed FP16 (f16 x f32 -> f32 mixed
 //         FMAs), NEAREST decided in FP32 alone, per-pixel sums in FP32 per
 //         64-VPL chunk, chunks combined with Kahan compensation

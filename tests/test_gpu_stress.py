@@ -6,7 +6,10 @@ mixed form through the batched kernels replayed from a HIP graph, with the
 light pose changed every frame (RSM rendered on the device). Frames 0 and 30
 of 31 are checked stage-wise against the oracle: VPL records, atlas and ISM
 bit-exact, irradiance within the mixed form's rel-L2 2e-3 of the FP64
-transcription.
+transcription. On frame 30 the single-frame gather then runs on the same VPLs
+and ISM in the plain FP32 form (rel-L2 <= 1e-4 of the same transcription)
+and, in both forms, with the VPL culling switched off: culled and brute-force
+sums are equal bit for bit.
 """
 import numpy as np
 import pytest
@@ -59,6 +62,22 @@ def test_4k_light_per_frame_graph_replays(ctx):
         err = float(np.linalg.norm(img - ref) / np.linalg.norm(ref))
         print(f"[4k] frame {i}: rel-L2 {err:.3e}, lit pairs {lit}", flush=True)
         assert err <= 2e-3, i
+    ctx.sync()
+    for name, prec, bar in (("fp32", mlgi.PRECISION_FP32, 1e-4), ("mixed", mlgi.PRECISION_MIXED, 2e-3)):
+        imgs = []
+        for cull_off in (0, 1):
+            st.params.precision = prec
+            st.params.disable_culling = cull_off
+            st.gi_buffer.copy_from(np.full((wl.height, wl.width, 3), -1.0, np.float32))
+            st.compute_final_gathering(gb)
+            ctx.sync()
+            imgs.append(st.gi_buffer.numpy())
+        st.params.precision, st.params.disable_culling = mlgi.PRECISION_MIXED, 0
+        err = float(np.linalg.norm(imgs[0] - ref) / np.linalg.norm(ref))
+        diff = int((imgs[0] != imgs[1]).sum())
+        print(f"[4k] frame 30 single-frame {name}: rel-L2 {err:.3e}, culled vs brute force {diff} differing", flush=True)
+        assert err <= bar, name
+        assert diff == 0, name
     g.close()
     batch.close()
     dev.close()
