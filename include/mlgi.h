@@ -364,9 +364,11 @@ int mlgi_autotune(mlgi_context* ctx, mlgi_frame_desc* frame, int32_t min_vpls_pe
  * admitted as in-flight slots and a fixed HBM budget free up (their work
  * buffers come from one arena allocated up front), and are batched
  * continuously: every admission round packs all admissible waiting requests
- * into one batch that is enqueued behind the batches already on the device.
- * The light pose's RSM is rendered on the device per request. A request's
- * result depends only on the request (the same bits alone or in any mix). */
+ * into one batch. Queued batches go to `streams` device streams of equal
+ * priority in turn, so a new batch runs beside those already on the device
+ * instead of behind them. The light pose's RSM is rendered on the device per
+ * request. A request's result depends only on the request (the same bits
+ * alone or in any mix, on any stream). */
 typedef struct mlgi_server mlgi_server;
 typedef struct mlgi_server_config {
   size_t hbm_budget;          /* arena bytes the admitted requests' buffers come from */
@@ -380,6 +382,8 @@ typedef struct mlgi_server_config {
   int32_t n_vpl_counts;
   const int32_t* resolutions; /* (width, height) pairs requests may use */
   int32_t n_resolutions;
+  int32_t streams;            /* device streams the batches are spread over, at most
+                                 max_queued_batches; 0 = max_queued_batches */
 } mlgi_server_config;
 typedef struct mlgi_server_scene {
   const struct mlgi_scene* scene; /* host scene (mlgi_scene.h) */
@@ -401,6 +405,8 @@ typedef struct mlgi_server_stats {
   int32_t batches, max_inflight_seen;
   double mean_batch;
   size_t peak_bytes;      /* peak arena use */
+  int32_t overlapped_batches; /* batches enqueued while another was still on the device */
+  int32_t streams;            /* device streams used */
 } mlgi_server_stats;
 /* Uploads the scenes, builds their point binnings and G-buffers, tunes every
  * (resolution, VPL count) shape. */

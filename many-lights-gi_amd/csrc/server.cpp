@@ -1,17 +1,24 @@
 // GI request server (include/mlgi.h, "serving"): resident scenes, an HBM
 // budget arena, continuous batching over the batched kernels.
 //
-// One thread drives one context stream. Each loop iteration
+// One host thread drives the server. Each queued-batch slot has a device
+// stream of its own (a lane: the server's context for slot 0, a context the
+// server creates for each further slot, all of default priority), so batches
+// in different slots run beside each other on the device. Each loop
+// iteration
 //   1. moves requests whose arrival time has passed into the FIFO queue,
 //   2. retires finished batches (event query), releasing their slots and
 //      arena ranges and recording completion times,
 //   3. if a queued-batch slot is free, admits queued requests in FIFO order
 //      while in-flight slots and arena space allow, renders each admitted
-//      request's RSM on the device and enqueues them as one batch behind the
-//      batches already on the device.
+//      request's RSM on the slot's stream and enqueues them there as one
+//      batch.
 // Requests arriving while batches run are therefore admitted as soon as
-// resources free up, without waiting for the device to drain. A request that
-// cannot fit the whole arena is rejected.
+// resources free up, without waiting for the device to drain, and their work
+// overlaps the batches already running. An arena range returns to the free
+// list only after the host has seen its batch's completion event, so a later
+// batch on any stream may reuse it. A request that cannot fit the whole arena
+// is rejected.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -161,6 +168,9 @@ struct mlgi_server {
   mlgi::BatchFrame* table_host = nullptr;  // pinned
   std::vector<hipEvent_t> events;
   std::vector<void*> owned;  // resident device allocations
+  // one context (stream) per lane; lanes[0] is ctx, the others are owned
+  std::vector<mlgi_context*> lanes;
+  mlgi_context* lane_of(int slot) const { return lanes[(size_t)slot % lanes.size()]; }
 };
 
 namespace {
@@ -220,9 +230,9 @@ mlgi_frame_desc request_frame(mlgi_server* s, const mlgi_request& r, size_t off,
   return d;
 }
 
-// Renders the request's RSM on the device (asynchronous) and returns its
+// Renders the request's RSM on lane L's stream (asynchronous) and returns its
 // inverse biased light matrix.
-int render_request_rsm(mlgi_server* s, const mlgi_request& r, size_t off, float inv_bvp[16]) {
+int render_request_rsm(mlgi_server* s, mlgi_context* L, const mlgi_request& r, size_t off, float inv_bvp[16]) {
   const int rs = s->cfg.rsm_size;
   const Footprint fp = footprint(r.width, r.height, r.n_vpl, rs, points_of(s, r));
   char* b = s->arena_base + off;
@@ -231,7 +241,13 @@ int render_request_rsm(mlgi_server* s, const mlgi_request& r, size_t off, float 
   float* dep = (float*)((char*)nrm + fp.rsm_normal);
   mlgi_camera light;
   mlgi_scene_light_pose(r.light_seed, &light);
-  return mlgi_scene_device_render_rsm(s->ctx, s->scenes[r.scene].dev, &light, rs, rs, dif, nrm, dep, inv_bvp);
+  return mlgi_scene_device_render_rsm(L, s->scenes[r.scene].dev, &light, rs, rs, dif, nrm, dep, inv_bvp);
+}
+
+// Copies a lane's error into the server context's (no-op for lane 0).
+int lane_fail(mlgi_server* s, mlgi_context* L, int rc) {
+  if (rc && L != s->ctx) s->ctx->error = L->error;
+  return rc;
 }
 
 bool configured(const std::vector<int>& v, int x) { return std::find(v.begin(), v.end(), x) != v.end(); }
@@ -242,6 +258,8 @@ extern "C" {
 
 void mlgi_server_destroy(mlgi_server* s) {
   if (!s) return;
+  for (mlgi_context* L : s->lanes) (void)hipStreamSynchronize(L->stream);
+  for (size_t k = 1; k < s->lanes.size(); ++k) mlgi_context_destroy(s->lanes[k]);
   if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
   for (auto e : s->events) (void)hipEventDestroy(e);
   for (auto& sc : s->scenes) mlgi_scene_device_destroy(sc.dev);
@@ -257,7 +275,7 @@ int mlgi_server_create(mlgi_context* c, const mlgi_server_config* cfg, const mlg
   *out = nullptr;
   if (cfg->max_inflight <= 0 || cfg->max_batch <= 0 || cfg->max_queued_batches <= 0 || cfg->rsm_size <= 0 ||
       cfg->rsm_size > 4096 || cfg->n_vpl_counts <= 0 || !cfg->vpl_counts || cfg->n_resolutions <= 0 ||
-      !cfg->resolutions || cfg->hbm_budget == 0)
+      !cfg->resolutions || cfg->hbm_budget == 0 || cfg->streams < 0 || cfg->max_queued_batches > 64)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "bad server configuration");
   if (cfg->precision != MLGI_PRECISION_FP32 && cfg->precision != MLGI_PRECISION_MIXED)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "unknown precision %d", cfg->precision);
@@ -270,11 +288,22 @@ int mlgi_server_create(mlgi_context* c, const mlgi_server_config* cfg, const mlg
   s->cfg.vpl_counts = nullptr;
   s->cfg.resolutions = nullptr;
   int rc = MLGI_OK;
+  // lanes: the server's context and one more context (stream) per further slot
+  const int n_lanes = std::min(cfg->streams > 0 ? cfg->streams : cfg->max_queued_batches, cfg->max_queued_batches);
+  s->lanes.push_back(c);
+  for (int k = 1; k < n_lanes && !rc; ++k) {
+    mlgi_context* L = nullptr;
+    const int r = mlgi_context_create(c->device, &L);
+    if (r) rc = fail(c, r, "server lane context %d could not be created", k);
+    else s->lanes.push_back(L);
+  }
   for (int n : s->vpl_counts) {
-    const int32_t* sh = nullptr;
-    if (n <= 0 || n % 16 != 0) rc = fail(c, MLGI_ERR_INVALID_ARGUMENT, "VPL count %d must be a multiple of 16", n);
-    if (!rc) rc = get_shuffle(c, n, &sh);
     if (rc) break;
+    if (n <= 0 || n % 16 != 0) rc = fail(c, MLGI_ERR_INVALID_ARGUMENT, "VPL count %d must be a multiple of 16", n);
+    for (mlgi_context* L : s->lanes) {  // each lane's shuffle table, before any request
+      const int32_t* sh = nullptr;
+      if (!rc) rc = lane_fail(s, L, get_shuffle(L, n, &sh));
+    }
   }
   for (auto& wh : s->resolutions)
     if (!rc && (wh.first <= 0 || wh.second <= 0 || wh.first > 16384 || wh.second > 16384))
@@ -342,7 +371,7 @@ int mlgi_server_create(mlgi_context* c, const mlgi_server_config* cfg, const mlg
       size_t off = 0;
       s->arena.alloc(fp.total, &off);
       float inv[16];
-      if ((rc = render_request_rsm(s, r, off, inv))) break;
+      if ((rc = render_request_rsm(s, c, r, off, inv))) break;
       Tune t{4, 8, 8};
       mlgi_frame_desc d = request_frame(s, r, off, t, inv, nullptr);
       mlgi_tune_result tr;
@@ -411,8 +440,10 @@ int mlgi_server_run(mlgi_server* s, const mlgi_request* reqs, int32_t n, mlgi_se
       progress = true;
     }
     if (rc) break;
-    // 3. admission into one new batch
+    // 3. admission into one new batch, on the stream of the next free slot
     if (!queue.empty() && !ring_free.empty()) {
+      const int slot = ring_free.back();
+      mlgi_context* L = s->lane_of(slot);
       std::vector<Admitted> adm;
       std::vector<mlgi_frame_desc> descs;
       while (!queue.empty() && (int)adm.size() < s->cfg.max_batch && inflight_reqs < s->cfg.max_inflight) {
@@ -430,18 +461,17 @@ int mlgi_server_run(mlgi_server* s, const mlgi_request* reqs, int32_t n, mlgi_se
         if (!s->arena.alloc(fp.total, &off)) break;  // wait for releases (FIFO, no overtaking)
         queue.pop_front();
         float inv[16];
-        if ((rc = render_request_rsm(s, r, off, inv))) break;
+        if ((rc = lane_fail(s, L, render_request_rsm(s, L, r, off, inv)))) break;
         float* img = nullptr;
         descs.push_back(request_frame(s, r, off, s->tuned.at({r.width, r.height, r.n_vpl}), inv, &img));
         adm.push_back({i, off, fp.total, img});
         ++inflight_reqs;
       }
       if (!rc && !adm.empty()) {
-        const int slot = ring_free.back();
         ring_free.pop_back();
         std::vector<mlgi::BatchFrame> table;
         BatchPlan plan;
-        if (!(rc = batch_plan(c, descs.data(), (int)descs.size(), &table, &plan))) {
+        if (!(rc = lane_fail(s, L, batch_plan(L, descs.data(), (int)descs.size(), &table, &plan)))) {
           for (size_t k = 0; k < adm.size(); ++k) {  // splat work areas: the end of each request's range
             const mlgi_request& r = reqs[adm[k].req];
             const Footprint fp = footprint(r.width, r.height, r.n_vpl, s->cfg.rsm_size, points_of(s, r));
@@ -455,18 +485,19 @@ int mlgi_server_run(mlgi_server* s, const mlgi_request* reqs, int32_t n, mlgi_se
           mlgi::BatchFrame* dev = s->table_dev + (size_t)slot * s->cfg.max_batch;
           std::memcpy(host, table.data(), sizeof(mlgi::BatchFrame) * table.size());
           rc = hip_check(c, hipMemcpyAsync(dev, host, sizeof(mlgi::BatchFrame) * table.size(), hipMemcpyHostToDevice,
-                                           c->stream),
+                                           L->stream),
                          "hipMemcpyAsync(batch table)");
-          if (!rc) rc = batch_enqueue(c, dev, plan);
+          if (!rc) rc = lane_fail(s, L, batch_enqueue(L, dev, plan));
           for (const Admitted& a : adm) {
             if (rc || !reqs[a.req].host_out) continue;
             const size_t bytes = (size_t)reqs[a.req].width * reqs[a.req].height * 12;
-            rc = hip_check(c, hipMemcpyAsync(reqs[a.req].host_out, a.image, bytes, hipMemcpyDeviceToHost, c->stream),
+            rc = hip_check(c, hipMemcpyAsync(reqs[a.req].host_out, a.image, bytes, hipMemcpyDeviceToHost, L->stream),
                            "hipMemcpyAsync(result)");
           }
-          if (!rc) rc = hip_check(c, hipEventRecord(s->events[slot], c->stream), "hipEventRecord");
+          if (!rc) rc = hip_check(c, hipEventRecord(s->events[slot], L->stream), "hipEventRecord");
         }
         if (!rc) {
+          if (!inflight.empty()) stats->overlapped_batches++;
           inflight.push_back({s->events[slot], slot, std::move(adm)});
           stats->batches++;
           batched += (int64_t)inflight.back().reqs.size();
@@ -477,7 +508,7 @@ int mlgi_server_run(mlgi_server* s, const mlgi_request* reqs, int32_t n, mlgi_se
     }
     if (!progress) std::this_thread::yield();
   }
-  (void)hipStreamSynchronize(c->stream);
+  for (mlgi_context* L : s->lanes) (void)hipStreamSynchronize(L->stream);
   if (rc) return rc;
   std::vector<double> ok;
   for (int i = 0; i < n; ++i)
@@ -499,6 +530,7 @@ int mlgi_server_run(mlgi_server* s, const mlgi_request* reqs, int32_t n, mlgi_se
   stats->latency_max_ms = ok.empty() ? 0.0 : ok.back();
   stats->mean_batch = stats->batches ? (double)batched / stats->batches : 0.0;
   stats->peak_bytes = s->arena.peak();
+  stats->streams = (int32_t)s->lanes.size();
   if (latencies_ms) std::copy(lat.begin(), lat.end(), latencies_ms);
   return MLGI_OK;
 }

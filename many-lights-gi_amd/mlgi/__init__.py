@@ -695,8 +695,9 @@ class Server:
 
     def __init__(self, ctx: Context, scenes, vpl_counts, resolutions, hbm_budget: int, max_inflight: int = 64,
                  max_batch: int = 32, max_queued_batches: int = 3, rsm_size: int = 128,
-                 precision: int = PRECISION_MIXED, min_vpls_per_pixel: int = 32):
-        """scenes: list of (Scene, n_points, point_seed); resolutions: list of (w, h)."""
+                 precision: int = PRECISION_MIXED, min_vpls_per_pixel: int = 32, streams: int = 0):
+        """scenes: list of (Scene, n_points, point_seed); resolutions: list of (w, h);
+        streams: device streams the queued batches are spread over (0: one per queued batch)."""
         self.ctx = ctx
         self._scenes = [sc for sc, _, _ in scenes]  # keep the host scenes alive
         arr = (N.ServerScene * len(scenes))()
@@ -705,7 +706,7 @@ class Server:
         vc = (C.c_int32 * len(vpl_counts))(*vpl_counts)
         rs = (C.c_int32 * (2 * len(resolutions)))(*[v for wh in resolutions for v in wh])
         cfg = N.ServerConfig(hbm_budget, max_inflight, max_batch, max_queued_batches, rsm_size, precision,
-                             min_vpls_per_pixel, vc, len(vpl_counts), rs, len(resolutions))
+                             min_vpls_per_pixel, vc, len(vpl_counts), rs, len(resolutions), streams)
         h = C.c_void_p()
         ctx.check(lib.mlgi_server_create(ctx.h, C.byref(cfg), arr, len(scenes), C.byref(h)), "mlgi_server_create")
         self.h = h

@@ -126,7 +126,8 @@ class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int32), ("max_batch", C.c_int32),
                 ("max_queued_batches", C.c_int32), ("rsm_size", C.c_int32), ("precision", C.c_int32),
                 ("min_vpls_per_pixel", C.c_int32), ("vpl_counts", C.POINTER(C.c_int32)),
-                ("n_vpl_counts", C.c_int32), ("resolutions", C.POINTER(C.c_int32)), ("n_resolutions", C.c_int32)]
+                ("n_vpl_counts", C.c_int32), ("resolutions", C.POINTER(C.c_int32)), ("n_resolutions", C.c_int32),
+                ("streams", C.c_int32)]
 
 
 class ServerScene(C.Structure):
@@ -142,7 +143,8 @@ class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_int64), ("rejected", C.c_int64), ("wall_s", C.c_double),
                 ("req_per_s", C.c_double), ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
                 ("latency_mean_ms", C.c_double), ("latency_max_ms", C.c_double), ("batches", C.c_int32),
-                ("max_inflight_seen", C.c_int32), ("mean_batch", C.c_double), ("peak_bytes", C.c_size_t)]
+                ("max_inflight_seen", C.c_int32), ("mean_batch", C.c_double), ("peak_bytes", C.c_size_t),
+                ("overlapped_batches", C.c_int32), ("streams", C.c_int32)]
 
 
 class StageTimes(C.Structure):
