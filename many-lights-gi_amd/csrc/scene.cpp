@@ -6,6 +6,7 @@
 // random.glsl:34-66). Deterministic for a given seed and independent of the
 // thread count.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -361,8 +362,15 @@ void to_float(const M4& a, float* o) {
   for (int i = 0; i < 16; ++i) o[i] = (float)a.m[i];
 }
 
+// Worker threads of the host renderer and sampler: `t` when positive, else
+// OMP_NUM_THREADS when set (the process's share of the host), else the
+// hardware's thread count (at most 64).
 int nthreads(int t) {
   if (t > 0) return t;
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int n = std::atoi(e);
+    if (n > 0) return std::min(n, 64);
+  }
   unsigned h = std::thread::hardware_concurrency();
   return h ? (int)std::min(h, 64u) : 4;
 }
