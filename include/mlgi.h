@@ -99,8 +99,11 @@ typedef struct mlgi_rsm {
 /* VPL records, MLGI_VPL_RECORD_FLOATS floats (128 bytes) per VPL:
  *  [0..2]   world position          [3]  plane offset n . position
  *  [4..6]   unpacked normal         [7]  packed normal (pack3SNToFloat bits, u32)
- *  [8..10]  lookAtRH s row          [12..14] lookAtRH u row
+ *  [8..10]  lookAtRH s row          [11] flux r | g << 16 as binary16 bits
+ *  [12..14] lookAtRH u row          [15] flux b as binary16 bits (high half 0)
  *  [16..18] flux (diffuse * lightIntensity); all other slots are 0.
+ * Slots 11 and 15 hold [16..18] rounded to nearest even; the mixed-form
+ * gather reads the flux from them (records built by hand must fill them).
  * (VPLProcessor vplBuffer + packedVplBuffer, VPLProcessor.cpp:29-41). */
 #define MLGI_VPL_RECORD_FLOATS 32
 typedef struct mlgi_vpls {

@@ -647,6 +647,12 @@ void pick_tile(const mlgi_params* p, int* tw, int* th) {
 
 }  // namespace mlgi_capi
 
+int check_precision(mlgi_context* c, const mlgi_params* p) {
+  if (p->precision != MLGI_PRECISION_FP32 && p->precision != MLGI_PRECISION_MIXED)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "unknown precision %d", p->precision);
+  return MLGI_OK;
+}
+
 extern "C" {
 
 int mlgi_final_gather(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* vpls,
@@ -657,8 +663,7 @@ int mlgi_final_gather(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
   if (rc) return rc;
   if (!out || !out->rgb || out->width != gb->width || out->height != gb->height)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "output image must match the gbuffer size");
-  if (p->precision != MLGI_PRECISION_FP32 && p->precision != MLGI_PRECISION_MIXED)
-    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "unknown precision %d", p->precision);
+  if ((rc = check_precision(c, p))) return rc;
   int tw, th;
   pick_tile(p, &tw, &th);
   if (!((tw == 8 && th == 8) || (tw == 16 && th == 16) || (tw == 32 && th == 8) || (tw == 16 && th == 4)))
@@ -900,8 +905,7 @@ int batch_frame(mlgi_context* c, const mlgi_frame_desc& d, mlgi::BatchFrame* f) 
   if ((rc = build_gather_args(c, &d.gbuffer, &d.vpls, &ism, p, &a))) return rc;
   if (!d.image.rgb || d.image.width != d.gbuffer.width || d.image.height != d.gbuffer.height)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "output image must match the gbuffer size");
-  if (p->precision != MLGI_PRECISION_FP32 && p->precision != MLGI_PRECISION_MIXED)
-    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "unknown precision %d", p->precision);
+  if ((rc = check_precision(c, p))) return rc;
   const int tw = p->tile_w > 0 ? p->tile_w : 8;
   if (tw != 8 && tw != 16 && tw != 32)
     return fail(c, MLGI_ERR_UNSUPPORTED, "batched gather strip width %d not in {8, 16, 32}", tw);

@@ -42,18 +42,24 @@ namespace {
 
 constexpr int kXcds = 8;
 
-// VPL fields the gather uses (64 bytes of the 128-byte record).
+// VPL fields the gather uses (72 bytes of the 128-byte record).
 struct VplLds {
   float px, py, pz, plane;
   float nx, ny, nz, sx;
   float sy, sz, ux, uy;
   float uz, cr, cg, cb;
+  uint32_t hrg, hb;  // the flux as halves: r | g << 16, b (record slots 11, 15)
 };
+
+// The bits of a float taken by value: __builtin_bit_cast applied directly to
+// a vector element (v.w) reads the wrong element with this compiler.
+__device__ __forceinline__ uint32_t float_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
 
 __device__ __forceinline__ VplLds load_vpl(const float* rec) {
   const float4* r = reinterpret_cast<const float4*>(rec);
   const float4 a = gload4(r), b = gload4(r + 1), c = gload4(r + 2), d = gload4(r + 3), e = gload4(r + 4);
-  return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
+  return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z,
+                float_bits(c.w), float_bits(d.w)};
 }
 
 // Wave64 min / max, result wave-uniform. Floats are mapped to unsigned keys
@@ -172,14 +178,10 @@ __device__ __forceinline__ bool wave_bounds(const Pixel& p, WaveBounds& wb) {
 // the wave gets angleFactor == 0 from it, or when its flux is zero (an RSM
 // texel outside the spot cone; MIXED: zero after rounding to halves, the
 // values the sums multiply): every contribution is then flux * weight = +0
-// with a finite weight, which leaves the sums' bits unchanged. `v` carries
-// the flux as floats.
+// with a finite weight, which leaves the sums' bits unchanged.
 template <bool kMixed>
 __device__ __forceinline__ bool cull_keep(const VplLds& v, const WaveBounds& w) {
-  const bool dark =
-      kMixed ? (__builtin_bit_cast(uint16_t, (_Float16)v.cr) | __builtin_bit_cast(uint16_t, (_Float16)v.cg) |
-                __builtin_bit_cast(uint16_t, (_Float16)v.cb)) == 0
-             : (v.cr == 0.0f && v.cg == 0.0f && v.cb == 0.0f);
+  const bool dark = kMixed ? (v.hrg | v.hb) == 0u : (v.cr == 0.0f && v.cg == 0.0f && v.cb == 0.0f);
   // (1) max over the box of n . (p - p_vpl) = n . (c - p_vpl) + |n| . h
   const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
   const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
@@ -298,7 +300,7 @@ __device__ __forceinline__ PairSetup pair_setup(const GatherArgs& a, float px, f
     const int oy = (vi >> a.n1d_log2) * a.ism_side;
     r.sbase = __builtin_fmaf(-dist, lc.k1, 1.0f);
     int tx, ty;
-    if (kMixed) {
+    if (kMixed) {  // NEAREST decided in FP32 alone
       tx = (int)floorf(cx);
       ty = (int)floorf(cy);
     } else {
@@ -346,57 +348,37 @@ __device__ __forceinline__ float pair_weight(const GatherArgs& a, float px, floa
   return pair_finish(s, __builtin_amdgcn_raw_buffer_load_b16(lc.ism, s.texel << 1, 0, 0), lc);
 }
 
-// The VPL's flux as floats (MIXED: packed halves, (r, g) in cr, (b, 0) in cg).
-template <bool kMixed>
-__device__ __forceinline__ void vpl_flux(const VplLds& v, float& r, float& g, float& b) {
-  if (kMixed) {
-    const uint32_t rg = __builtin_bit_cast(uint32_t, v.cr);
-    const uint32_t bb = __builtin_bit_cast(uint32_t, v.cg);
-    r = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg & 0xFFFFu));
-    g = (float)__builtin_bit_cast(_Float16, (uint16_t)(rg >> 16));
-    b = (float)__builtin_bit_cast(_Float16, (uint16_t)(bb & 0xFFFFu));
-  } else {
-    r = v.cr;
-    g = v.cg;
-    b = v.cb;
-  }
-}
+// The flux half h of a record (binary16 bits) as a float: the conversion
+// folds into the mixed-precision FMA that multiplies it (v_fma_mix_f32).
+__device__ __forceinline__ float half_bits(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
 
 // Contribution of one lit pair to the pixel's running sums (one fused
-// multiply-add per channel).
+// multiply-add per channel; MIXED: the flux as the record's halves).
 template <bool kMixed>
 __device__ __forceinline__ void accumulate_pair(const GatherArgs& a, const Pixel& p, const VplTest& vt,
                                                 const VplLds& v, int vi, float ndot, float pdot,
                                                 bool shadow, const LitConsts& lc, float& ar, float& ag,
                                                 float& ab) {
   const float g = pair_weight<kMixed>(a, p.fx, p.fy, p.fz, p.x, p.y, p.depth, vt, v, vi, ndot, pdot, shadow, lc);
-  float fr, fg, fb;
-  vpl_flux<kMixed>(v, fr, fg, fb);
+  const float fr = kMixed ? half_bits(v.hrg & 0xFFFFu) : v.cr;
+  const float fg = kMixed ? half_bits(v.hrg >> 16) : v.cg;
+  const float fb = kMixed ? half_bits(v.hb & 0xFFFFu) : v.cb;
   ar = __builtin_fmaf(fr, g, ar);
   ag = __builtin_fmaf(fg, g, ag);
   ab = __builtin_fmaf(fb, g, ab);
 }
 
 // Record of VPL vi (a wave-uniform index) read through the scalar data
-// cache into scalar registers, in the form the precision uses (MIXED: flux
-// rounded to halves, round to nearest even, (r, g) packed in cr and b in
-// cg). No LDS traffic, and the VALU takes the fields as scalar operands.
-template <bool kMixed>
+// cache into scalar registers. No LDS traffic, and the VALU takes the fields
+// as scalar operands.
 __device__ __forceinline__ VplLds uniform_vpl(const float* vpl, int vi) {
   typedef float cf4 __attribute__((ext_vector_type(4)));
   typedef const __attribute__((address_space(4))) cf4 ConstF4;
   const int u = __builtin_amdgcn_readfirstlane(vi);
   ConstF4* r = (ConstF4*)(vpl + (size_t)u * kVplFloats);
   const cf4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
-  VplLds v{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z};
-  if (kMixed) {
-    const uint32_t hr = __builtin_bit_cast(uint16_t, (_Float16)v.cr);
-    const uint32_t hg = __builtin_bit_cast(uint16_t, (_Float16)v.cg);
-    const uint32_t hb = __builtin_bit_cast(uint16_t, (_Float16)v.cb);
-    v.cr = __builtin_bit_cast(float, hr | (hg << 16));
-    v.cg = __builtin_bit_cast(float, hb);
-  }
-  return v;
+  return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z,
+                float_bits(c.w), float_bits(d.w)};
 }
 
 // The hemisphere-test fields of VPL vi (wave-uniform), as uniform_vpl.
@@ -428,12 +410,62 @@ enum FetchVariant { kFetchWhole = 0, kFetchTestFirst = 1 };
 //   FP32  the reference's arithmetic in FP32, NEAREST texel decisions that
 //         FP32 cannot resolve re-decided in float-float arithmetic
 //         (rel-L2 <= 1e-4; no FP64 anywhere).
-//   MIXED flux staged and multiplied as packed FP16 (f16 x f32 -> f32 mixed
-//         FMAs), NEAREST decided in FP32 alone, per-pixel sums in FP32 per
-//         64-VPL chunk, chunks combined with Kahan compensation
-//         (rel-L2 <= 2e-3). Positions, normals, the projection basis and the
+//   MIXED the flux as halves (rounded once, at VPL extraction) multiplied by
+//         f16 x f32 + f32 mixed FMAs, NEAREST decided in FP32 alone,
+//         per-pixel sums in FP32 per 64-VPL chunk, chunks combined with
+//         Kahan compensation (rel-L2 <= 2e-3). The geometry, projection and
 //         distance terms stay FP32: an FP16 basis moves texel decisions by up
-//         to 0.03 texel, which alone costs ~6e-3 rel-L2.
+//         to 0.03 texel, which alone costs ~6e-3 rel-L2, and packed-half
+//         geometry and smoothstep terms on pairs of VPLs measured slower
+//         (more scalar work pairing the VPLs than VALU saved) and 2.5x less
+//         precise.
+
+// Sums of one pixel over its class's VPLs [first, first + per_class), in
+// chunks of 64: the culling survivors of each chunk in order, the hemisphere
+// tests first, the rest of the pair only where a lane is lit. MIXED: each
+// chunk's FP32 sums are added to the totals with Kahan compensation.
+template <bool kMixed, bool kShadow, int kFetch>
+__device__ __forceinline__ void class_sums(const GatherArgs& a, const Pixel& p, const WaveBounds& wb, int first,
+                                           const LitConsts& lc, float& sr, float& sg, float& sb) {
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f;  // FP32: the sums; MIXED: this chunk's sums
+  float tr = 0.0f, tg = 0.0f, tb = 0.0f;  // MIXED: Kahan totals
+  float er = 0.0f, eg = 0.0f, eb = 0.0f;  // MIXED: Kahan compensations
+  for (int chunk = 0; chunk < a.per_class; chunk += 64) {
+    const int count = min(64, a.per_class - chunk);
+    const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
+    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb, a.cull);
+    while (mask) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      const int vi = first + chunk + j;
+      VplTest vt;
+      VplLds vf;
+      if (kFetch == kFetchWhole) {
+        vf = uniform_vpl(a.vpl, vi);
+        vt = VplTest{vf.px, vf.py, vf.pz, vf.plane, vf.nx, vf.ny, vf.nz};
+      } else {
+        vt = uniform_test(a.vpl, vi);
+      }
+      // n_vpl . (p - p_vpl) and n_p . (p - p_vpl) from the plane offsets
+      const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
+      const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
+      if (ndot > 0.0f && pdot < 0.0f) {
+        const VplLds v = (kFetch == kFetchTestFirst) ? uniform_vpl(a.vpl, vi) : vf;
+        accumulate_pair<kMixed>(a, p, vt, v, vi, ndot, pdot, kShadow, lc, ar, ag, ab);
+      }
+    }
+    if (kMixed) {
+      kahan_add(tr, er, ar);
+      kahan_add(tg, eg, ag);
+      kahan_add(tb, eb, ab);
+      ar = ag = ab = 0.0f;
+    }
+  }
+  sr = kMixed ? tr : ar;
+  sg = kMixed ? tg : ag;
+  sb = kMixed ? tb : ab;
+}
+
 template <int TW, int TH, bool kShadow, int kFetch, bool kMixed>
 __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int tiles_x,
                                                               int num_work, int per_xcd) {
@@ -454,87 +486,14 @@ __global__ __launch_bounds__(TW* TH) void final_gather_kernel(GatherArgs a, int 
 
   const int first = k * a.per_class;
   const LitConsts lc = lit_consts(a);
-  float ar = 0.0f, ag = 0.0f, ab = 0.0f;     // FP32 form: the sums; MIXED: this chunk's sums
-  float tr = 0.0f, tg = 0.0f, tb = 0.0f;     // MIXED: Kahan totals
-  float er = 0.0f, eg = 0.0f, eb = 0.0f;     // MIXED: Kahan compensations
-
-  for (int chunk = 0; chunk < a.per_class; chunk += 64) {
-    const int count = min(64, a.per_class - chunk);
-    const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-    if (!wave_live) continue;
-    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb, a.cull);
-    while (mask) {
-      const int j = __builtin_ctzll(mask);
-      mask &= mask - 1ull;
-      const int vi = first + chunk + j;
-      VplTest vt;
-      VplLds vf;
-      if (kFetch == kFetchWhole) {
-        vf = uniform_vpl<kMixed>(a.vpl, vi);
-        vt = VplTest{vf.px, vf.py, vf.pz, vf.plane, vf.nx, vf.ny, vf.nz};
-      } else {
-        vt = uniform_test(a.vpl, vi);
-      }
-      // n_vpl . (p - p_vpl) and n_p . (p - p_vpl) from the plane offsets
-      const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
-      const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
-      if (ndot > 0.0f && pdot < 0.0f) {
-        const VplLds v = (kFetch == kFetchTestFirst) ? uniform_vpl<kMixed>(a.vpl, vi) : vf;
-        accumulate_pair<kMixed>(a, p, vt, v, vi, ndot, pdot, kShadow, lc, ar, ag, ab);
-      }
-    }
-    if (kMixed) {
-      kahan_add(tr, er, ar);
-      kahan_add(tg, eg, ag);
-      kahan_add(tb, eb, ab);
-      ar = ag = ab = 0.0f;
-    }
-  }
-  if (kMixed) {
-    ar = tr;
-    ag = tg;
-    ab = tb;
-  }
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+  if (wave_live) class_sums<kMixed, kShadow, kFetch>(a, p, wb, first, lc, ar, ag, ab);
   if (p.valid) {
     float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
     gstore(o, ar * a.out_scale);
     gstore(o + 1, ag * a.out_scale);
     gstore(o + 2, ab * a.out_scale);
   }
-}
-
-// Sums of one batched item's pixel over its class's VPLs, with the
-// single-frame kernel's arithmetic and order (MIXED: per-chunk sums combined
-// with Kahan compensation).
-template <bool kMixed, bool kShadow>
-__device__ __forceinline__ void batch_item_sums(const GatherArgs& a, const Pixel& p, const WaveBounds& wb,
-                                                int first, const LitConsts& lc, float& sr, float& sg, float& sb) {
-  float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-  float tr = 0.0f, tg = 0.0f, tb = 0.0f, er = 0.0f, eg = 0.0f, eb = 0.0f;
-  for (int chunk = 0; chunk < a.per_class; chunk += 64) {
-    const int count = min(64, a.per_class - chunk);
-    const float* crec = a.vpl + (size_t)(first + chunk) * kVplFloats;
-    unsigned long long mask = cull_chunk<kMixed>(crec, count, wb, a.cull);
-    while (mask) {
-      const int j = __builtin_ctzll(mask);
-      mask &= mask - 1ull;
-      const VplLds v = uniform_vpl<kMixed>(a.vpl, first + chunk + j);
-      const VplTest vt{v.px, v.py, v.pz, v.plane, v.nx, v.ny, v.nz};
-      const float ndot = __builtin_fmaf(vt.nx, p.fx, __builtin_fmaf(vt.ny, p.fy, __builtin_fmaf(vt.nz, p.fz, -vt.plane)));
-      const float pdot = __builtin_fmaf(-p.nx, vt.px, __builtin_fmaf(-p.ny, vt.py, __builtin_fmaf(-p.nz, vt.pz, p.plane)));
-      if (ndot > 0.0f && pdot < 0.0f)
-        accumulate_pair<kMixed>(a, p, vt, v, first + chunk + j, ndot, pdot, kShadow, lc, ar, ag, ab);
-    }
-    if (kMixed) {
-      kahan_add(tr, er, ar);
-      kahan_add(tg, eg, ag);
-      kahan_add(tb, eb, ab);
-      ar = ag = ab = 0.0f;
-    }
-  }
-  sr = kMixed ? tr : ar;
-  sg = kMixed ? tg : ag;
-  sb = kMixed ? tb : ab;
 }
 
 // Batched gather: one wave64 workgroup per (frame, class, strip) item, over
@@ -563,8 +522,10 @@ __device__ __forceinline__ void batch_item_sums(const GatherArgs& a, const Pixel
 // chunk's culling records before the pixel loads, to overlap the two
 // latencies, measured 9% slower at 4K and 15% slower for FP32: the records
 // stay live across the pixel reconstruction and the bound reductions.)
+// Occupancy: eight waves per SIMD (64 VGPRs) for MIXED; the FP32 form's
+// exact-texel path needs 67, which spill at eight.
 template <bool kMixed>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMixed ? 8 : 7))) void batch_gather_kernel(const BatchFrame* __restrict__ frames, int n,
                                                            uint32_t total, int chunk_log2) {
   const int lane = threadIdx.x;
   // One single-wave workgroup per item, dispatched by the hardware as waves
@@ -598,11 +559,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void ba
     const LitConsts lc = lit_consts(a);
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;
     // the frame's shadowing flag picks one of two instances of the VPL loop
-    // (a scalar branch), so that the lit path holds no per-lane copy of it
+    // (a scalar branch), so that the lit path holds no per-lane copy of it;
+    // the loop is the single-frame kernel's with the whole-record fetch
     if (__builtin_amdgcn_readfirstlane(a.shadowing))
-      batch_item_sums<kMixed, true>(a, p, wb, first, lc, ar, ag, ab);
+      class_sums<kMixed, true, kFetchWhole>(a, p, wb, first, lc, ar, ag, ab);
     else
-      batch_item_sums<kMixed, false>(a, p, wb, first, lc, ar, ag, ab);
+      class_sums<kMixed, false, kFetchWhole>(a, p, wb, first, lc, ar, ag, ab);
     if (p.valid) {
       float* o = a.out + ((size_t)p.y * a.width + p.x) * 3;
       gstore(o, ar * a.out_scale);

@@ -5,7 +5,8 @@
 // One lane per VPL. Each lane reads its RSM texel (RGB diffuse, encoded
 // normal, depth), unprojects with inverse(bias*P*V) using an explicit fused
 // multiply-add chain, quantises the normal through the 10:10:10 packing, and
-// also stores the lookAtRH basis so the splat and gather never recompute it.
+// also stores the lookAtRH basis so the splat and gather never recompute it,
+// and the flux rounded to halves for the mixed-form gather.
 #include "batch_device.hpp"
 #include "kernels.hpp"
 #include "mlgi_device.hpp"
@@ -64,9 +65,14 @@ __device__ __forceinline__ void vpl_extract_one(const float* __restrict__ diffus
   const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
   out[0] = make_float4(px, py, pz, plane);
   out[1] = make_float4(un.x, un.y, un.z, __uint_as_float(pk));
-  out[2] = make_float4(s.x, s.y, s.z, 0.f);
-  out[3] = make_float4(u.x, u.y, u.z, 0.f);
-  out[4] = make_float4(dr * light_intensity, dg * light_intensity, db * light_intensity, 0.f);
+  const float fr = dr * light_intensity, fg = dg * light_intensity, fb = db * light_intensity;
+  // the flux as halves (round to nearest even) for the mixed-form gather
+  const uint32_t hrg = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)fr) |
+                       ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)fg) << 16);
+  const uint32_t hb = __builtin_bit_cast(uint16_t, (_Float16)fb);
+  out[2] = make_float4(s.x, s.y, s.z, __uint_as_float(hrg));
+  out[3] = make_float4(u.x, u.y, u.z, __uint_as_float(hb));
+  out[4] = make_float4(fr, fg, fb, 0.f);
   out[5] = zero;
   out[6] = zero;
   out[7] = zero;

@@ -28,7 +28,7 @@ __all__ = [
     "MlgiError", "Params", "PRECISION_FP32", "PRECISION_MIXED", "VPL_RECORD_FLOATS",
     "Context", "DeviceArray", "Scene", "Camera", "VPLProcessor", "ImperfectShadowmap",
     "GIStage", "default_params", "shuffle_indices", "ism_indices_1d", "device_count",
-    "vpl_fields",
+    "vpl_fields", "fill_flux_halves",
 ]
 
 lib = N.lib
@@ -44,7 +44,18 @@ def vpl_fields(rec: np.ndarray) -> dict:
         "s": rec[:, 8:11],
         "u": rec[:, 12:15],
         "flux": rec[:, 16:19],
+        "flux_halves": rec[:, [11, 15]].view(np.uint32),
     }
+
+
+def fill_flux_halves(rec: np.ndarray) -> np.ndarray:
+    """Writes record slots 11 and 15 (the flux as binary16 bits, round to
+    nearest even: r | g << 16 and b) from slots 16..18, as VPL extraction
+    does; records built by hand need them for the mixed-form gather."""
+    h = rec[:, 16:19].astype(np.float16).view(np.uint16).astype(np.uint32)
+    rec[:, 11] = (h[:, 0] | (h[:, 1] << 16)).view(np.float32)
+    rec[:, 15] = h[:, 2].view(np.float32)
+    return rec
 
 
 def default_params(**kw) -> Params:

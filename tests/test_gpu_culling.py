@@ -115,6 +115,7 @@ def plane_case(ctx, seed):
     flux = rng.uniform(0.1, 3.0, (n, 3)).astype(np.float32)
     flux[rng.random(n) < 0.1] = 0.0  # zero-flux VPLs (dropped by the culling)
     rec[:, 16:19] = flux
+    mlgi.fill_flux_halves(rec)
     p = mlgi.default_params(interleave=il, use_interleaving=1, enable_shadowing=0, atlas_side=1024,
                             vpl_end_index=n, z_far=50.0, tile_w=16, tile_h=16, vpl_clamping_value=10.0)
     st = mlgi.GIStage(ctx, p, n, w, h)

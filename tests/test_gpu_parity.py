@@ -2,7 +2,8 @@
 
 Bars (stage-wise: each oracle stage consumes the GPU's previous stage):
   * VPL records: bit-exact vs the FP32 twin (positions, packed and unpacked
-    normals, flux, lookAtRH basis); positions within 1e-5 of FP64.
+    normals, flux and its halves, lookAtRH basis); positions within 1e-5 of
+    FP64.
   * ISM atlas: bit-exact vs the FP32 twin (integer atomicMin is order-free).
   * Pull-push R16: bit-exact vs the FP32 twin.
   * Irradiance: rel-L2 <= 1e-4 vs the FP64 transcription (plain FP32 form),
@@ -80,6 +81,10 @@ def test_vpl_extract_bit_exact(frame):
     assert np.array_equal(r[:, 7].view(np.uint32), pk)
     assert np.array_equal(r[:, 4:7], nrm)
     assert np.array_equal(r[:, 16:19], col)
+    # the flux as halves (slots 11 and 15), round to nearest even
+    h = col.astype(np.float16).view(np.uint16).astype(np.uint32)
+    assert np.array_equal(r[:, 11].view(np.uint32), h[:, 0] | (h[:, 1] << 16))
+    assert np.array_equal(r[:, 15].view(np.uint32), h[:, 2])
     f = mlgi.vpl_fields(r)
     n, x = f["normal"], f["position"]
     plane = n[:, 0] * x[:, 0] + n[:, 1] * x[:, 1] + n[:, 2] * x[:, 2]  # float32, unfused
