@@ -16,8 +16,11 @@ Four forms run on the same inputs:
                 (mlgi_batch_run of the one frame)
   mixed         the mixed-precision form through the batched kernels
 The timed region is exactly K frames of a form bracketed by a device
-synchronize on both sides; ms_per_step is wall time / K; HIP events on the
-context stream bracket every stage of every timed frame. `value` is the
+synchronize on both sides, with no timing events inside; ms_per_step is
+wall time / K. A second timed region of the same K frames then records HIP
+events on the context stream around every stage of every frame (the
+per-stage spans and the kernels' rooflines come from it; its wall time per
+frame is reported as profiled_ms_per_frame). `value` is the
 mixed form through the faster of its two paths in this run (both drop-in
 entry points for one frame; config.form names it; the other is reported
 under forms). Batched forms are also
@@ -38,7 +41,10 @@ plus 4 bytes per atlas texel written). Every kernel's figure is under
 "kernels"; "traffic" is the HBM bytes per launch measured with rocprofv3
 counters (profiles/traffic_<workload>.json), null until measured.
 cpu_baseline: the oracle's FP32 instantiation of the same four stages on the
-host for one full frame (final gathering on --cpu-threads threads).
+host for one full frame, splat, pull-push and final gathering on
+--cpu-threads host threads (default: OMP_NUM_THREADS, else the CPUs this
+process may run on). The reference shaders under Mesa llvmpipe are not
+measured (no OpenGL context can be created in this image); the line says so.
 
 Other workloads (one JSON line each, not the default):
   small     512x512, 1,024 VPLs, 256k points, 2048^2 atlas, 4x4 interleave
@@ -70,6 +76,9 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction
 VALU_PEAK_GINST_S = 614.4
 HBM_PEAK_GBS = 8000.0
+# the served workload's stated p99 latency budget (arrival to result), at the
+# configured 200 requests/s
+SERVE_P99_BUDGET_MS = 5.0
 DATA = "synthetic: seeded procedural room, ray-cast G-buffer/RSM, area-proportional points"
 
 
@@ -100,16 +109,24 @@ def run_form(ctx, wl, inp, form: str, steps: int, warmup: int):
     for _ in range(warmup):
         step()
     ctx.sync()
+    # the headline region: K frames, no events on the stream
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    # the same K frames with HIP events around every stage (stage spans and
+    # kernel rooflines)
     ctx.profile(True)
     ctx.profile_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     ctx.sync()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    prof_ms = (time.perf_counter() - t0) * 1e3 / steps
     stages = ctx.profile_read()
     ctx.profile(False)
-    out = {"st": st, "wall_ms": wall_ms, "stages": stages, "lit": st.count_lit_pairs(gb),
+    out = {"st": st, "wall_ms": wall_ms, "profiled_ms": prof_ms, "stages": stages, "lit": st.count_lit_pairs(gb),
            "path": "batched (mlgi_batch_run)" if batch else "single frame (mlgi_frame)"}
     if batch:
         g = batch.capture()
@@ -140,9 +157,10 @@ def valu_roof(kernel_ms, sq, peak_fma_s):
             "peak_source": "measured FMA probe" if peak_fma_s else "spec"}
 
 
-def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None, single=False):
+def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None, single=False, splats=None, spills=None):
     """Per-kernel rooflines of one form's timed frames; `single` selects the
-    single-frame kernels' counters (profile keys with a _single suffix)."""
+    single-frame kernels' counters (profile keys with a _single suffix);
+    splats / spills: the frame's LDS and global atomic mins of the splat."""
     sq = sq or {}
     sfx = "_single" if single else ""
     a2 = wl.atlas_side * wl.atlas_side
@@ -163,6 +181,12 @@ def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None, single=False):
         out["ism_splat"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": b,
                             "kernel_ms": round(stages["ism_splat_ms"], 4)}
+        if splats is not None:
+            # the tile pass's LDS atomic mins (one per accepted (point, ISM)
+            # splat) over the whole splat span (mask + tiles + spill)
+            out["ism_splat"]["lds_atomic_min_per_launch"] = splats
+            out["ism_splat"]["lds_atomics_per_s"] = round(splats / t, 1)
+            out["ism_splat"]["global_atomic_min_per_launch"] = spills
     t = stages["final_gather_ms"] * 1e-3
     if t > 0:
         tf = FLOPS_PER_LIT_PAIR * lit / t / 1e12
@@ -172,6 +196,19 @@ def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None, single=False):
                                "kernel_ms": round(stages["final_gather_ms"], 4),
                                "valu": valu_roof(stages["final_gather_ms"], sq.get("final_gather" + sfx), peak_fma_s)}
     return out
+
+
+def host_threads() -> int:
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the
+    process's share of the host), else the CPUs this process may run on."""
+    n = os.environ.get("OMP_NUM_THREADS", "")
+    if n.isdigit() and int(n) > 0:
+        return int(n)
+    return len(os.sched_getaffinity(0))
+
+
+LLVMPIPE = ("not measured: the reference shaders need an OpenGL 4.3 context, and this image has Mesa's "
+            "llvmpipe libraries but no X server, EGL or OSMesa to create one")
 
 
 def cpu_baseline(wl, inp, threads: int):
@@ -187,10 +224,10 @@ def cpu_baseline(wl, inp, threads: int):
                                            p.light_intensity, sh, wl.n_vpl, np.float32)
     t["vpl"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    atlas = oracle.ism_splat(inp.points, inp.bin_offsets, pos, nrm, wl.atlas_side, p.z_far)
+    atlas = oracle.ism_splat(inp.points, inp.bin_offsets, pos, nrm, wl.atlas_side, p.z_far, threads=threads)
     t["splat"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    r16 = oracle.pullpush(atlas, wl.ism_side, wl.levels, p.z_far)
+    r16 = oracle.pullpush(atlas, wl.ism_side, wl.levels, p.z_far, threads=threads)
     t["pullpush"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     oracle.final_gather(inp.gb_depth, inp.gb_normal, inp.vp_inv, pos, nrm, col, r16, wl.atlas_side, p.z_far,
@@ -199,9 +236,9 @@ def cpu_baseline(wl, inp, threads: int):
     t["gather"] = time.perf_counter() - t0
     total_ms = sum(t.values()) * 1e3
     return {"value": round(total_ms, 3), "unit": "ms/frame", "cores": threads, "kind": "port",
-            "sample": (f"one full {wl.name} frame: oracle FP32 instantiation; final gathering on {threads} "
-                       "threads, VPL extraction/splat/pull-push on 1 thread"),
-            "stages_s": {k: round(v, 4) for k, v in t.items()}}
+            "sample": (f"one full {wl.name} frame: oracle FP32 instantiation; splat, pull-push and final "
+                       f"gathering on {threads} host threads, VPL extraction on 1"),
+            "stages_s": {k: round(v, 4) for k, v in t.items()}, "llvmpipe": LLVMPIPE}
 
 
 def config_of(wl, form):
@@ -234,7 +271,16 @@ def frame_bench(a, ctx, wl):
     sq_path = ROOT / "profiles" / f"sq_counters_{wl.name}.json"
     if sq_path.exists():
         sq = {k: v for k, v in json.loads(sq_path.read_text()).items() if not k.startswith("_")}
-    kernels = kernel_rooflines(wl, h["stages"], h["lit"], sq, peak_fma_s, single)
+    # the splat's atomics: the last eager single-frame splat of these inputs
+    # (the single-frame forms ran it through mlgi_ism_splat on this context)
+    splats = spills = None
+    eager = [f for f in ("mixed_single", "fp32") if f in res]
+    if eager:
+        ism = res[eager[0]]["st"].ism
+        splats = ism.splat_count()
+        spills = ism.spill_stats()[0]
+        log(f"[bench] splat: {splats} LDS atomic mins, {spills} spilled to global atomics")
+    kernels = kernel_rooflines(wl, h["stages"], h["lit"], sq, peak_fma_s, single, splats, spills)
     dominant = max(kernels, key=lambda k: kernels[k]["kernel_ms"])
     roof = dict(kernels[dominant])
     roof["kernel"] = dominant
@@ -247,6 +293,7 @@ def frame_bench(a, ctx, wl):
     forms_out = {}
     for k, v in res.items():
         forms_out[k] = {"ms_per_frame": round(v["wall_ms"], 4), "path": v["path"],
+                        "profiled_ms_per_frame": round(v["profiled_ms"], 4),
                         "stages_ms": {s: round(x, 4) for s, x in v["stages"].items() if s != "frames"}}
         if "graph_ms" in v:
             forms_out[k]["graph_ms_per_frame"] = round(v["graph_ms"], 4)
@@ -280,7 +327,7 @@ def frame_bench(a, ctx, wl):
             out["mixed_speedup_same_path_graph"] = round(g32 / gmx, 4)
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(wl, inp, a.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(wl, inp, a.cpu_threads or host_threads())
     return out
 
 
@@ -345,40 +392,13 @@ def stress_bench(a, ctx, wl):
 
 def batch_bench(a, ctx):
     """64 frames of the 40k scene with random shapes and light poses, autotuned,
-    one batch replayed from a HIP graph."""
-    import numpy as np
+    one batch replayed from a HIP graph (mlgi.workloads.Batch64)."""
     import mlgi
     from mlgi import workloads as W
 
-    rng = np.random.default_rng(2024)
-    scene = W._scene(1, 40_000)
-    dev = scene.upload(ctx)
-    sizes = [128, 256, 512, 1024]
-    vpls = [256, 1024, 2048, 4096]
-    points, gbufs, frames = {}, {}, []
-    for i in range(64):
-        s = int(rng.choice(sizes))
-        n = int(rng.choice(vpls))
-        ism = 32 if n <= 1024 else 64
-        side = mlgi.ism_indices_1d(n) * ism
-        if n not in points:
-            rec, off = scene.sample_points(1_000_000, n, n, 11)
-            points[n] = mlgi.points_desc(ctx.upload(rec), ctx.upload(off))
-        if s not in gbufs:
-            d, nm = ctx.empty((s, s), np.float32), ctx.empty((s, s, 3), np.float32)
-            inv = dev.render_gbuffer(mlgi.Camera.default(1.0), d, nm)
-            gbufs[s] = (d, nm, mlgi.gbuffer_desc(d, nm, inv))
-        p = mlgi.default_params(atlas_side=side, pull_push_levels=3 if ism == 32 else 4, vpl_end_index=n,
-                                z_far=50.0, precision=mlgi.PRECISION_MIXED)
-        st = mlgi.GIStage(ctx, p, n, s, s)
-        dif, nrm, dep = (ctx.empty((128, 128, 3), np.float32), ctx.empty((128, 128, 3), np.float32),
-                         ctx.empty((128, 128), np.float32))
-        inv = dev.render_rsm(mlgi.Camera.light(5000 + i), dif, nrm, dep)
-        rsm = mlgi.rsm_desc(dif, nrm, dep, inv)
-        st.rsm_buffers = (dif, nrm, dep)
-        tune = mlgi.autotune(ctx, st, rsm, points[n], gbufs[s][2])
-        frames.append((st, rsm, points[n], gbufs[s][2], s, n, tune))
-    batch = mlgi.Batch(ctx, [(st, rsm, pts, gb) for st, rsm, pts, gb, _, _, _ in frames])
+    log("[bench] building the 64-frame batch (device G-buffers and RSMs, autotuning) ...")
+    b64 = W.Batch64(ctx)
+    batch = mlgi.Batch(ctx, b64.frames)
     g = batch.capture()
     for _ in range(a.warmup):
         g.launch()
@@ -389,17 +409,18 @@ def batch_bench(a, ctx):
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / a.steps
     tuned = {}
-    for _, _, _, _, s, n, tune in frames:
-        tuned[f"{s}x{s}/{n}"] = {"interleave": tune["interleave"], "tile": "%dx%d" % tuple(tune["tile"])}
+    for sp, tune in zip(b64.specs, b64.tune):
+        tuned[f"{sp.size}x{sp.size}/{sp.n_vpl}"] = {"interleave": tune["interleave"], "tile": "%dx%d" % tuple(tune["tile"])}
     g.close()
     batch.close()
-    dev.close()
+    b64.close()
     return {"metric": "GI ms per batch of 64 mixed-shape frames (mixed, batched kernels, graph replay)",
             "value": round(ms, 4), "unit": "ms/batch", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
             "ms_per_step": round(ms, 4), "ms_per_frame": round(ms / 64, 4), "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp16x2+fp32", "data": DATA,
-            "config": {"workload": "batch64", "frames": 64, "resolutions": sizes, "vpls": vpls,
-                       "triangles": 40_012, "points": 1_000_000, "autotuned": tuned}}
+            "config": {"workload": "batch64", "frames": 64, "resolutions": list(W.BATCH64_SIZES),
+                       "vpls": list(W.BATCH64_VPLS), "triangles": 40_012, "points": W.Batch64.POINTS,
+                       "autotuned": tuned}}
 
 
 def serve_bench(a, ctx):
@@ -414,31 +435,139 @@ def serve_bench(a, ctx):
     log("[bench] server create (resident scenes, point binnings, G-buffers, tuning) ...")
     t0 = time.perf_counter()
     srv = mlgi.Server(ctx, scenes, vpls, [(s, s) for s in sizes], hbm_budget=a.budget_gib << 30,
-                      max_inflight=64, max_batch=32, max_queued_batches=3)
+                      max_inflight=64, max_batch=32, max_queued_batches=3, streams=3)
     log(f"[bench] server ready in {time.perf_counter() - t0:.1f}s")
-    rng = np.random.default_rng(7)
-    t, reqs = 0.0, []
-    while True:
-        t += rng.exponential(1.0 / a.rate)
-        if t >= a.seconds:
-            break
-        s = int(rng.choice(sizes))
-        reqs.append((int(rng.integers(3)), s, s, int(rng.choice(vpls)), int(rng.integers(1 << 40)), t))
+
+    def trace(rate, seconds, seed):
+        rng = np.random.default_rng(seed)
+        t, reqs = 0.0, []
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t >= seconds:
+                return reqs
+            s = int(rng.choice(sizes))
+            reqs.append((int(rng.integers(3)), s, s, int(rng.choice(vpls)), int(rng.integers(1 << 40)), t))
+
+    reqs = trace(a.rate, a.seconds, 7)
     log(f"[bench] serving {len(reqs)} requests over {a.seconds:.0f} s ...")
     st, lat = srv.run(reqs)
+    # higher arrival rates (short traces): where batching and overlap engage
+    sweep = []
+    for rate in (500.0, 1000.0, 2000.0, 4000.0, 8000.0):
+        r = trace(rate, 2.0, 100 + int(rate))
+        s2, _ = srv.run(r)
+        sweep.append({"arrival_rate_per_s": rate, "requests": len(r), "req_per_s": round(s2["req_per_s"], 2),
+                      "latency_p50_ms": round(s2["latency_p50_ms"], 3), "latency_p99_ms": round(s2["latency_p99_ms"], 3),
+                      "mean_batch": round(s2["mean_batch"], 3), "batches": s2["batches"],
+                      "overlapped_batches": s2["overlapped_batches"]})
+        log(f"[bench] serve sweep {sweep[-1]}")
     srv.close()
+    engages = [x["arrival_rate_per_s"] for x in sweep if x["mean_batch"] > 1.05]
     return {"metric": "served GI req/s + p99 latency (3 resident scenes, Poisson arrivals, mixed form)",
             "value": round(st["req_per_s"], 3), "unit": "req/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
             "ms_per_step": round(st["latency_mean_ms"], 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "fp16x2+fp32", "data": DATA,
             "latency_ms": {"p50": round(st["latency_p50_ms"], 3), "p99": round(st["latency_p99_ms"], 3),
                            "mean": round(st["latency_mean_ms"], 3), "max": round(st["latency_max_ms"], 3)},
+            "latency_budget_ms": {"p99": SERVE_P99_BUDGET_MS, "met": st["latency_p99_ms"] <= SERVE_P99_BUDGET_MS},
             "server": {k: st[k] for k in ("completed", "rejected", "batches", "mean_batch", "max_inflight_seen",
-                                          "peak_bytes", "wall_s")},
+                                          "peak_bytes", "wall_s", "overlapped_batches", "streams")},
+            "rate_sweep": sweep, "batching_engages_at_per_s": engages[0] if engages else None,
             "config": {"workload": "serve", "arrival_rate_per_s": a.rate, "seconds": a.seconds,
                        "scenes_triangles": [4_012, 40_012, 160_012],
                        "scene_points": [256_000, 1_000_000, 4_000_000], "resolutions": sizes, "vpls": vpls,
                        "hbm_budget_gib": a.budget_gib, "rsm": 128}}
+
+
+# ---------------------------------------------------------------- summaries
+# rocprofv3 output -> the summaries under profiles/ (no GPU needed).
+# Profile keys and the kernel each one names (batched kernels; _single: the
+# single-frame entry point's kernels of the mixed form).
+PROFILE_KERNELS = {
+    "pullpush": "batch_pullpush_kernel<", "final_gather": "batch_gather_kernel<true>",
+    "final_gather_fp32": "batch_gather_kernel<false>", "ism_mask": "batch_mask_kernel",
+    "ism_tiles": "batch_tiles_kernel", "ism_spill": "batch_spill_kernel", "vpl": "batch_vpl_kernel",
+    "pullpush_single": "pullpush_kernel<", "final_gather_single": "final_gather_kernel<16, 16, true, 0, true>",
+    "final_gather_fp32_single": "final_gather_kernel<16, 16, true, 0, false>", "ism_mask_single": "ism_mask_kernel",
+    "ism_tiles_single": "ism_tiles_kernel", "ism_spill_single": "ism_spill_kernel", "vpl_single": "vpl_extract_kernel",
+}
+
+
+def short_kernel_name(name: str) -> str:
+    """'void mlgi::(anonymous namespace)::batch_gather_kernel<true>(args)' -> 'batch_gather_kernel<true>'."""
+    n = name.replace("(anonymous namespace)", "").split("(")[0]
+    return n.replace("void ", "").split("::")[-1].strip()
+
+
+def counter_dispatches(root: str) -> dict:
+    """{kernel: {dispatch: {counter: value}}} of a --pmc run."""
+    import csv
+    import glob
+    from collections import defaultdict
+    out = defaultdict(lambda: defaultdict(lambda: defaultdict(float)))
+    for f in glob.glob(f"{root}/**/*counter_collection.csv", recursive=True):
+        for r in csv.DictReader(open(f)):
+            out[short_kernel_name(r["Kernel_Name"])][(f, r["Dispatch_Id"])][r["Counter_Name"]] += float(
+                r["Counter_Value"])
+    return out
+
+
+def counter_means(root: str) -> dict:
+    """{kernel: {"dispatches": n, counter: mean per dispatch}} of a --pmc run."""
+    out = {}
+    for k, disp in counter_dispatches(root).items():
+        names = sorted({c for d in disp.values() for c in d})
+        out[k] = {"dispatches": len(disp), **{c: sum(d.get(c, 0.0) for d in disp.values()) / len(disp) for c in names}}
+    return out
+
+
+def profile_rows(means: dict) -> dict:
+    out = {}
+    for key, prefix in PROFILE_KERNELS.items():
+        hits = [k for k in means if k.startswith(prefix)]
+        if hits:
+            out[key] = {"kernel": hits[0], **means[hits[0]]}
+    return out
+
+
+def summarize(kind: str, dirs: list, how: str) -> dict:
+    """kind sq: SQ counters per profile key; traffic: FETCH_SIZE (dirs[0])
+    and WRITE_SIZE (dirs[1]) in bytes per launch, each scaled by what the
+    counter reads for the --calibrate-copy-mib copy (dirs[2]: its MiB)."""
+    if kind == "sq":
+        return {"_how": how, **profile_rows(counter_means(dirs[0]))}
+    fetch, write, mib = counter_means(dirs[0]), counter_means(dirs[1]), int(dirs[2])
+    copy = next(k for k in fetch if "copyBuffer" in k)
+    kib = mib * 1024.0
+    # the calibration copy: the largest single dispatch of the copy kernel
+    fk = max(d["FETCH_SIZE"] for d in counter_dispatches(dirs[0])[copy].values())
+    wk = max(d["WRITE_SIZE"] for d in counter_dispatches(dirs[1])[copy].values())
+    fs, ws = kib / fk, kib / wk
+    out = {"_how": how, "_calibration": {"copy_kernel": copy, "fetch_kib_read": fk, "fetch_scale": round(fs, 4),
+                                         "write_kib_read": wk, "write_scale": round(ws, 4)}}
+    for key, row in profile_rows(fetch).items():
+        w = write.get(row["kernel"], {}).get("WRITE_SIZE")
+        if w is None:
+            continue
+        f = row["FETCH_SIZE"]
+        out[key] = {"kernel": row["kernel"], "fetch_kib": round(f, 1), "write_kib": round(w, 1),
+                    "bytes_per_launch": int(round((f * fs + w * ws) * 1024))}
+    return out
+
+
+def trace_table(root: str) -> str:
+    """Markdown table of a --kernel-trace --stats run (its kernel_stats.csv)."""
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(f"{root}/**/*kernel_stats.csv", recursive=True):
+        rows += list(csv.DictReader(open(f)))
+    rows.sort(key=lambda r: -float(r["Percentage"]))
+    out = ["| kernel | calls | avg us | min us | max us | % of GPU time |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        out.append(f"| `{short_kernel_name(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                   f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
 
 
 def main() -> int:
@@ -449,14 +578,24 @@ def main() -> int:
     ap.add_argument("--workload", default="1080p", choices=["1080p", "small", "4k", "batch64", "serve"])
     ap.add_argument("--form", default="both", choices=["fp32", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: host_threads())")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate (req/s)")
     ap.add_argument("--seconds", type=float, default=60.0, help="serve: length of the arrival trace")
     ap.add_argument("--budget-gib", type=int, default=16, help="serve: HBM budget of admitted requests")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the frames, copy this many MiB device-to-device (a plain copy kernel of "
                          "known size for scaling the HBM traffic counters)")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="ARG",
+                    help="no GPU: summarise rocprofv3 output for profiles/ and exit: 'sq DIR' (SQ counters), "
+                         "'traffic FETCH_DIR WRITE_DIR COPY_MIB' or 'trace DIR' (kernel-trace stats table)")
+    ap.add_argument("--how", default="", help="--pmc-summary: the commands that made the input, for the summary")
     a = ap.parse_args()
+    if a.pmc_summary:
+        if a.pmc_summary[0] == "trace":
+            print(trace_table(a.pmc_summary[1]))
+        else:
+            print(json.dumps(summarize(a.pmc_summary[0], a.pmc_summary[1:], a.how), indent=1))
+        return 0
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
