@@ -48,8 +48,15 @@ typedef enum mlgi_status {
 } mlgi_status;
 
 typedef enum mlgi_precision {
-  MLGI_PRECISION_FP32 = 0,  /* plain FP32 final gathering */
-  MLGI_PRECISION_MIXED = 1  /* packed-FP16 geometry/flux terms, FP32 compensated sums */
+  /* plain FP32 final gathering; NEAREST texels within 2.5e-4 texel of an
+   * edge re-decided exactly (float-float arithmetic, no FP64); rel-L2 <= 1e-4
+   * against an FP64 evaluation */
+  MLGI_PRECISION_FP32 = 0,
+  /* the flux as binary16 (record slots 11 and 15) in f16 x f32 mixed FMAs,
+   * NEAREST decided in FP32 alone, per-pixel FP32 sums per 64 VPLs combined
+   * with Kahan compensation; geometry, projection and shadow terms FP32;
+   * rel-L2 <= 2e-3 */
+  MLGI_PRECISION_MIXED = 1
 } mlgi_precision;
 
 typedef struct mlgi_context mlgi_context;
@@ -64,8 +71,10 @@ typedef struct mlgi_params {
   float vpl_clamping_value;       /* geometry-term clamp, default 1e-3 */
   int32_t vpl_start_index;        /* vplStartIndex (normalisation only) */
   int32_t vpl_end_index;          /* vplEndIndex; <=0 means vpl count */
-  int32_t scale_isms;             /* must be 0 (MLGI_ERR_UNSUPPORTED otherwise) */
-  int32_t points_only_into_scaled_isms; /* must be 0 */
+  int32_t scale_isms;             /* must be 0 (MLGI_ERR_UNSUPPORTED otherwise); ISMs
+                                     for a VPL sub-range: pass that sub-range as the
+                                     VPL set (records + 32 * start, end - start) */
+  int32_t points_only_into_scaled_isms; /* must be 0 (likewise) */
   float tess_level_factor;        /* accepted for interface parity; no effect */
   int32_t use_push_pull;          /* must be 1: the point-splat ISM path */
   int32_t enable_shadowing;       /* ENABLE_SHADOWING */

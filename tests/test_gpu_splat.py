@@ -153,3 +153,37 @@ def test_small_vpl_counts_and_ism_sides(ctx, n_vpl, atlas):
     assert (ref != 0xFFFFFFFF).any()
     assert np.array_equal(outs[1], ref)
     assert np.array_equal(outs[0], ref)
+
+
+def test_forced_drain_of_a_long_sparse_bin(ctx):
+    """One VPL (ISM side 64) and one bin of 20,000 points of which about 2%
+    face it: the ISM-major splat's queue then carries fewer than 64 entries
+    from one 128-point step to the next, so the forced drain every 64 steps
+    (which keeps the 16-bit queue codes within their bits) runs twice, at
+    points 8,192 and 16,384. Bit-exact against the oracle, with the
+    point-major splat alongside."""
+    rng = np.random.default_rng(77)
+    n = 20_000
+    rec = np.zeros((1, mlgi.VPL_RECORD_FLOATS), np.float32)
+    nrm = np.array([0.0, 0.0, 1.0], np.float32)
+    rec[0, 4:7] = nrm
+    rec[0, 8:11], rec[0, 12:15] = oracle.look_at_rh(nrm, np.float32)
+    lit = rng.random(n) < 0.02
+    pts = np.zeros((n, 4), np.float32)
+    pts[:, 0:2] = rng.uniform(-4.0, 4.0, (n, 2))
+    pts[:, 2] = np.where(lit, rng.uniform(0.5, 4.0, n), rng.uniform(-4.0, -0.5, n))  # z < 0: behind the VPL
+    for k in range(n):
+        d = -pts[k, :3] / np.linalg.norm(pts[k, :3])  # facing the VPL
+        pn = d if lit[k] else -d
+        pts[k, 3] = unorm4x8(*(pn * 0.5 + 0.5), rng.uniform())
+    off = np.array([0, n], np.uint32)
+    ref = oracle.ism_splat(pts, off, rec[:, 0:3], rec[:, 4:7], 64, Z_FAR)
+    assert 200 < int((ref != 0xFFFFFFFF).sum()) < 2000  # sparse
+    for blk in (0, 256):
+        vp = mlgi.VPLProcessor(ctx, 1)
+        vp.records.copy_from(rec)
+        ism = mlgi.ImperfectShadowmap(ctx, 64, 1)
+        p = mlgi.default_params(atlas_side=64, z_far=Z_FAR, splat_block=blk)
+        ism.render(mlgi.points_desc(ctx.upload(pts), ctx.upload(off)), vp, p)
+        ctx.sync()
+        assert np.array_equal(ism.softrender.numpy(), ref), blk
