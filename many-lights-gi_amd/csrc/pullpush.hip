@@ -234,9 +234,13 @@ constexpr int ism_log2() {
   return (S == 8) ? 3 : (S == 16) ? 4 : (S == 32) ? 5 : 6;
 }
 
+// unorm16 of a depth in [0, 1] (R16 storage clamps first). Every depth here
+// is an atlas depth (s >> 8) / 2^24 or a rounded weighted mean of such
+// depths, in [0, 1 + 2^-22] at worst, and anything below 1 + 2^-17 rounds to
+// 65535 as 1.0 does, so the device form leaves the clamp out.
 __host__ __device__ __forceinline__ uint16_t to_r16(float depth) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const float r = __builtin_amdgcn_fmed3f(depth, 0.0f, 1.0f);  // clamp (depth is never NaN)
+  const float r = depth;
 #else
   const float r = fminf(fmaxf(depth, 0.0f), 1.0f);
 #endif
