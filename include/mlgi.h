@@ -69,12 +69,18 @@ typedef struct mlgi_params {
   int32_t shuffle_lights;         /* permute VPLs with mt19937(1979982) shuffle */
   float gi_intensity_factor;      /* default 3000 */
   float vpl_clamping_value;       /* geometry-term clamp, default 1e-3 */
-  int32_t vpl_start_index;        /* vplStartIndex (normalisation only) */
-  int32_t vpl_end_index;          /* vplEndIndex; <=0 means vpl count */
-  int32_t scale_isms;             /* must be 0 (MLGI_ERR_UNSUPPORTED otherwise); ISMs
-                                     for a VPL sub-range: pass that sub-range as the
-                                     VPL set (records + 32 * start, end - start) */
-  int32_t points_only_into_scaled_isms; /* must be 0 (likewise) */
+  int32_t vpl_start_index;        /* vplStartIndex: the VPL range's start */
+  int32_t vpl_end_index;          /* vplEndIndex; <=0 means vpl count. The gather
+                                     normalises by end - start (all VPLs are gathered) */
+  int32_t scale_isms;             /* scaleISMs: the atlas holds one ISM per VPL of the
+                                     range, VPL v splatting into and gathered from ISM
+                                     v - start (indices outside the grid as GL treats
+                                     them: invalid atomics, repeat-wrapped sampling);
+                                     single-frame entry points only */
+  int32_t points_only_into_scaled_isms; /* pointsOnlyIntoScaledISMs: the points are
+                                     binned over the range's count and bin b splats
+                                     into VPLs ((b + i) mod n) mod count + start;
+                                     single-frame entry points only */
   float tess_level_factor;        /* accepted for interface parity; no effect */
   int32_t use_push_pull;          /* must be 1: the point-splat ISM path */
   int32_t enable_shadowing;       /* ENABLE_SHADOWING */

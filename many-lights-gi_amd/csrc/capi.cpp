@@ -120,8 +120,6 @@ int get_shuffle(mlgi_context* c, int n, const int32_t** out) {
 
 int check_params(mlgi_context* c, const mlgi_params* p) {
   if (!p) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "params is NULL");
-  if (p->scale_isms || p->points_only_into_scaled_isms)
-    return fail(c, MLGI_ERR_UNSUPPORTED, "scaleISMs / pointsOnlyIntoScaledISMs are not supported");
   if (!p->use_push_pull)
     return fail(c, MLGI_ERR_UNSUPPORTED, "usePushPull=false (raster ISM path) is not supported");
   if (!(p->z_far > 0.0f)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "z_far must be > 0");
@@ -151,6 +149,29 @@ int ensure_splat_work(mlgi_context* c, size_t bytes) {
     return fail(c, MLGI_ERR_OUT_OF_MEMORY, "hipMalloc(splat work, %zu bytes)", bytes);
   }
   c->splat_ws_bytes = bytes;
+  return MLGI_OK;
+}
+
+// The VPL range [vplStartIndex, vplEndIndex) of n VPLs and what the ISM
+// flags make of it (ism.comp:31-40; final_gathering.comp:40-45): scaleISMs
+// lays out one ISM per VPL of the range (VPL v in ISM v - start), else one
+// per VPL; pointsOnlyIntoScaledISMs bins the points over the range's VPLs.
+struct VplRange {
+  int start, end, count;
+  int ism_count, ism_offset;
+  bool points_only;
+};
+int vpl_range(mlgi_context* c, const mlgi_params* p, int n, VplRange* r) {
+  r->start = p->vpl_start_index;
+  r->end = p->vpl_end_index > 0 ? p->vpl_end_index : n;
+  r->count = r->end - r->start;
+  const bool scale = p->scale_isms != 0;
+  r->points_only = p->points_only_into_scaled_isms != 0;
+  if ((scale || r->points_only) && !(r->start >= 0 && r->count > 0 && r->end <= n))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "VPL range [%d, %d) must lie in [0, %d) for scaleISMs / "
+                "pointsOnlyIntoScaledISMs", r->start, r->end, n);
+  r->ism_count = scale ? r->count : n;
+  r->ism_offset = scale ? r->start : 0;
   return MLGI_OK;
 }
 
@@ -399,19 +420,25 @@ int ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls, co
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "points / atlas is NULL");
   if (pts->count < 0 || (pts->count > 0 && (!pts->records || !pts->bin_offsets)))
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "points buffers are NULL");
-  if (pts->count > 0 && pts->n_bins != vpls->count)
-    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "point bins (%d) must equal the VPL count (%d)",
-                pts->n_bins, vpls->count);
+  VplRange r;
+  if ((rc = vpl_range(c, p, vpls->count, &r))) return rc;
+  const int bins = r.points_only ? r.count : vpls->count;  // sampledVplCount
+  if (pts->count > 0 && pts->n_bins != bins)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "point bins (%d) must equal the %s (%d)", pts->n_bins,
+                r.points_only ? "VPL range's count" : "VPL count", bins);
   if (p->splat_block != 0 && p->splat_block != 64 && p->splat_block != 128 && p->splat_block != 256)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "splat_block %d not in {0, 64, 128, 256}", p->splat_block);
   int n1d, s;
-  if ((rc = atlas_geometry(c, atlas->side, vpls->count, &n1d, &s))) return rc;
+  if ((rc = atlas_geometry(c, atlas->side, r.ism_count, &n1d, &s))) return rc;
   if (atlas->ism_side == 0) atlas->ism_side = s;
   if (atlas->ism_side != s)
-    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas ism_side %d != %d for %d VPLs in a %d atlas",
-                atlas->ism_side, s, vpls->count, atlas->side);
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas ism_side %d != %d for %d ISMs in a %d atlas",
+                atlas->ism_side, s, r.ism_count, atlas->side);
   const size_t texels = (size_t)atlas->side * atlas->side;
-  if (p->splat_block == 0 && mlgi::ism_major_supported(s) && ((uintptr_t)atlas->texels & 15u) == 0) {
+  // the ISM-major form assumes one ISM per VPL with VPL v's points in the
+  // bins v - i: the VPL-range flags take the point-major form
+  const bool ranged = p->scale_isms || p->points_only_into_scaled_isms;
+  if (p->splat_block == 0 && !ranged && mlgi::ism_major_supported(s) && ((uintptr_t)atlas->texels & 15u) == 0) {
     // ISM-major form: writes every texel, no clear
     const uint32_t np = pts->count > 0 ? (uint32_t)pts->count : 0u;
     const size_t need = mlgi::splat_work_bytes(np);
@@ -447,7 +474,8 @@ int ism_splat(mlgi_context* c, const mlgi_points* pts, const mlgi_vpls* vpls, co
                    mlgi::launch_ism_splat(c->stream, pts->records, pts->bin_offsets, pts->n_bins,
                                           pts->count, vpls->records, vpls->count, n1d, atlas->side,
                                           p->z_far, atlas->texels,
-                                          p->splat_block > 0 ? p->splat_block : 256),
+                                          p->splat_block > 0 ? p->splat_block : 256, r.start, r.count,
+                                          r.ism_offset, r.points_only ? 1 : 0),
                    "ism_splat");
 }
 
@@ -603,11 +631,14 @@ int build_gather_args(mlgi_context* c, const mlgi_gbuffer* gb, const mlgi_vpls* 
   std::memset(a, 0, sizeof *a);
   if (p->enable_shadowing) {
     if (!ism || !ism->texels) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "ism is NULL");
+    VplRange r;
+    if ((rc = vpl_range(c, p, vpls->count, &r))) return rc;
     int n1d, s;
-    if ((rc = atlas_geometry(c, ism->side, vpls->count, &n1d, &s))) return rc;
+    if ((rc = atlas_geometry(c, ism->side, r.ism_count, &n1d, &s))) return rc;
     if (ism->ism_side != s)
-      return fail(c, MLGI_ERR_INVALID_ARGUMENT, "ism side %d != %d for %d VPLs in a %d atlas",
-                  ism->ism_side, s, vpls->count, ism->side);
+      return fail(c, MLGI_ERR_INVALID_ARGUMENT, "ism side %d != %d for %d ISMs in a %d atlas",
+                  ism->ism_side, s, r.ism_count, ism->side);
+    a->ism_offset = r.ism_offset;
     a->ism = ism->texels;
     a->atlas_side = ism->side;
     a->ism_side = s;
@@ -871,6 +902,9 @@ namespace mlgi_capi {
 int batch_frame(mlgi_context* c, const mlgi_frame_desc& d, mlgi::BatchFrame* f) {
   const mlgi_params* p = &d.params;
   int rc = check_params(c, p);
+  if (!rc && (p->scale_isms || p->points_only_into_scaled_isms))
+    return fail(c, MLGI_ERR_UNSUPPORTED,
+                "scaleISMs / pointsOnlyIntoScaledISMs run through the single-frame entry points, not batches");
   if (rc || (rc = check_vpls(c, &d.vpls))) return rc;
   const mlgi_rsm& rsm = d.rsm;
   if (!rsm.diffuse || !rsm.face_normal || !rsm.depth)

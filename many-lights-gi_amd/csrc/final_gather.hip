@@ -303,8 +303,14 @@ __device__ __forceinline__ PairSetup pair_setup(const GatherArgs& a, float px, f
     const float ud = __builtin_fmaf(v.ux, dx, __builtin_fmaf(v.uy, dy, v.uz * dz));
     const float cx = __builtin_fmaf(sd, rh, lc.hs);
     const float cy = __builtin_fmaf(ud, rh, lc.hs);
-    const int ox = (vi & (a.n1d - 1)) * a.ism_side;  // the VPL's ISM tile
-    const int oy = (vi >> a.n1d_log2) * a.ism_side;
+    // the VPL's ISM tile: ISM index vi - offset (the VPL range start with
+    // scaleISMs), tile row int(index / n1d) truncated toward zero; a negative
+    // index gives tiles left of or above the atlas, which the texel wrap maps
+    // back as GL_REPEAT does (final_gathering.comp:43-45, ism_utils.glsl:33-37)
+    const int ism = vi - a.ism_offset;
+    const int trow = (ism < 0 ? ism + a.n1d - 1 : ism) >> a.n1d_log2;
+    const int ox = (ism - trow * a.n1d) * a.ism_side;
+    const int oy = trow * a.ism_side;
     r.sbase = __builtin_fmaf(-dist, lc.k1, 1.0f);
     int tx, ty;
     if (kMixed) {  // NEAREST decided in FP32 alone

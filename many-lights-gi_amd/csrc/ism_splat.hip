@@ -58,12 +58,13 @@ struct Candidate {
   float nx, ny, nz;
   float sx, sy, sz;
   float ux, uy, uz;
-  int ism;  // ISM index == global VPL id (scaleISMs == false)
+  int ism;  // ISM index: the VPL id, minus the range start with scaleISMs
   int tile_x, tile_y;
   int pad;
 };
 
-__device__ __forceinline__ Candidate load_candidate(const float* __restrict__ vpl, int idx, int n1d) {
+__device__ __forceinline__ Candidate load_candidate(const float* __restrict__ vpl, int idx, int n1d,
+                                                    int ism_offset = 0) {
   const float4* r = reinterpret_cast<const float4*>(vpl + (size_t)idx * kVplFloats);
   const float4 r0 = gload4(r + kVplPos / 4), r1 = gload4(r + kVplNormal / 4), r2 = gload4(r + kVplS / 4),
                r3 = gload4(r + kVplU / 4);
@@ -72,10 +73,12 @@ __device__ __forceinline__ Candidate load_candidate(const float* __restrict__ vp
   c.nx = r1.x; c.ny = r1.y; c.nz = r1.z;
   c.sx = r2.x; c.sy = r2.y; c.sz = r2.z;
   c.ux = r3.x; c.uy = r3.y; c.uz = r3.z;
-  c.ism = idx;
-  // int(ismIndex / ismIndices1d), exact for idx < 2^24
-  c.tile_y = idx / n1d;
-  c.tile_x = idx - c.tile_y * n1d;
+  c.ism = idx - ism_offset;
+  // int(ismIndex / ismIndices1d): exact for |index| < 2^24, truncated toward
+  // zero (a negative index, possible with scaleISMs, gives tiles left of or
+  // above the atlas)
+  c.tile_y = c.ism / n1d;
+  c.tile_x = c.ism - c.tile_y * n1d;
   c.pad = 0;
   return c;
 }
@@ -151,12 +154,24 @@ __device__ __forceinline__ bool project(const Candidate& c, f3 pos, uint32_t rad
   return true;
 }
 
+// The VPL range of a point-major splat (ism.comp:31-40,56-66,124): with
+// points_only, bin b's candidates are ((b + i) mod n_vpl) mod count + start;
+// VPL v splats into ISM v - ism_offset (the range start with scaleISMs,
+// else 0). The identity range is {0, n_vpl, 0, 0}.
+struct SplatRange {
+  int start, count, ism_offset, points_only;
+};
+
 // Stages the 16 candidates of bin b in LDS (lanes 0..15); the caller
 // synchronises.
 __device__ __forceinline__ void stage_candidates(Candidate* cand, const float* __restrict__ vpl, int n_vpl,
-                                                 int n1d, int b) {
+                                                 int n1d, int b, const SplatRange& r) {
   const int lane = threadIdx.x;
-  if (lane < kMaxTest) cand[lane] = load_candidate(vpl, (b + lane) % n_vpl, n1d);  // index %= totalVplCount
+  if (lane < kMaxTest) {
+    int idx = (b + lane) % n_vpl;  // index %= totalVplCount
+    if (r.points_only) idx = idx % r.count + r.start;
+    cand[lane] = load_candidate(vpl, idx, n1d, r.ism_offset);
+  }
 }
 
 // Point-major splat of bin b of one frame: every lane splats one point per
@@ -167,7 +182,7 @@ __device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restr
                                           const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
                                           const float* __restrict__ vpl, int n_vpl, int n1d,
                                           int atlas_side, float z_far, uint32_t* __restrict__ atlas,
-                                          int b) {
+                                          int b, const SplatRange& range) {
   const int lane = threadIdx.x;
   // clamp to the point count so malformed offsets cannot index past the buffer
   const uint32_t end = min(gload(bin_offsets + b + 1), n_points);
@@ -177,7 +192,7 @@ __device__ __forceinline__ void splat_bin(Candidate* cand, const float4* __restr
   const uint32_t last = end > 0 ? end - 1 : 0;
   uint32_t p = begin + lane;
   float4 rec = gload4(points + min(p, last));
-  stage_candidates(cand, vpl, n_vpl, n1d, b);
+  stage_candidates(cand, vpl, n_vpl, n1d, b, range);
   __syncthreads();
 
   const float inv_n1d = 1.0f / (float)n1d;  // exact: n1d is a power of two
@@ -204,10 +219,10 @@ template <int kThreads>
 __global__ __launch_bounds__(kThreads) void ism_splat_kernel(
     const float4* __restrict__ points, const uint32_t* __restrict__ bin_offsets, uint32_t n_points,
     const float* __restrict__ vpl, int n_vpl, int n1d, int atlas_side, float z_far,
-    uint32_t* __restrict__ atlas) {
+    uint32_t* __restrict__ atlas, SplatRange range) {
   __shared__ Candidate cand[kMaxTest];
   splat_bin<kThreads>(cand, points, bin_offsets, n_points, vpl, n_vpl, n1d, atlas_side, z_far, atlas,
-                      blockIdx.x);
+                      blockIdx.x, range);
 }
 
 // ------------------------------------------------------------ ISM-major form
@@ -226,7 +241,7 @@ __device__ __forceinline__ void mask_bin(Candidate* cand, const float4* __restri
   uint32_t p = begin + lane;
   float4 rec = gload4(points + min(p, last));
   if (b == 0 && lane < 2) gstore(spill_state + lane, 0u);
-  stage_candidates(cand, vpl, n_vpl, n1d, b);
+  stage_candidates(cand, vpl, n_vpl, n1d, b, SplatRange{0, n_vpl, 0, 0});
   __syncthreads();
   while (p < end) {
     const uint32_t pn = p + 256;
@@ -421,7 +436,7 @@ __device__ __forceinline__ void spill_bin(Candidate* cand, const TileJob& j, int
   const uint32_t overflow = gload(j.spill_state + 1);
   if (overflow) {
     splat_bin<256>(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, j.atlas_side, j.z_far,
-                   j.atlas, b);
+                   j.atlas, b, SplatRange{0, j.n_vpl, 0, 0});
     return;
   }
   const uint32_t n = min(gload(j.spill_state), kSpillCap);
@@ -565,18 +580,20 @@ hipError_t launch_atlas_clear(hipStream_t s, uint32_t* atlas, size_t texels) {
 
 hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* bin_offsets,
                             int n_bins, int n_points, const float* vpl_records, int n_vpl, int n1d,
-                            int atlas_side, float z_far, uint32_t* atlas, int block) {
+                            int atlas_side, float z_far, uint32_t* atlas, int block, int range_start,
+                            int range_count, int ism_offset, int points_only) {
   if (n_bins <= 0) return hipSuccess;
   const float4* pts = reinterpret_cast<const float4*>(points);
+  const SplatRange r{range_start, range_count, ism_offset, points_only};
   if (block == 64)
     hipLaunchKernelGGL(ism_splat_kernel<64>, dim3(n_bins), dim3(64), 0, s, pts, bin_offsets,
-                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas);
+                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas, r);
   else if (block == 128)
     hipLaunchKernelGGL(ism_splat_kernel<128>, dim3(n_bins), dim3(128), 0, s, pts, bin_offsets,
-                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas);
+                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas, r);
   else
     hipLaunchKernelGGL(ism_splat_kernel<256>, dim3(n_bins), dim3(256), 0, s, pts, bin_offsets,
-                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas);
+                       (uint32_t)n_points, vpl_records, n_vpl, n1d, atlas_side, z_far, atlas, r);
   return hipGetLastError();
 }
 

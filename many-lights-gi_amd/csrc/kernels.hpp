@@ -28,9 +28,14 @@ hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* 
 
 // ism_splat.hip
 hipError_t launch_atlas_clear(hipStream_t s, uint32_t* atlas, size_t texels);
+// Point-major splat. The VPL range (scaleISMs / pointsOnlyIntoScaledISMs):
+// with points_only, bin b takes its candidates from the range_count VPLs at
+// range_start; VPL v splats into ISM v - ism_offset. The plain splat is
+// (0, n_vpl, 0, 0).
 hipError_t launch_ism_splat(hipStream_t s, const float* points, const uint32_t* bin_offsets,
                             int n_bins, int n_points, const float* vpl_records, int n_vpl, int n1d,
-                            int atlas_side, float z_far, uint32_t* atlas, int block);
+                            int atlas_side, float z_far, uint32_t* atlas, int block, int range_start,
+                            int range_count, int ism_offset, int points_only);
 // ISM-major splat: per-point candidate masks, one LDS tile per ISM, then the
 // spill list of splats that land outside their ISM's tile. Needs a work area
 // of splat_work_bytes(n_points) bytes (no initial contents) and a 16-byte
@@ -83,6 +88,7 @@ struct GatherArgs {
   float out_scale;   // giIntensityFactor / vplCount * I^2
   int shadowing;
   int cull;          // 1: per-wave VPL culling; 0: every VPL of the class is evaluated
+  int ism_offset;    // VPL v samples ISM v - ism_offset (scaleISMs: the VPL range start)
   float* out;
 };
 hipError_t launch_final_gather(hipStream_t s, const GatherArgs& a, int precision, int tile_w,

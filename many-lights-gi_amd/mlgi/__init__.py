@@ -28,7 +28,7 @@ __all__ = [
     "MlgiError", "Params", "PRECISION_FP32", "PRECISION_MIXED", "VPL_RECORD_FLOATS",
     "Context", "DeviceArray", "Scene", "Camera", "VPLProcessor", "ImperfectShadowmap",
     "GIStage", "default_params", "shuffle_indices", "ism_indices_1d", "device_count",
-    "vpl_fields", "fill_flux_halves",
+    "vpl_fields", "fill_flux_halves", "ism_count",
 ]
 
 lib = N.lib
@@ -426,15 +426,30 @@ class VPLProcessor:
         return self.records.numpy()
 
 
+def ism_count(params: Params, vpl_count: int) -> int:
+    """ISMs in the atlas (ImperfectShadowmap.cpp:188): one per VPL, or with
+    scaleISMs one per VPL of [vplStartIndex, vplEndIndex)."""
+    if params.scale_isms:
+        end = params.vpl_end_index if params.vpl_end_index > 0 else vpl_count
+        return end - params.vpl_start_index
+    return vpl_count
+
+
 class ImperfectShadowmap:
     """Owns the atlas ("ISM softrender") and the pull-push result."""
 
     def __init__(self, ctx: Context, atlas_side: int, vpl_count: int):
         self.ctx = ctx
         self.side = atlas_side
+        self.vpl_count = vpl_count
         self.ism_side = atlas_side // ism_indices_1d(vpl_count)
         self.softrender = ctx.empty((atlas_side, atlas_side), np.uint32)
         self.pushpull_result = ctx.empty((atlas_side, atlas_side), np.uint16)
+
+    def layout(self, params: Params) -> None:
+        """ISM side for the ISM count params give (scaleISMs sizes the ISMs
+        for the VPL range)."""
+        self.ism_side = self.side // ism_indices_1d(ism_count(params, self.vpl_count))
 
     def atlas_desc(self) -> N.Atlas:
         a = N.Atlas()
@@ -447,6 +462,7 @@ class ImperfectShadowmap:
         return d
 
     def render(self, points: N.Points, vpl_processor: VPLProcessor, params: Params) -> None:
+        self.layout(params)
         self.ctx.check(lib.mlgi_ism_splat(self.ctx.h, C.byref(points), C.byref(vpl_processor.desc()),
                                           C.byref(params), C.byref(self.atlas_desc())), "mlgi_ism_splat")
 
@@ -478,6 +494,7 @@ class ImperfectShadowmap:
         p.use_push_pull = int(use_push_pull)
         p.z_far = z_far
         p.atlas_side = self.side
+        self.layout(p)
         self.ctx.check(lib.mlgi_ism_process(self.ctx.h, C.byref(points), C.byref(vpl_processor.desc()),
                                             C.byref(p), C.byref(self.atlas_desc()),
                                             C.byref(self.depth_desc())), "mlgi_ism_process")
@@ -505,6 +522,7 @@ class GIStage:
                                              C.byref(self.image_desc())), "mlgi_final_gather")
 
     def process(self, rsm: N.Rsm, points: N.Points, gbuffer: N.GBuffer) -> None:
+        self.ism.layout(self.params)
         v, a, d, im = self.vpl_processor.desc(), self.ism.atlas_desc(), self.ism.depth_desc(), self.image_desc()
         self.ctx.check(lib.mlgi_frame(self.ctx.h, C.byref(rsm), C.byref(points), C.byref(gbuffer),
                                       C.byref(self.params), C.byref(v), C.byref(a), C.byref(d),

@@ -60,11 +60,14 @@ _sigs = {
     "oracle_pullpush_f32": (None, [vp, i32, i32, i32, f32, vp]),
     "oracle_pullpush_f64": (None, [vp, i32, i32, i32, f32, vp, vp]),
     "oracle_ism_splat_mt_f32": (None, [vp, vp, i32, vp, vp, i32, i32, f32, vp, i32]),
+    "oracle_ism_splat_range_f32": (None, [vp, vp, i32, vp, vp, i32, i32, f32, vp, i32, i32, i32, i32]),
     "oracle_pullpush_mt_f32": (None, [vp, i32, i32, i32, f32, vp, i32]),
     "oracle_final_gather_f64": (None, [vp, vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, vp, i32, f32,
                                        f32, f32, i32, i32, i32, vp, vp]),
     "oracle_final_gather_f32": (None, [vp, vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, vp, i32, f32,
                                        f32, f32, i32, i32, i32, vp, vp]),
+    "oracle_final_gather_range_f64": (None, [vp, vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, vp, i32, f32,
+                                             f32, f32, i32, i32, i32, vp, vp, i32, i32]),
     "oracle_final_gather_rows_f32": (None, [vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, i32, i32, i32,
                                             vp, i32, f32, f32, f32, i32, i32, i32, vp, vp]),
     "oracle_gi_blur_f32": (None, [vp, vp, vp, i32, i32, vp, vp]),
@@ -166,12 +169,21 @@ def vpl_extract(diffuse, normal, depth, inv_bvp, light_intensity, shuffle, n, dt
     return pos, nrm, pk, col
 
 
-def ism_splat(points, bin_offsets, vpl_pos, vpl_nrm, atlas_side, z_far, dtype=np.float32, threads=1):
-    """threads > 1 (FP32 only): bins on that many host threads, same bits."""
+def ism_splat(points, bin_offsets, vpl_pos, vpl_nrm, atlas_side, z_far, dtype=np.float32, threads=1,
+              vpl_range=None, scale_isms=False, points_only=False):
+    """threads > 1 (FP32 only): bins on that many host threads, same bits.
+    vpl_range = (start, end) with scale_isms / points_only (FP32 only): the
+    reference's scaleISMs / pointsOnlyIntoScaledISMs over those VPLs."""
     pts = _c(points, np.float32)
     off = _c(bin_offsets, np.uint32)
     vpos, vnrm = _c(vpl_pos, np.float32), _c(vpl_nrm, np.float32)
     atlas = np.empty((atlas_side, atlas_side), np.uint32)
+    if vpl_range is not None:
+        assert dtype == np.float32
+        start, end = vpl_range
+        _lib.oracle_ism_splat_range_f32(_p(pts), _p(off), len(off) - 1, _p(vpos), _p(vnrm), len(vpos), atlas_side,
+                                        z_far, _p(atlas), start, end - start, int(scale_isms), int(points_only))
+        return atlas
     if threads > 1 and dtype == np.float32:
         _lib.oracle_ism_splat_mt_f32(_p(pts), _p(off), len(off) - 1, _p(vpos), _p(vnrm), len(vpos), atlas_side,
                                      z_far, _p(atlas), threads)
@@ -200,8 +212,10 @@ def pullpush(atlas, ism_side, levels, z_far, dtype=np.float32, threads=1):
 
 def final_gather(depth, normal, vp_inv, vpl_pos, vpl_nrm, vpl_col, ism, atlas_side, z_far,
                  gi_intensity, clamp_value, interleave, shadowing=True, vpl_start=0, vpl_end=None,
-                 threads=None, dtype=np.float64):
-    """Brute-force interleaved final gathering. Returns (rgb[h,w,3], lit_pairs)."""
+                 threads=None, dtype=np.float64, scale_isms=False):
+    """Brute-force interleaved final gathering. Returns (rgb[h,w,3], lit_pairs).
+    scale_isms (FP64 only): SCALE_ISMS, ISM index j - vpl_start in a grid of
+    vpl_end - vpl_start ISMs."""
     dp, nm = _c(depth, np.float32), _c(normal, np.float32)
     h, w = dp.shape
     vpos, vnrm, vcol = _c(vpl_pos, np.float32), _c(vpl_nrm, np.float32), _c(vpl_col, np.float32)
@@ -210,9 +224,17 @@ def final_gather(depth, normal, vp_inv, vpl_pos, vpl_nrm, vpl_col, ism, atlas_si
     out = np.empty((h, w, 3), dtype)
     lit = C.c_uint64(0)
     th = threads or min(8, os.cpu_count() or 1)
+    ve = n if vpl_end is None else vpl_end
+    if scale_isms:
+        assert dtype == np.float64
+        _lib.oracle_final_gather_range_f64(_p(dp), _p(nm), w, h, _p(_c(vp_inv, np.float32)), _p(vpos), _p(vnrm),
+                                           _p(vcol), n, vpl_start, ve, _p(ism_a), atlas_side, z_far, gi_intensity,
+                                           clamp_value, interleave, int(shadowing), th, _p(out), C.addressof(lit),
+                                           vpl_start, ve - vpl_start)
+        return out, int(lit.value)
     fn = _lib.oracle_final_gather_f64 if dtype == np.float64 else _lib.oracle_final_gather_f32
     fn(_p(dp), _p(nm), w, h, _p(_c(vp_inv, np.float32)), _p(vpos), _p(vnrm), _p(vcol), n, vpl_start,
-       n if vpl_end is None else vpl_end, _p(ism_a), atlas_side, z_far, gi_intensity, clamp_value,
+       ve, _p(ism_a), atlas_side, z_far, gi_intensity, clamp_value,
        interleave, int(shadowing), th, _p(out), C.addressof(lit))
     return out, int(lit.value)
 

@@ -281,23 +281,32 @@ void vpl_extract(const float* diffuse, const float* normal, const float* depth, 
 }
 
 // ------------------------------------------------------------- ISM splat
-// ism.comp:54-139 (usePushPull branch, scaleISMs = pointsOnlyIntoScaledISMs = false)
+// ism.comp:54-139 (usePushPull branch). The VPL range [start, start + count)
+// and the two flags follow ism.comp:31-40,56-66,124: with scale_isms the
+// atlas holds `count` ISMs and VPL v splats into ISM v - start (indices
+// outside the atlas's grid write nothing or, through the truncating
+// ivec2(), the first row / column, as GL's invalid image atomics and int()
+// do); with points_only the `count` bins take their 16 candidates from the
+// range only, ((b + i) mod n_vpl) mod count + start.
 // The atlas is a per-texel minimum, so the bins may run on `threads`
 // threads in any order (atomic min) with the same result.
 template <class T>
 void ism_splat(const float* points, const uint32_t* bin_off, int n_bins, const float* vpos,
-               const float* vnrm, int n_vpl, int atlas_side, float z_far, uint32_t* atlas, int threads = 1) {
+               const float* vnrm, int n_vpl, int atlas_side, float z_far, uint32_t* atlas, int threads = 1,
+               int range_start = 0, int range_count = -1, bool scale_isms = false, bool points_only = false) {
   const int max_test = 16, max_collect = 4;
+  if (range_count < 0) range_count = n_vpl;
   std::fill(atlas, atlas + (size_t)atlas_side * atlas_side, 0xFFFFFFFFu);
-  const int n1d = ism_indices_1d(n_vpl);
+  const int n1d = ism_indices_1d(scale_isms ? range_count : n_vpl);
   const T zf = T(z_far);
   const bool atomic = threads > 1;
   parallel_rows(n_bins, threads, [&](int b) {
     int ids[max_test];
     V3<T> cp[max_test], cn[max_test];
     for (int i = 0; i < max_test; ++i) {
-      int idx = (b + i) % n_vpl;
-      ids[i] = idx;
+      int idx = (b + i) % n_vpl;  // index %= totalVplCount
+      if (points_only) idx = idx % range_count + range_start;
+      ids[i] = scale_isms ? idx - range_start : idx;  // the ISM index
       cp[i] = v3<T>(T(vpos[idx * 3]), T(vpos[idx * 3 + 1]), T(vpos[idx * 3 + 2]));
       cn[i] = v3<T>(T(vnrm[idx * 3]), T(vnrm[idx * 3 + 1]), T(vnrm[idx * 3 + 2]));
     }
@@ -561,15 +570,19 @@ template <class T> T smoothstep(T e0, T e1, T x) {
   return t * t * (T(3) - T(2) * t);
 }
 
+// ism_offset / ism_count: SCALE_ISMS's ismIndexOffset and ismCount
+// (final_gathering.comp:43-45): VPL j samples ISM j - ism_offset of a grid
+// laid out for ism_count ISMs (ism_count < 0: n_vpl).
 template <class T>
 void final_gather_rows(const float* depth, const float* normal, int w, int h, int y0, int y1,
                        const float* m, const float* vpos, const float* vnrm, const float* vcol,
                        int n_vpl, int vpl_start, int vpl_end, const uint16_t* ism, int atlas_side,
                        float z_far, float gi_intensity, float clamp_value, int interleave,
-                       int shadowing, int threads, T* out, uint64_t* lit_pairs) {
+                       int shadowing, int threads, T* out, uint64_t* lit_pairs, int ism_offset = 0,
+                       int ism_count = -1) {
   const int classes = interleave * interleave;
   const int per_class = n_vpl / classes;
-  const int n1d = ism_indices_1d(n_vpl);
+  const int n1d = ism_indices_1d(ism_count < 0 ? n_vpl : ism_count);
   const T zf = T(z_far);
   const T bias = T(0.02f);
   const T vpl_count = T((float)(vpl_end - vpl_start));
@@ -612,7 +625,7 @@ void final_gather_rows(const float* depth, const float* normal, int w, int h, in
           T att = T(1) / (d2 * d2);
           T g = std::min(angle * att, T(clamp_value));
           if (shadowing) {
-            V3<T> v = paraboloid_project<T>(diff, dist, vn, zf, j, n1d, false);
+            V3<T> v = paraboloid_project<T>(diff, dist, vn, zf, j - ism_offset, n1d, false);
             T fx = std::floor(v.x * T(atlas_side)), fy = std::floor(v.y * T(atlas_side));
             long long tx = ((long long)fx % atlas_side + atlas_side) % atlas_side;
             long long ty = ((long long)fy % atlas_side + atlas_side) % atlas_side;
@@ -919,6 +932,20 @@ void oracle_final_gather_f32(const float* depth, const float* normal, int w, int
                              float* out, uint64_t* lit) {
   final_gather_rows<float>(depth, normal, w, h, 0, h, vp_inv, vpos, vnrm, vcol, n_vpl, vs, ve, ism,
                            side, zf, gi, cl, il, sh, th, out, lit);
+}
+// VPL-range forms (scaleISMs / pointsOnlyIntoScaledISMs).
+void oracle_ism_splat_range_f32(const float* p, const uint32_t* bo, int nb, const float* vp, const float* vn,
+                                int nv, int side, float zf, uint32_t* atlas, int start, int count, int scale,
+                                int points_only) {
+  ism_splat<float>(p, bo, nb, vp, vn, nv, side, zf, atlas, 1, start, count, scale != 0, points_only != 0);
+}
+void oracle_final_gather_range_f64(const float* depth, const float* normal, int w, int h,
+                                   const float* vp_inv, const float* vpos, const float* vnrm,
+                                   const float* vcol, int n_vpl, int vs, int ve, const uint16_t* ism,
+                                   int side, float zf, float gi, float cl, int il, int sh, int th,
+                                   double* out, uint64_t* lit, int ism_offset, int ism_count) {
+  final_gather_rows<double>(depth, normal, w, h, 0, h, vp_inv, vpos, vnrm, vcol, n_vpl, vs, ve, ism,
+                            side, zf, gi, cl, il, sh, th, out, lit, ism_offset, ism_count);
 }
 void oracle_gi_blur_f32(const float* gi, const float* normal, const float* depth, int w, int h,
                         const float* proj16, float* out) {

@@ -121,6 +121,18 @@ void oracle_final_gather_f32(const float* depth, const float* normal, int w, int
 
 /* Final gathering restricted to a pixel subset (rows [y0, y1)), for bounded
  * CPU baselines. Same semantics as above; out covers the selected rows. */
+/* VPL-range forms: scaleISMs / pointsOnlyIntoScaledISMs over the VPLs
+ * [start, start + count) (ism.comp:31-40,56-66,124; final_gathering.comp:43-45). */
+void oracle_ism_splat_range_f32(const float* points, const uint32_t* bin_offsets, int n_bins,
+                                const float* vpl_pos, const float* vpl_nrm, int n_vpl, int atlas_side,
+                                float z_far, uint32_t* atlas, int start, int count, int scale_isms,
+                                int points_only);
+void oracle_final_gather_range_f64(const float* depth, const float* normal, int w, int h,
+                                   const float* vp_inv, const float* vpl_pos, const float* vpl_nrm,
+                                   const float* vpl_col, int n_vpl, int vpl_start, int vpl_end,
+                                   const uint16_t* ism, int atlas_side, float z_far, float gi_intensity,
+                                   float clamp_value, int interleave, int shadowing, int threads,
+                                   double* out, uint64_t* lit_pairs, int ism_offset, int ism_count);
 void oracle_final_gather_rows_f32(const float* depth, const float* normal, int w, int h,
                                   int y0, int y1, const float* vp_inv, const float* vpl_pos,
                                   const float* vpl_nrm, const float* vpl_color, int n_vpl,

@@ -151,10 +151,14 @@ def test_invalid_arguments_rejected(ctx):
     with pytest.raises(mlgi.MlgiError) as e:
         st.compute_final_gathering(gb)
     assert e.value.code == N.MLGI_ERR_INVALID_ARGUMENT
-    st.params = wl.params(scale_isms=1)
+    st.params = wl.params(use_push_pull=0)
     with pytest.raises(mlgi.MlgiError) as e:
         st.ism.render(pts, st.vpl_processor, st.params)
     assert e.value.code == N.MLGI_ERR_UNSUPPORTED
+    st.params = wl.params(scale_isms=1, vpl_start_index=200, vpl_end_index=300)  # past the 256 VPLs
+    with pytest.raises(mlgi.MlgiError) as e:
+        st.ism.render(pts, st.vpl_processor, st.params)
+    assert e.value.code == N.MLGI_ERR_INVALID_ARGUMENT
     st.params = wl.params()
     wrong_bins = mlgi.points_desc(ctx.upload(inp.points), ctx.upload(np.zeros(11, np.uint32)))
     with pytest.raises(mlgi.MlgiError) as e:
