@@ -113,12 +113,16 @@ typedef struct mlgi_rsm {
 
 /* VPL records, MLGI_VPL_RECORD_FLOATS floats (128 bytes) per VPL:
  *  [0..2]   world position          [3]  plane offset n . position
- *  [4..6]   unpacked normal         [7]  packed normal (pack3SNToFloat bits, u32)
+ *  [4..6]   unpacked normal         [7]  zero-flux flags (u32): bit 0 when the
+ *                                        flux floats are all +-0, bit 1 when
+ *                                        its halves (11, 15) are all zero
  *  [8..10]  lookAtRH s row          [11] flux r | g << 16 as binary16 bits
  *  [12..14] lookAtRH u row          [15] flux b as binary16 bits (high half 0)
- *  [16..18] flux (diffuse * lightIntensity); all other slots are 0.
- * Slots 11 and 15 hold [16..18] rounded to nearest even; the mixed-form
- * gather reads the flux from them (records built by hand must fill them).
+ *  [16..18] flux (diffuse * lightIntensity)
+ *                                   [19] packed normal (pack3SNToFloat bits, u32)
+ * All other slots are 0. Slots 11 and 15 hold [16..18] rounded to nearest
+ * even; the mixed-form gather reads the flux from them, and the gather's
+ * culling reads slot 7 (records built by hand must fill 7, 11 and 15).
  * (VPLProcessor vplBuffer + packedVplBuffer, VPLProcessor.cpp:29-41). */
 #define MLGI_VPL_RECORD_FLOATS 32
 typedef struct mlgi_vpls {

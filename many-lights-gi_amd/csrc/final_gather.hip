@@ -49,6 +49,7 @@ struct VplLds {
   float sy, sz, ux, uy;
   float uz, cr, cg, cb;
   uint32_t hrg, hb;  // the flux as halves: r | g << 16, b (record slots 11, 15)
+  uint32_t zero;     // zero-flux flags (slot 7): bit 0 floats, bit 1 halves
 };
 
 // The bits of a float taken by value: __builtin_bit_cast applied directly to
@@ -59,7 +60,7 @@ __device__ __forceinline__ VplLds load_vpl(const float* rec) {
   const float4* r = reinterpret_cast<const float4*>(rec);
   const float4 a = gload4(r), b = gload4(r + 1), c = gload4(r + 2), d = gload4(r + 3), e = gload4(r + 4);
   return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z,
-                float_bits(c.w), float_bits(d.w)};
+                float_bits(c.w), float_bits(d.w), float_bits(b.w)};
 }
 
 // Wave64 min / max, result wave-uniform. Floats are mapped to unsigned keys
@@ -178,17 +179,12 @@ __device__ __forceinline__ bool wave_bounds(const Pixel& p, WaveBounds& wb) {
 // the wave gets angleFactor == 0 from it, or when its flux is zero (an RSM
 // texel outside the spot cone; MIXED: zero after rounding to halves, the
 // values the sums multiply): every contribution is then flux * weight = +0
-// with a finite weight, which leaves the sums' bits unchanged.
+// with a finite weight, which leaves the sums' bits unchanged. Reads only
+// the record's first 32 bytes (the zero-flux flags sit beside the normal):
+// every load per lane is paid by every wave for every chunk.
 template <bool kMixed>
 __device__ __forceinline__ bool cull_keep(const VplLds& v, const WaveBounds& w) {
-  // (the halves are formed from the flux floats here rather than read from
-  // record slots 11 and 15: the chunk's records are already being read as
-  // 16-byte vectors, and two more loads per lane made the 4K gather 50 us
-  // slower; the values are the same)
-  const bool dark =
-      kMixed ? (__builtin_bit_cast(uint16_t, (_Float16)v.cr) | __builtin_bit_cast(uint16_t, (_Float16)v.cg) |
-                __builtin_bit_cast(uint16_t, (_Float16)v.cb)) == 0
-             : (v.cr == 0.0f && v.cg == 0.0f && v.cb == 0.0f);
+  const bool dark = (v.zero >> (kMixed ? 1 : 0)) & 1u;
   // (1) max over the box of n . (p - p_vpl) = n . (c - p_vpl) + |n| . h
   const float dx = w.cx - v.px, dy = w.cy - v.py, dz = w.cz - v.pz;
   const float ax = fabsf(v.nx), ay = fabsf(v.ny), az = fabsf(v.nz);
@@ -391,7 +387,7 @@ __device__ __forceinline__ VplLds uniform_vpl(const float* vpl, int vi) {
   ConstF4* r = (ConstF4*)(vpl + (size_t)u * kVplFloats);
   const cf4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
   return VplLds{a.x, a.y, a.z, a.w, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, e.x, e.y, e.z,
-                float_bits(c.w), float_bits(d.w)};
+                float_bits(c.w), float_bits(d.w), float_bits(b.w)};
 }
 
 // The hemisphere-test fields of VPL vi (wave-uniform), as uniform_vpl.

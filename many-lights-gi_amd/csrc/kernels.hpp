@@ -15,10 +15,10 @@ struct Mat4 {
 // VPL record: 32 floats (128 bytes) per VPL, see include/mlgi.h.
 constexpr int kVplFloats = 32;
 constexpr int kVplPos = 0;       // xyz, [3] plane offset n . position
-constexpr int kVplNormal = 4;    // xyz, [7] packed normal bits
+constexpr int kVplNormal = 4;    // xyz, [7] zero-flux flags (bit 0 floats, bit 1 halves)
 constexpr int kVplS = 8;         // lookAtRH s row
 constexpr int kVplU = 12;        // lookAtRH u row
-constexpr int kVplFlux = 16;     // rgb
+constexpr int kVplFlux = 16;     // rgb, [19] packed normal bits
 
 // vpl_extract.hip
 hipError_t launch_vpl_extract(hipStream_t s, const float* diffuse, const float* normal,

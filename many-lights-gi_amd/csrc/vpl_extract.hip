@@ -61,18 +61,23 @@ __device__ __forceinline__ void vpl_extract_one(const float* __restrict__ diffus
   dev::look_at_rh(un, s, u);
   const float px = r[0] / r[3], py = r[1] / r[3], pz = r[2] / r[3];
   const float plane = un.x * px + un.y * py + un.z * pz;  // n . position
-  float4* out = reinterpret_cast<float4*>(records + (size_t)dst * kVplFloats);
-  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-  out[0] = make_float4(px, py, pz, plane);
-  out[1] = make_float4(un.x, un.y, un.z, __uint_as_float(pk));
   const float fr = dr * light_intensity, fg = dg * light_intensity, fb = db * light_intensity;
   // the flux as halves (round to nearest even) for the mixed-form gather
   const uint32_t hrg = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)fr) |
                        ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)fg) << 16);
   const uint32_t hb = __builtin_bit_cast(uint16_t, (_Float16)fb);
+  // zero-flux flags for the gather's culling (bit 0: the floats are all
+  // +-0; bit 1: the halves are all zero), beside the normal it reads anyway
+  const uint32_t zero_flags =
+      (((__float_as_uint(fr) | __float_as_uint(fg) | __float_as_uint(fb)) & 0x7FFFFFFFu) == 0u ? 1u : 0u) |
+      ((hrg | hb) == 0u ? 2u : 0u);
+  float4* out = reinterpret_cast<float4*>(records + (size_t)dst * kVplFloats);
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  out[0] = make_float4(px, py, pz, plane);
+  out[1] = make_float4(un.x, un.y, un.z, __uint_as_float(zero_flags));
   out[2] = make_float4(s.x, s.y, s.z, __uint_as_float(hrg));
   out[3] = make_float4(u.x, u.y, u.z, __uint_as_float(hb));
-  out[4] = make_float4(fr, fg, fb, 0.f);
+  out[4] = make_float4(fr, fg, fb, __uint_as_float(pk));
   out[5] = zero;
   out[6] = zero;
   out[7] = zero;

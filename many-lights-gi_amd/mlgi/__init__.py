@@ -40,7 +40,8 @@ def vpl_fields(rec: np.ndarray) -> dict:
         "position": rec[:, 0:3],
         "plane": rec[:, 3],
         "normal": rec[:, 4:7],
-        "packed_normal": rec[:, 7].view(np.uint32),
+        "packed_normal": rec[:, 19].view(np.uint32),
+        "zero_flux_flags": rec[:, 7].view(np.uint32),
         "s": rec[:, 8:11],
         "u": rec[:, 12:15],
         "flux": rec[:, 16:19],
@@ -49,12 +50,17 @@ def vpl_fields(rec: np.ndarray) -> dict:
 
 
 def fill_flux_halves(rec: np.ndarray) -> np.ndarray:
-    """Writes record slots 11 and 15 (the flux as binary16 bits, round to
-    nearest even: r | g << 16 and b) from slots 16..18, as VPL extraction
-    does; records built by hand need them for the mixed-form gather."""
+    """Writes the record slots VPL extraction derives from the flux (slots
+    16..18): 11 and 15, the flux as binary16 bits (round to nearest even:
+    r | g << 16 and b), and 7, the zero-flux flags (bit 0: the floats are all
+    +-0; bit 1: the halves are all zero). Records built by hand need them for
+    the gather."""
     h = rec[:, 16:19].astype(np.float16).view(np.uint16).astype(np.uint32)
-    rec[:, 11] = (h[:, 0] | (h[:, 1] << 16)).view(np.float32)
-    rec[:, 15] = h[:, 2].view(np.float32)
+    rg, b = h[:, 0] | (h[:, 1] << 16), h[:, 2]
+    rec[:, 11] = rg.view(np.float32)
+    rec[:, 15] = b.view(np.float32)
+    fz = ((np.ascontiguousarray(rec[:, 16:19]).view(np.uint32) & 0x7FFFFFFF) == 0).all(axis=1)
+    rec[:, 7] = (fz.astype(np.uint32) | (((rg | b) == 0).astype(np.uint32) << 1)).view(np.float32)
     return rec
 
 

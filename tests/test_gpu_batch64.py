@@ -88,7 +88,7 @@ def test_batch64_graph_oracle_and_alone(ctx, b64):
         sh = mlgi.shuffle_indices(sp.n_vpl)
         pos, vn, pk, col = oracle.vpl_extract(dif, nrm, dep, inv, p.light_intensity, sh, sp.n_vpl, np.float32)
         assert np.array_equal(rec[:, 0:3], pos) and np.array_equal(rec[:, 16:19], col), i
-        assert np.array_equal(rec[:, 7].view(np.uint32), pk), i
+        assert np.array_equal(rec[:, 19].view(np.uint32), pk), i
         assert np.array_equal(atlas, oracle.ism_splat(h["points"], h["bin_offsets"], pos, vn, p.atlas_side, p.z_far,
                                                       threads=16)), i
         assert np.array_equal(r16, oracle.pullpush(atlas, sp.ism_side, sp.levels, p.z_far, threads=16)), i

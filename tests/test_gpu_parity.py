@@ -78,13 +78,17 @@ def test_vpl_extract_bit_exact(frame):
                                            frame.params.light_intensity, sh, wl.n_vpl, np.float32)
     r = frame.rec
     assert np.array_equal(r[:, 0:3], pos)
-    assert np.array_equal(r[:, 7].view(np.uint32), pk)
+    assert np.array_equal(r[:, 19].view(np.uint32), pk)
     assert np.array_equal(r[:, 4:7], nrm)
     assert np.array_equal(r[:, 16:19], col)
     # the flux as halves (slots 11 and 15), round to nearest even
     h = col.astype(np.float16).view(np.uint16).astype(np.uint32)
     assert np.array_equal(r[:, 11].view(np.uint32), h[:, 0] | (h[:, 1] << 16))
     assert np.array_equal(r[:, 15].view(np.uint32), h[:, 2])
+    # zero-flux flags: bit 0 floats all zero, bit 1 halves all zero
+    flags = (col == 0).all(axis=1).astype(np.uint32) | ((h == 0).all(axis=1).astype(np.uint32) << 1)
+    assert np.array_equal(r[:, 7].view(np.uint32), flags)
+    assert (flags != 0).any() and (flags == 0).any()
     f = mlgi.vpl_fields(r)
     n, x = f["normal"], f["position"]
     plane = n[:, 0] * x[:, 0] + n[:, 1] * x[:, 1] + n[:, 2] * x[:, 2]  # float32, unfused
