@@ -430,17 +430,26 @@ __device__ __forceinline__ void splat_tile(const TileJob& j, uint32_t* tile, Que
   }
 }
 
-// Spill pass, item b of the frame's n_bins: every spill entry j == b (mod
-// n_bins) in the normal case; bin b's point-major splat after an overflow.
-__device__ __forceinline__ void spill_bin(Candidate* cand, const TileJob& j, int n_bins, int b) {
+// Workgroups of the spill pass.
+constexpr unsigned kSpillGrid = 256;
+
+// Spill pass of one frame by the whole grid: its spill entries in a
+// grid-stride loop or, after an overflow, its point-major splat with the
+// bins in a grid-stride loop. The overflow flag is read once per workgroup,
+// so every thread of a workgroup takes the same branch (splat_bin
+// synchronises the workgroup).
+__device__ __forceinline__ void spill_frame(Candidate* cand, const TileJob& j, int n_bins) {
   const uint32_t overflow = gload(j.spill_state + 1);
   if (overflow) {
-    splat_bin<256>(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, j.atlas_side, j.z_far,
-                   j.atlas, b, SplatRange{0, j.n_vpl, 0, 0});
+    for (int b = blockIdx.x; b < n_bins; b += gridDim.x) {
+      splat_bin<256>(cand, j.points, j.bin_offsets, j.n_points, j.vpl, j.n_vpl, j.n1d, j.atlas_side, j.z_far,
+                     j.atlas, b, SplatRange{0, j.n_vpl, 0, 0});
+      __syncthreads();  // cand is restaged for the next bin
+    }
     return;
   }
   const uint32_t n = min(gload(j.spill_state), kSpillCap);
-  for (uint32_t k = (uint32_t)b * 256u + threadIdx.x; k < n; k += (uint32_t)n_bins * 256u) {
+  for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
     const uint64_t e = gload(reinterpret_cast<const uint64_t*>(j.spill + k));
     gatomic_min(j.atlas + (uint32_t)e, (uint32_t)(e >> 32));
   }
@@ -478,10 +487,7 @@ __global__ __launch_bounds__(256) void ism_tiles_kernel(TileJob j, int n_tiles) 
 
 __global__ __launch_bounds__(256) void ism_spill_kernel(TileJob j, int n_bins) {
   __shared__ Candidate cand[kMaxTest];
-  for (int b = blockIdx.x; b < n_bins; b += gridDim.x) {
-    spill_bin(cand, j, n_bins, b);
-    __syncthreads();
-  }
+  spill_frame(cand, j, n_bins);
 }
 
 // Batched kernels: one workgroup per (frame, item) of a stage's item space
@@ -507,12 +513,11 @@ __global__ __launch_bounds__(256) void batch_tiles_kernel(const BatchFrame* __re
   splat_tile(tile_job(f), tile_lds, queues, (int)(item - f.first_tile));
 }
 
-__global__ __launch_bounds__(256) void batch_spill_kernel(const BatchFrame* __restrict__ frames, int n,
-                                                          uint32_t total) {
+// Batched spill: the frames one after the other, each by the whole grid.
+__global__ __launch_bounds__(256) void batch_spill_kernel(const BatchFrame* __restrict__ frames, int n) {
   __shared__ Candidate cand[kMaxTest];
-  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-    const BatchFrame& f = frames[batch_find<&BatchFrame::first_bin>(frames, n, item)];
-    spill_bin(cand, tile_job(f), f.g.n_vpl, (int)(item - f.first_bin));
+  for (int f = 0; f < n; ++f) {
+    spill_frame(cand, tile_job(frames[f]), frames[f].g.n_vpl);
     __syncthreads();
   }
 }
@@ -616,7 +621,9 @@ hipError_t launch_ism_splat_tiled(hipStream_t s, const float* points, const uint
     hipLaunchKernelGGL(ism_tiles_kernel<uint32_t>, grid, dim3(256), tile_lds_bytes<uint32_t>(ism_side), s, j, n_tiles);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !any) return e;
-  hipLaunchKernelGGL(ism_spill_kernel, dim3(n_bins < 2048 ? n_bins : 2048), dim3(256), 0, s, j, n_bins);
+  // a few entries in the normal case: one workgroup per CU at most (the
+  // overflow fallback then strides over the bins)
+  hipLaunchKernelGGL(ism_spill_kernel, dim3(kSpillGrid), dim3(256), 0, s, j, n_bins);
   return hipGetLastError();
 }
 
@@ -664,7 +671,7 @@ hipError_t launch_batch_tiles(hipStream_t s, const BatchFrame* frames, int n, ui
 
 hipError_t launch_batch_spill(hipStream_t s, const BatchFrame* frames, int n, uint32_t total) {
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_spill_kernel, dim3(persistent_grid(total, 8)), dim3(256), 0, s, frames, n, total);
+  hipLaunchKernelGGL(batch_spill_kernel, dim3(kSpillGrid), dim3(256), 0, s, frames, n);
   return hipGetLastError();
 }
 
