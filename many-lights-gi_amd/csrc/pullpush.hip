@@ -5,14 +5,21 @@
 // FP32 oracle twin.
 //
 // The reference runs 2L full-atlas dispatches through RGBA32F mip chains.
-// Here one 256-lane workgroup owns one S x S ISM: it reads the S^2 u32 atlas
-// tile once with 16-byte loads into LDS, builds every pull level in LDS,
-// pushes back down in place (push level l overwrites pull level l pixel by
-// pixel; each pixel's original sample is read by the lane that writes it),
-// and writes the unorm16 result once. 2x2 pull blocks never cross ISMs and
-// the push border test gives zero weight to every texel of another ISM, so
-// ISMs are independent and nothing outside the tile is read. Zero-weight
-// terms contribute +-0 to the push sums, so skipping them is exact.
+// Here one 256-lane workgroup owns one S x S ISM: the first pull level reads
+// the u32 atlas tile (its 2x2 blocks with 8-byte loads), every pull level
+// lives in LDS, the push goes back down in place (push level l overwrites
+// pull level l pixel by pixel), and the level-0 push reads each pixel's
+// original sample from the atlas (an L2 hit by then) and writes the unorm16
+// result. 2x2 pull blocks never cross ISMs and the push border test gives
+// zero weight to every texel of another ISM, so ISMs are independent and
+// nothing outside the tile is read. Zero-weight terms contribute +-0 to the
+// push sums, so skipping them is exact.
+//
+// The LDS holds the pull levels only (21.8 KB at S = 64, L = 4), so seven
+// workgroups fit a CU: the small levels leave most of a workgroup's waves
+// waiting at barriers, and other workgroups fill the issue slots meanwhile.
+// (With the atlas tile and two radius-byte tables staged in LDS as well, 40
+// KB, four workgroups fit, and the 1080p pull-push took 102 us instead of 84.)
 //
 // Levels are compile-time (one kernel per ISM side and level count), so every
 // level's extents, scale and the fine-to-coarse offsets are constants.
@@ -169,7 +176,7 @@ constexpr float kRcp314 = 1.0f / 3.14f;
 
 // Level-0 decode of one atlas texel (pull.comp:37-39, :51-66): depth,
 // maxDepth and the radius in ISM pixels. r = float(s & 255) / 10 and
-// madd = r * 2 / zFar * 0.6 are passed in (tabulated per radius byte).
+// madd = r * 2 / zFar * 0.6 are passed in.
 struct Decoded {
   float depth, max_depth, radius;
 };
@@ -234,25 +241,29 @@ constexpr int ism_log2() {
   return (S == 8) ? 3 : (S == 16) ? 4 : (S == 32) ? 5 : 6;
 }
 
-// unorm16 of a depth in [0, 1] (R16 storage clamps first). Every depth here
-// is an atlas depth (s >> 8) / 2^24 or a rounded weighted mean of such
-// depths, in [0, 1 + 2^-22] at worst, and anything below 1 + 2^-17 rounds to
-// 65535 as 1.0 does, so the device form leaves the clamp out.
-__host__ __device__ __forceinline__ uint16_t to_r16(float depth) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const float r = depth;
-#else
-  const float r = fminf(fmaxf(depth, 0.0f), 1.0f);
-#endif
-  return (uint16_t)rintf(r * 65535.0f);
+// unorm16 of a depth in [0, 1] (R16 storage clamps first), as a 32-bit
+// word. Every depth here is an atlas depth (s >> 8) / 2^24 or a rounded
+// weighted mean of such depths, in [0, 1 + 2^-22] at worst, and anything
+// below 1 + 2^-17 rounds to 65535 as 1.0 does, so neither the clamp nor a
+// 16-bit mask is needed.
+__device__ __forceinline__ uint32_t to_r16(float depth) {
+  return (uint32_t)rintf(depth * 65535.0f);
+}
+
+// RN(c + 2 h) for the binary16 h in the low 16 bits of `bits`: doubling the
+// widened half is exact, so one FMA (v_fma_mix_f32, which widens the half
+// itself) gives the reference's c + h * 2.
+__device__ __forceinline__ float plus_twice_half(float c, uint32_t bits) {
+  const float h = (float)__builtin_bit_cast(_Float16, (uint16_t)(bits & 0xFFFFu));
+  return __builtin_fmaf(h, 2.0f, c);
 }
 
 // Pull of output (ox, oy) of level l >= 1 (pull.comp:20-126). Level 1 reads
-// the atlas tile and decodes it; higher levels read pull level l - 1.
+// the atlas tile at base (row stride `stride`) and decodes it; higher levels
+// read pull level l - 1.
 template <int S, int l>
 __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y,
-                                                   const uint32_t* base, const Texel* src,
-                                                   const float* rtab, const float* mtab,
+                                                   const uint32_t* base, int stride, const Texel* src,
                                                    const Decoded& empty, float z_far) {
   constexpr int so = S >> l, si = so * 2;
   constexpr float scale = 1.0f / (float)(1 << l);  // pow(2, -level)
@@ -267,9 +278,12 @@ __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, in
     const int offx = i >> 1, offy = i & 1;
     const int ix = ox * 2 + offx, iy = oy * 2 + offy;
     if constexpr (l == 1) {
-      const uint32_t s = base[iy * S + ix];
+      const uint32_t s = base[iy * stride + ix];
       const uint32_t b = s & 0xFFu;
-      const Decoded d = (s == 0xFFFFFFFFu) ? empty : decode_texel(s, rtab[b], mtab[b], z_far, S);
+      // r = b / 10 (div_by_recip is exact for the divisor 10) and the
+      // max-depth offset r * 2 / zFar * 0.6, as the reference forms them
+      const float r = div_by_recip((float)b, 10.0f, 1.0f / 10.0f);
+      const Decoded d = (s == 0xFFFFFFFFu) ? empty : decode_texel(s, r, r * 2.0f / z_far * 0.6f, z_far, S);
       ds[i] = d.depth;
       mds[i] = d.max_depth;
       rs[i] = d.radius;
@@ -306,15 +320,16 @@ __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, in
 // (a displacement c + d with c != 0 that cancels rounds to +0), so summing
 // the kept terms over +0 gives the same bits.
 template <int S, int l>
-__host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, Texel* same,
-                                                   const float* rcp) {
+__device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, Texel* same,
+                                          const float* rcp) {
   static_assert(l >= 1, "level 0 is push_quad_depth");
   constexpr int so = S >> l, sc = so / 2;
   constexpr float scale = 1.0f / (float)(1 << l);
   // offsets {0,0},{0,1},{1,1},{1,0} for both neighbours and fine pixels
   constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
   bool usable[4];
-  float cd[4], cmd[4], cr[4], cdx2[4], cdy2[4], bound[4];
+  uint32_t cdisp[4];
+  float cd[4], cmd[4], cr[4], bound[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
@@ -325,10 +340,7 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
     cd[i] = c.depth;
     cmd[i] = c.max_depth;
     cr[i] = c.radius;
-    float dxh, dyh;
-    unpack_half2_hd(c.disp, dxh, dyh);
-    cdx2[i] = dxh * 2.0f;
-    cdy2[i] = dyh * 2.0f;
+    cdisp[i] = c.disp;
     usable[i] = inside && c.depth != 1.0f;
     bound[i] = rd_bound(c.radius * scale);
   }
@@ -340,16 +352,19 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
     bool valid[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      // (coarse + 0.5) * 2 - (fine + 0.5) = 2 o_i - o_op - 0.5, exact
-      ddx[i] = ((float)(2 * ox[i] - ox[op]) - 0.5f) + cdx2[i];
-      ddy[i] = ((float)(2 * oy[i] - oy[op]) - 0.5f) + cdy2[i];
+      // (coarse + 0.5) * 2 - (fine + 0.5) = 2 o_i - o_op - 0.5, exact, plus
+      // twice the displacement
+      ddx[i] = plus_twice_half((float)(2 * ox[i] - ox[op]) - 0.5f, cdisp[i]);
+      ddy[i] = plus_twice_half((float)(2 * oy[i] - oy[op]) - 0.5f, cdisp[i] >> 16);
       const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
       valid[i] = usable[i] && !(x > bound[i]);  // sqrt(x) <= radius * 2^-level
     }
+    // the reference starts the minimum at 9001; every depth is at most
+    // 1 + 2^-22, so the first valid neighbour is always taken
     float minimum = 9001.0f, max_depth = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {  // fminf(cd, minimum) == cd is cd <= minimum
-      const bool take = valid[i] && !(cd[i] > minimum);
+      const bool take = valid[i] && (i == 0 || !(cd[i] > minimum));
       minimum = take ? cd[i] : minimum;
       max_depth = take ? cmd[i] : max_depth;
     }
@@ -399,9 +414,11 @@ __host__ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* 
 // survives the max-depth test is `keep`, and a fill needs an occluding kept
 // neighbour, so `wacc > 0 && occluded` is just `occluded`. Sums over the kept
 // terms in neighbour order are the sums of push_quad (its zero terms add +0).
+// The pixels' original samples come from the ISM's atlas tile src, the
+// results go to its result tile dst (both of row stride `stride`).
 template <int S>
-__device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coarse, uint32_t* base,
-                                                const float* rcp) {
+__device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coarse, const uint32_t* src,
+                                                uint16_t* dst, int stride, const float* rcp) {
   constexpr int so = S, sc = S / 2;
   constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
   bool usable[4];
@@ -418,13 +435,10 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
     cmd[i] = c.max_depth;
     bnd[i] = c.radius;
     usable[i] = inside && c.depth != 1.0f;
-    float dxh, dyh;
-    unpack_half2_hd(c.disp, dxh, dyh);
-    const float dx2 = dxh * 2.0f, dy2 = dyh * 2.0f;
 #pragma unroll
     for (int f = 0; f < 2; ++f) {  // fine offset f along x / y
-      const float ex = ((float)(2 * ox[i] - f) - 0.5f) + dx2;
-      const float ey = ((float)(2 * oy[i] - f) - 0.5f) + dy2;
+      const float ex = plus_twice_half((float)(2 * ox[i] - f) - 0.5f, c.disp);
+      const float ey = plus_twice_half((float)(2 * oy[i] - f) - 0.5f, c.disp >> 16);
       hx[f][i] = ex * ex;
       hy[f][i] = ey * ey;
     }
@@ -436,15 +450,17 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
     bool valid[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) valid[i] = usable[i] && !(hx[ox[op]][i] + hy[oy[op]][i] > bnd[i]);
+    // the reference starts the minimum at 9001; every depth is at most
+    // 1 + 2^-22, so the first valid neighbour is always taken
     float minimum = 9001.0f, max_depth = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const bool take = valid[i] && !(cd[i] > minimum);
+      const bool take = valid[i] && (i == 0 || !(cd[i] > minimum));
       minimum = take ? cd[i] : minimum;
       max_depth = take ? cmd[i] : max_depth;
     }
-    const int t = py * so + px;
-    const float orig = (float)(base[t] >> 8) * (1.0f / 16777216.0f);
+    const int t = py * stride + px;
+    const float orig = (float)(src[t] >> 8) * (1.0f / 16777216.0f);
     bool occluded = false;
     float wacc = 0.0f, dacc = 0.0f;
 #pragma unroll
@@ -452,12 +468,16 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
       const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
       const bool keep = valid[i] && !(cd[i] > max_depth);
       occluded = occluded || (keep && orig > cmd[i]);
-      wacc = wacc + (keep ? (wt == 0 ? 9.0f : (wt == 2 ? 1.0f : 3.0f)) : 0.0f);
-      dacc = dacc + (keep ? (wt == 0 ? cd9[i] : (wt == 2 ? cd[i] : cd3[i])) : 0.0f);
+      // every term is +0 or positive (depths are >= +0), so the first one
+      // is the sum +0 + term itself
+      const float w = keep ? (wt == 0 ? 9.0f : (wt == 2 ? 1.0f : 3.0f)) : 0.0f;
+      const float d = keep ? (wt == 0 ? cd9[i] : (wt == 2 ? cd[i] : cd3[i])) : 0.0f;
+      wacc = i == 0 ? w : wacc + w;
+      dacc = i == 0 ? d : dacc + d;
     }
     // rcp[0] = 0 keeps the unused wacc = 0 case finite
     const float depth = occluded ? div_by_recip(dacc, wacc, rcp[(int)wacc]) : orig;
-    base[t] = to_r16(depth);  // the tile slot now holds the result
+    dst[t] = (uint16_t)to_r16(depth);
   }
 }
 
@@ -495,18 +515,23 @@ template <int S, int L>
 constexpr bool kCompactPull1 = list_fits<S, L>(kGroupMax<S> * (S / 2) * (S / 2), 2);
 
 
-// One workgroup's group of ISMs ism0 .. ism0 + groups - 1: tile g at
-// base + g * S * S, its pull levels in lv (lv_offset).
+// One workgroup's group of ISMs ism0 .. ism0 + groups - 1: member g's atlas
+// tile and result at origin(S, g) of the atlas and the result (row stride
+// atlas_side), its pull levels in lv (lv_offset).
 struct Tables {
-  uint32_t* base;
+  const uint32_t* atlas;  // the atlas (global memory)
+  uint16_t* out;          // the unorm16 result (global memory)
   Texel* lv;
-  const float* rtab;
-  const float* mtab;
   const float* rcp;
   int* counts;  // list length of the two-pass pull level 1
   Decoded empty;
   float z_far;
-  int ism0, n1d_log2, groups;
+  int ism0, n1d_log2, groups, atlas_side;
+  // offset of group member g's tile in the atlas
+  __device__ size_t origin(int S, int g) const {
+    const int ism = ism0 + g;
+    return (size_t)(ism >> n1d_log2) * S * atlas_side + (size_t)(ism & ((1 << n1d_log2) - 1)) * S;
+  }
 };
 
 // The group size, a compile-time 1 where a group holds one ISM (S = 64), so
@@ -539,9 +564,9 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
       const int g = t / n, o = t - g * n;
       const int ism = tb.ism0 + g;
       const int ism_x = ism & ((1 << tb.n1d_log2) - 1), ism_y = ism >> tb.n1d_log2;
-      tb.lv[lv_offset<S>(l) + t] = pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.base + g * (S * S),
-                                                  tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.rtab, tb.mtab,
-                                                  tb.empty, tb.z_far);
+      tb.lv[lv_offset<S>(l) + t] =
+          pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.atlas + tb.origin(S, g), tb.atlas_side,
+                         tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.empty, tb.z_far);
     };
     if constexpr (l == 1 && kCompactPull1<S, L>) {
       // pass 1: all-empty 2x2 blocks are E_1, the others go to the list
@@ -549,9 +574,9 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
       for (int t = threadIdx.x; t < total; t += kThreads) {
         const int g = t / n, o = t - g * n;
         const int ox = o % so, oy = o / so;
-        const uint32_t* base = tb.base + g * (S * S);
-        const uint2 a = *reinterpret_cast<const uint2*>(&base[(2 * oy) * S + 2 * ox]);
-        const uint2 b = *reinterpret_cast<const uint2*>(&base[(2 * oy + 1) * S + 2 * ox]);
+        const uint32_t* base = tb.atlas + tb.origin(S, g);
+        const uint2 a = *reinterpret_cast<const uint2*>(&base[(2 * oy) * tb.atlas_side + 2 * ox]);
+        const uint2 b = *reinterpret_cast<const uint2*>(&base[(2 * oy + 1) * tb.atlas_side + 2 * ox]);
         const bool empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
         if (empty) tb.lv[lv_offset<S>(1) + t] = pc.e1;
         list_append(list, &tb.counts[0], !empty, t);
@@ -576,7 +601,8 @@ __device__ __forceinline__ void push_levels(const Tables& tb) {
       const int g = t / nq, q = t - g * nq;
       const Texel* coarse = tb.lv + lv_offset<S>(l + 1) + g * (sc * sc);
       if constexpr (l == 0)
-        push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.base + g * (S * S), tb.rcp);
+        push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.atlas + tb.origin(S, g), tb.out + tb.origin(S, g),
+                           tb.atlas_side, tb.rcp);
       else
         push_quad<S, l>(q % nq1, q / nq1, coarse, tb.lv + lv_offset<S>(l) + g * (so * so), tb.rcp);
     };
@@ -586,14 +612,13 @@ __device__ __forceinline__ void push_levels(const Tables& tb) {
   }
 }
 
-// LDS of one workgroup's pull-push: kGroupMax<S> tiles (4096 texels) and
-// their pull levels.
+// LDS of one workgroup's pull-push: the pull levels of kGroupMax<S> ISMs
+// (4096 texels).
 template <int S, int L>
 struct TileLds {
   static constexpr int kLv = level_offset<S>(L + 1);
-  uint32_t base[kGroupMax<S> * S * S];
   Texel lv[kGroupMax<S> * kLv];
-  float rtab[256], mtab[256], rcp[17];
+  float rcp[17];
   int counts[1];  // list length of the two-pass pull level 1
 };
 
@@ -606,33 +631,15 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
                                               uint16_t* __restrict__ out, int ism0, int groups_) {
   const int tid = threadIdx.x;
   const int groups = group_count<S>(groups_);
-  const int n1d = atlas_side / S;
-  const int n1d_log2 = 31 - __builtin_clz(n1d);
-  constexpr int n4 = S * S / 4;  // 16-byte words per tile
-  auto origin = [&](int g) {
-    const int ism = ism0 + g;
-    return (size_t)(ism >> n1d_log2) * S * atlas_side + (size_t)(ism & (n1d - 1)) * S;
-  };
+  const int n1d_log2 = 31 - __builtin_clz(atlas_side / S);
 
-  // ---- radius-byte tables (same expressions as the per-texel decode)
-  for (int b = tid; b < 256; b += kThreads) {
-    const float r = (float)b / 10.0f;
-    sm.rtab[b] = r;
-    sm.mtab[b] = r * 2.0f / z_far * 0.6f;
-  }
   if (tid < 17) sm.rcp[tid] = kRcp[tid];
   if (tid == 0) sm.counts[0] = 0;
-  // ---- load the group's atlas tiles (S*S u32 each) with 16-byte loads
-  for (int i = tid; i < groups * n4; i += kThreads) {
-    const int g = i / n4, w = i - g * n4;
-    const int row = (w * 4) / S, col = (w * 4) % S;
-    const uint4 v = gload4(reinterpret_cast<const uint4*>(atlas + origin(g) + (size_t)row * atlas_side + col));
-    *reinterpret_cast<uint4*>(&sm.base[g * S * S + row * S + col]) = v;
-  }
   __syncthreads();
-  Tables tb{sm.base, sm.lv, sm.rtab, sm.mtab, sm.rcp, sm.counts, {}, z_far, ism0, n1d_log2, groups};
+  Tables tb{atlas, out, sm.lv, sm.rcp, sm.counts, {}, z_far, ism0, n1d_log2, groups, atlas_side};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
-  tb.empty = decode_texel(0xFFFFFFFFu, sm.rtab[255], sm.mtab[255], z_far, S);
+  const float r255 = (float)255 / 10.0f;
+  tb.empty = decode_texel(0xFFFFFFFFu, r255, r255 * 2.0f / z_far * 0.6f, z_far, S);
 
   pull_levels<S, 1, L>(tb, pc);
   if constexpr (L == 1) {
@@ -645,15 +652,6 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
     __syncthreads();
   }
   push_levels<S, L - 1, L>(tb);
-
-  // ---- write the unorm16 tiles (held in the low halves of the tile slots)
-  for (int i = tid; i < groups * n4; i += kThreads) {
-    const int g = i / n4, w = i - g * n4;
-    const int row = (w * 4) / S, col = (w * 4) % S;
-    const uint4 v = *reinterpret_cast<const uint4*>(&sm.base[g * S * S + row * S + col]);
-    gstore(reinterpret_cast<uint64_t*>(out + origin(g) + (size_t)row * atlas_side + col),
-           (uint64_t)(v.x | (v.y << 16)) | ((uint64_t)(v.z | (v.w << 16)) << 32));
-  }
 }
 
 template <int S, int L>
