@@ -15,11 +15,12 @@
 // nothing outside the tile is read. Zero-weight terms contribute +-0 to the
 // push sums, so skipping them is exact.
 //
-// The LDS holds the pull levels only (21.8 KB at S = 64, L = 4), so seven
-// workgroups fit a CU: the small levels leave most of a workgroup's waves
-// waiting at barriers, and other workgroups fill the issue slots meanwhile.
-// (With the atlas tile and two radius-byte tables staged in LDS as well, 40
-// KB, four workgroups fit, and the 1080p pull-push took 102 us instead of 84.)
+// The LDS holds the pull levels and a 1 KB table of max-depth offsets (22.9
+// KB at S = 64, L = 4), so seven workgroups fit a CU: the small levels leave
+// most of a workgroup's waves waiting at barriers, and other workgroups fill
+// the issue slots meanwhile. (With the atlas tile and a second radius-byte
+// table staged in LDS as well, 40 KB, four workgroups fit, and the 1080p
+// pull-push took 102 us instead of 83.)
 //
 // Levels are compile-time (one kernel per ISM side and level count), so every
 // level's extents, scale and the fine-to-coarse offsets are constants.
@@ -264,7 +265,7 @@ __device__ __forceinline__ float plus_twice_half(float c, uint32_t bits) {
 template <int S, int l>
 __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y,
                                                    const uint32_t* base, int stride, const Texel* src,
-                                                   const Decoded& empty, float z_far) {
+                                                   const float* mtab, const Decoded& empty, float z_far) {
   constexpr int so = S >> l, si = so * 2;
   constexpr float scale = 1.0f / (float)(1 << l);  // pow(2, -level)
   float ds[4], mds[4], rs[4], dx[4], dy[4];
@@ -280,10 +281,10 @@ __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, in
     if constexpr (l == 1) {
       const uint32_t s = base[iy * stride + ix];
       const uint32_t b = s & 0xFFu;
-      // r = b / 10 (div_by_recip is exact for the divisor 10) and the
-      // max-depth offset r * 2 / zFar * 0.6, as the reference forms them
+      // r = b / 10 (div_by_recip is exact for the divisor 10); the max-depth
+      // offset r * 2 / zFar * 0.6 is tabulated per radius byte
       const float r = div_by_recip((float)b, 10.0f, 1.0f / 10.0f);
-      const Decoded d = (s == 0xFFFFFFFFu) ? empty : decode_texel(s, r, r * 2.0f / z_far * 0.6f, z_far, S);
+      const Decoded d = (s == 0xFFFFFFFFu) ? empty : decode_texel(s, r, mtab[b], z_far, S);
       ds[i] = d.depth;
       mds[i] = d.max_depth;
       rs[i] = d.radius;
@@ -333,10 +334,11 @@ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, T
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
-    // ISM border test: coarse neighbours outside this ISM get weight 0
+    // ISM border test: coarse neighbours outside this ISM get weight 0 (as
+    // unusable, their values are never used: they are read from the nearest
+    // texel inside)
     const bool inside = cx >= 0 && cy >= 0 && cx < sc && cy < sc;
-    Texel c{1.0f, 0.0f, 0.0f, 0u};
-    if (inside) c = coarse[cy * sc + cx];
+    const Texel c = coarse[min(max(cy, 0), sc - 1) * sc + min(max(cx, 0), sc - 1)];
     cd[i] = c.depth;
     cmd[i] = c.max_depth;
     cr[i] = c.radius;
@@ -427,8 +429,7 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
   for (int i = 0; i < 4; ++i) {
     const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
     const bool inside = cx >= 0 && cy >= 0 && cx < sc && cy < sc;
-    Texel c{1.0f, 0.0f, 0.0f, 0u};
-    if (inside) c = coarse[cy * sc + cx];
+    const Texel c = coarse[min(max(cy, 0), sc - 1) * sc + min(max(cx, 0), sc - 1)];  // as in push_quad
     cd[i] = c.depth;
     cd3[i] = c.depth * 3.0f;
     cd9[i] = c.depth * 9.0f;
@@ -523,6 +524,7 @@ struct Tables {
   uint16_t* out;          // the unorm16 result (global memory)
   Texel* lv;
   const float* rcp;
+  const float* mtab;  // r * 2 / zFar * 0.6 per radius byte (r = b / 10)
   int* counts;  // list length of the two-pass pull level 1
   Decoded empty;
   float z_far;
@@ -566,7 +568,7 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
       const int ism_x = ism & ((1 << tb.n1d_log2) - 1), ism_y = ism >> tb.n1d_log2;
       tb.lv[lv_offset<S>(l) + t] =
           pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.atlas + tb.origin(S, g), tb.atlas_side,
-                         tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.empty, tb.z_far);
+                         tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.mtab, tb.empty, tb.z_far);
     };
     if constexpr (l == 1 && kCompactPull1<S, L>) {
       // pass 1: all-empty 2x2 blocks are E_1, the others go to the list
@@ -618,7 +620,7 @@ template <int S, int L>
 struct TileLds {
   static constexpr int kLv = level_offset<S>(L + 1);
   Texel lv[kGroupMax<S> * kLv];
-  float rcp[17];
+  float mtab[256], rcp[17];
   int counts[1];  // list length of the two-pass pull level 1
 };
 
@@ -633,10 +635,12 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
   const int groups = group_count<S>(groups_);
   const int n1d_log2 = 31 - __builtin_clz(atlas_side / S);
 
+  // max-depth offsets per radius byte (the reference's expression)
+  for (int b = tid; b < 256; b += kThreads) sm.mtab[b] = (float)b / 10.0f * 2.0f / z_far * 0.6f;
   if (tid < 17) sm.rcp[tid] = kRcp[tid];
   if (tid == 0) sm.counts[0] = 0;
   __syncthreads();
-  Tables tb{atlas, out, sm.lv, sm.rcp, sm.counts, {}, z_far, ism0, n1d_log2, groups, atlas_side};
+  Tables tb{atlas, out, sm.lv, sm.rcp, sm.mtab, sm.counts, {}, z_far, ism0, n1d_log2, groups, atlas_side};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
   const float r255 = (float)255 / 10.0f;
   tb.empty = decode_texel(0xFFFFFFFFu, r255, r255 * 2.0f / z_far * 0.6f, z_far, S);
