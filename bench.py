@@ -147,14 +147,17 @@ def valu_roof(kernel_ms, sq, peak_fma_s):
     """VALU issue roofline of a kernel: VALU wave-instructions per launch
     (rocprofv3 SQ_INSTS_VALU, committed profile) over the live kernel time,
     against the measured FP32 FMA issue rate (one wave64 instruction = 64
-    lane FMAs)."""
+    lane FMAs), and against the issue ceiling of non-packed instructions
+    (spec_peak: one wave64 instruction per SIMD every 4 cycles at 2.4 GHz),
+    which the probe reaches to within a few percent."""
     if not sq or kernel_ms <= 0:
         return None
     peak = peak_fma_s / 64.0 / 1e9 if peak_fma_s else VALU_PEAK_GINST_S
     ach = sq["SQ_INSTS_VALU"] / (kernel_ms * 1e-3) / 1e9
     return {"insts_per_launch": sq["SQ_INSTS_VALU"], "achieved": round(ach, 1), "peak": round(peak, 1),
             "unit": "G wave64-inst/s", "frac": round(ach / peak, 4),
-            "peak_source": "measured FMA probe" if peak_fma_s else "spec"}
+            "peak_source": "measured FMA probe" if peak_fma_s else "spec",
+            "spec_peak": VALU_PEAK_GINST_S, "spec_frac": round(ach / VALU_PEAK_GINST_S, 4)}
 
 
 def kernel_rooflines(wl, stages, lit, sq=None, peak_fma_s=None, single=False, splats=None, spills=None):
