@@ -256,7 +256,8 @@ int mlgi_splat_spill_stats(mlgi_context* ctx, uint32_t* spills, int32_t* overflo
 int mlgi_splat_count(mlgi_context* ctx, uint64_t* splats);
 /* Pull-push hole filling, one workgroup per ISM, R16 result. Bit-exact.
  * atlas->ism_side (ismPixelSize) must be one of 8, 16, 32, 64; the atlas
- * must be 16-byte and the output 8-byte aligned (as mlgi_malloc returns). */
+ * must be 16-byte and the output 8-byte aligned (as mlgi_malloc returns),
+ * and the output must not overlap the atlas (MLGI_ERR_INVALID_ARGUMENT). */
 int mlgi_ism_pullpush(mlgi_context* ctx, const mlgi_atlas* atlas, const mlgi_params* p,
                       mlgi_ism_depth* out);
 /* Splat + pull-push (ImperfectShadowmap::process). */
@@ -339,7 +340,7 @@ typedef struct mlgi_frame_desc {
   mlgi_params params;   /* per frame; tile_w selects the gather strip width (8, 16, 32) */
   mlgi_vpls vpls;       /* caller-owned work and output buffers */
   mlgi_atlas atlas;
-  mlgi_ism_depth ism;
+  mlgi_ism_depth ism;   /* must not overlap the atlas, as for mlgi_ism_pullpush */
   mlgi_image image;
 } mlgi_frame_desc;
 typedef struct mlgi_batch mlgi_batch;

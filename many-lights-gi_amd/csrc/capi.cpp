@@ -48,6 +48,12 @@ int hip_check(mlgi_context* c, hipError_t e, const char* what) {
 }
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Whether the byte ranges [a, a + na) and [b, b + nb) share a byte.
+bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + nb && y < x + na;
+}
 int log2i(int v) {
   int k = 0;
   while ((1 << k) < v) ++k;
@@ -588,9 +594,15 @@ int mlgi_ism_pullpush(mlgi_context* c, const mlgi_atlas* atlas, const mlgi_param
   if (p->pull_push_levels < 1 || p->pull_push_levels > log2i(s))
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "pull-push levels %d out of range 1..%d",
                 p->pull_push_levels, log2i(s));
-  // the kernel reads the atlas in 16-byte and writes the ISM in 8-byte units
+  // the alignment every entry point states for these buffers (the splat's
+  // tile pass writes the atlas in 16-byte units; the pull-push reads it in 8-
+  // and writes the ISM in 2-byte units)
   if (((uintptr_t)atlas->texels & 15u) != 0 || ((uintptr_t)out->texels & 7u) != 0)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas must be 16-byte and ISM 8-byte aligned");
+  // the level-0 push writes the result while other workgroups still read the atlas
+  const size_t texels = (size_t)atlas->side * (size_t)atlas->side;
+  if (overlaps(atlas->texels, texels * sizeof(uint32_t), out->texels, texels * sizeof(uint16_t)))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "the pull-push output overlaps the atlas");
   out->ism_side = s;
   Scope sc(c, kPullPush);
   return hip_check(c,
@@ -918,10 +930,15 @@ int batch_frame(mlgi_context* c, const mlgi_frame_desc& d, mlgi::BatchFrame* f) 
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "point bins (%d) must equal the VPL count (%d)", pts.n_bins,
                 d.vpls.count);
   if (!d.atlas.texels || !d.ism.texels) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas / ism is NULL");
-  // the batched tile pass writes and the pull-push reads the atlas in 16-byte
-  // units; the pull-push writes the ISM in 8-byte units
+  // the batched tile pass writes the atlas in 16-byte units (and the
+  // alignment of mlgi_ism_pullpush holds for the ISM)
   if (((uintptr_t)d.atlas.texels & 15u) != 0 || ((uintptr_t)d.ism.texels & 7u) != 0)
     return fail(c, MLGI_ERR_INVALID_ARGUMENT, "atlas must be 16-byte and ISM 8-byte aligned");
+  // the pull-push writes the ISM while other workgroups still read the atlas
+  if (d.atlas.side > 0 && d.ism.side > 0 &&
+      overlaps(d.atlas.texels, (size_t)d.atlas.side * d.atlas.side * sizeof(uint32_t), d.ism.texels,
+               (size_t)d.ism.side * d.ism.side * sizeof(uint16_t)))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "the ISM overlaps the atlas");
   int n1d, s;
   if ((rc = atlas_geometry(c, d.atlas.side, d.vpls.count, &n1d, &s))) return rc;
   if ((d.atlas.ism_side != 0 && d.atlas.ism_side != s) || d.ism.side != d.atlas.side ||

@@ -170,6 +170,17 @@ def test_invalid_arguments_rejected(ctx):
     rc = mlgi.lib.mlgi_final_gather(ctx.h, C.byref(gb), C.byref(st.vpl_processor.desc()), C.byref(d),
                                     C.byref(st.params), C.byref(im))
     assert rc == N.MLGI_ERR_INVALID_ARGUMENT
+    # a pull-push output that overlaps the atlas (written while other
+    # workgroups still read it), alone and in a batch
+    aliased = st.ism.depth_desc()
+    aliased.texels = st.ism.softrender.ptr + 4 * wl.atlas_side  # inside the atlas, 8-byte aligned
+    rc = mlgi.lib.mlgi_ism_pullpush(ctx.h, C.byref(st.ism.atlas_desc()), C.byref(st.params), C.byref(aliased))
+    assert rc == N.MLGI_ERR_INVALID_ARGUMENT
+    fd = mlgi.frame_desc(st, rsm, pts, gb)
+    fd.ism.texels = st.ism.softrender.ptr
+    bp = C.c_void_p()
+    rc = mlgi.lib.mlgi_batch_create(ctx.h, C.byref(fd), 1, C.byref(bp))
+    assert rc == N.MLGI_ERR_INVALID_ARGUMENT
     # the context still works afterwards
     st.process(rsm, pts, gb)
     ctx.sync()
