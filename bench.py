@@ -1,7 +1,7 @@
 """GI benchmarks on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1080p]
-                    [--form fp32|mixed|both] [--no-cpu]
+                    [--form fp32|mixed|both] [--slots 3] [--no-cpu]
 
 Default (workload 1080p): a step is one GI frame of the many-lights hot path
 on inputs already resident in HBM -- VPL extraction from the RSM, ISM atlas
@@ -20,13 +20,19 @@ synchronize on both sides, with no timing events inside; ms_per_step is
 wall time / K. A second timed region of the same K frames then records HIP
 events on the context stream around every stage of every frame (the
 per-stage spans and the kernels' rooflines come from it; its wall time per
-frame is reported as profiled_ms_per_frame). `value` is the
-mixed form through the faster of its two paths in this run (both drop-in
-entry points for one frame; config.form names it; the other is reported
-under forms). Batched forms are also
-replayed from a captured HIP graph (graph_ms_per_frame). Precision is
+frame is reported as profiled_ms_per_frame). Batched forms are also
+replayed from a captured HIP graph (graph_ms_per_frame). Then, with
+--slots S > 1 (default 3), the batched forms run K frames once more with S
+frames in flight (fp32_pipelined, mixed_pipelined): S frame slots, each
+with its own buffers and its own context and stream, frame i on slot
+i mod S, so that one frame's last workgroups of a stage share the GPU with
+the next frame's work; every frame does the whole work. `value` is
+mixed_pipelined (config.form, config.frames_in_flight); serial_ms_per_frame
+is the mixed form one frame after another on one stream, through the faster
+of its two paths (both drop-in entry points for one frame). Precision is
 compared within a path: mixed_speedup_vs_fp32 = fp32 / mixed_single,
-mixed_speedup_same_path = fp32_batched / mixed (and its graph replays).
+mixed_speedup_same_path = fp32_batched / mixed (and its graph replays),
+mixed_speedup_pipelined = fp32_pipelined / mixed_pipelined.
 
 roofline: for the kernel with the largest mean duration in the timed frames:
 its VALU issue rate ("valu": SQ_INSTS_VALU per launch from the committed
@@ -141,6 +147,42 @@ def run_form(ctx, wl, inp, form: str, steps: int, warmup: int):
         g.close()
         batch.close()
     return out
+
+
+def run_pipelined(ctx, wl, inp, prec, steps: int, warmup: int, slots: int):
+    """K frames through the batched kernels with `slots` frames in flight:
+    each slot has its own frame buffers (GIStage, uploaded inputs) and its
+    own context, hence its own stream, and frame i runs on slot i mod slots.
+    Every frame does the whole work; a frame's slot is free again once the
+    frame `slots` earlier on it has finished (stream order), and frames on
+    different slots overlap on the device, so one frame's last workgroups
+    of a stage share the GPU with the next frame's work. Returns wall ms per
+    frame."""
+    import mlgi
+
+    ctxs = [ctx] + [mlgi.Context(ctx.device) for _ in range(slots - 1)]
+    batches = []
+    try:
+        for c in ctxs:
+            st = mlgi.GIStage(c, wl.params(precision=prec), wl.n_vpl, wl.width, wl.height)
+            rsm, pts, gb = upload_frame(c, inp)
+            batches.append(mlgi.Batch(c, [(st, rsm, pts, gb)]))
+        # untimed: every slot's first frames (at least 8 per slot)
+        for i in range(max(warmup, 8 * slots)):
+            batches[i % slots].run()
+        for c in ctxs:
+            c.sync()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            batches[i % slots].run()
+        for c in ctxs:
+            c.sync()
+        return (time.perf_counter() - t0) * 1e3 / steps
+    finally:
+        for b in batches:
+            b.close()
+        for c in ctxs[1:]:
+            c.close()
 
 
 def valu_roof(kernel_ms, sq, peak_fma_s):
@@ -264,10 +306,26 @@ def frame_bench(a, ctx, wl):
         log(f"[bench] form {form}: warmup {a.warmup}, steps {a.steps}")
         res[form] = run_form(ctx, wl, inp, form, a.steps, a.warmup)
         log(f"[bench] {form}: {res[form]['wall_ms']:.4f} ms/frame, stages {res[form]['stages']}")
+    # frames in flight: the batched forms again with a.slots frame slots
+    piped = {}
+    if a.slots > 1:
+        import mlgi
+
+        for form, prec in (("fp32_pipelined", mlgi.PRECISION_FP32), ("mixed_pipelined", mlgi.PRECISION_MIXED)):
+            if form.split("_")[0] in [f.split("_")[0] for f in forms]:
+                log(f"[bench] form {form}: {a.slots} frame slots, warmup {a.warmup}, steps {a.steps}")
+                piped[form] = run_pipelined(ctx, wl, inp, prec, a.steps, a.warmup, a.slots)
+                log(f"[bench] {form}: {piped[form]:.4f} ms/frame")
     mixed_forms = [f for f in ("mixed_single", "mixed") if f in res]
     head = min(mixed_forms, key=lambda f: res[f]["wall_ms"]) if mixed_forms else "fp32"
     h = res[head]
     single = head in ("mixed_single", "fp32")
+    # value: the mixed form's frames in flight when measured, else its
+    # faster one-frame-at-a-time path
+    vform = "mixed_pipelined" if "mixed_pipelined" in piped else head
+    if "mixed" not in forms and "mixed_single" not in forms:
+        vform = "fp32_pipelined" if "fp32_pipelined" in piped else head
+    value_ms = piped[vform] if vform in piped else h["wall_ms"]
     peak_fma_s = ctx.probe_valu()
     log(f"[bench] FP32 FMA probe: {peak_fma_s * 2 / 1e12:.1f} TFLOP/s")
     sq = None
@@ -300,20 +358,26 @@ def frame_bench(a, ctx, wl):
                         "stages_ms": {s: round(x, 4) for s, x in v["stages"].items() if s != "frames"}}
         if "graph_ms" in v:
             forms_out[k]["graph_ms_per_frame"] = round(v["graph_ms"], 4)
+    for k, v in piped.items():
+        forms_out[k] = {"ms_per_frame": round(v, 4),
+                        "path": f"batched (mlgi_batch_run), {a.slots} frames in flight (one frame slot and "
+                                f"stream each)"}
     out = {
         "metric": "GI ms/frame 1080p·4096 VPL (mixed vs FP32 HIP)",
-        "value": round(h["wall_ms"], 4),
+        "value": round(value_ms, 4),
         "unit": "ms/frame",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(h["wall_ms"], 4),
+        "ms_per_step": round(value_ms, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16x2+fp32" if head.startswith("mixed") else "fp32",
+        "dtype": "fp16x2+fp32" if vform.startswith("mixed") else "fp32",
         "data": DATA,
-        "config": config_of(wl, head),
+        "config": dict(config_of(wl, vform), frames_in_flight=a.slots if vform in piped else 1),
+        # one frame after another on one stream: the mixed form's faster path
+        "serial_ms_per_frame": round(h["wall_ms"], 4),
         "forms": forms_out,
         "roofline": roof,
         "kernels": kernels,
@@ -328,6 +392,8 @@ def frame_bench(a, ctx, wl):
         g32, gmx = res["fp32_batched"].get("graph_ms"), res["mixed"].get("graph_ms")
         if g32 and gmx:
             out["mixed_speedup_same_path_graph"] = round(g32 / gmx, 4)
+    if "fp32_pipelined" in piped and "mixed_pipelined" in piped:
+        out["mixed_speedup_pipelined"] = round(piped["fp32_pipelined"] / piped["mixed_pipelined"], 4)
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(wl, inp, a.cpu_threads or host_threads())
@@ -581,6 +647,8 @@ def main() -> int:
     ap.add_argument("--workload", default="1080p", choices=["1080p", "small", "4k", "batch64", "serve"])
     ap.add_argument("--form", default="both", choices=["fp32", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--slots", type=int, default=3,
+                    help="frame workloads: frames in flight for the *_pipelined forms (1: not run)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: host_threads())")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate (req/s)")
     ap.add_argument("--seconds", type=float, default=60.0, help="serve: length of the arrival trace")
