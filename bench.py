@@ -189,9 +189,10 @@ def valu_roof(kernel_ms, sq, peak_fma_s):
     """VALU issue roofline of a kernel: VALU wave-instructions per launch
     (rocprofv3 SQ_INSTS_VALU, committed profile) over the live kernel time,
     against the measured FP32 FMA issue rate (one wave64 instruction = 64
-    lane FMAs), and against the issue ceiling of non-packed instructions
-    (spec_peak: one wave64 instruction per SIMD every 4 cycles at 2.4 GHz),
-    which the probe reaches to within a few percent."""
+    lane FMAs), and against the nominal FMA issue rate (spec_peak: one wave64
+    FMA per SIMD every 4 cycles at 2.4 GHz), which the probe reaches to
+    within a few percent; instruction mixes with compares and selects can
+    issue faster than either."""
     if not sq or kernel_ms <= 0:
         return None
     peak = peak_fma_s / 64.0 / 1e9 if peak_fma_s else VALU_PEAK_GINST_S
