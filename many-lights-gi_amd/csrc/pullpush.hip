@@ -251,6 +251,16 @@ __device__ __forceinline__ uint32_t to_r16(float depth) {
   return (uint32_t)rintf(depth * 65535.0f);
 }
 
+// The atlas and the result are read and written through raw buffer
+// resources (base and size in scalar registers, a 32-bit byte offset per
+// lane: the atlas holds at most 2^28 texels), which spares every access the
+// 64-bit address arithmetic of a global pointer. Resource word 3 0x00020000
+// selects the 32-bit data format raw buffer accesses use on CDNA; accesses
+// past the size read 0 and write nothing.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+
 // RN(c + 2 h) for the binary16 h in the low 16 bits of `bits`: doubling the
 // widened half is exact, so one FMA (v_fma_mix_f32, which widens the half
 // itself) gives the reference's c + h * 2.
@@ -260,12 +270,12 @@ __device__ __forceinline__ float plus_twice_half(float c, uint32_t bits) {
 }
 
 // Pull of output (ox, oy) of level l >= 1 (pull.comp:20-126). Level 1 reads
-// the atlas tile at base (row stride `stride`) and decodes it; higher levels
-// read pull level l - 1.
+// the atlas tile at texel `base` of the atlas (row stride `stride`) and
+// decodes it; higher levels read pull level l - 1.
 template <int S, int l>
-__host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y,
-                                                   const uint32_t* base, int stride, const Texel* src,
-                                                   const float* mtab, const Decoded& empty, float z_far) {
+__device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y, __amdgpu_buffer_rsrc_t atlas,
+                                          uint32_t base, int stride, const Texel* src, const float* mtab,
+                                          const Decoded& empty, float z_far) {
   constexpr int so = S >> l, si = so * 2;
   constexpr float scale = 1.0f / (float)(1 << l);  // pow(2, -level)
   float ds[4], mds[4], rs[4], dx[4], dy[4];
@@ -279,7 +289,7 @@ __host__ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, in
     const int offx = i >> 1, offy = i & 1;
     const int ix = ox * 2 + offx, iy = oy * 2 + offy;
     if constexpr (l == 1) {
-      const uint32_t s = base[iy * stride + ix];
+      const uint32_t s = __builtin_amdgcn_raw_buffer_load_b32(atlas, (base + iy * stride + ix) * 4u, 0, 0);
       const uint32_t b = s & 0xFFu;
       // r = b / 10 (div_by_recip is exact for the divisor 10); the max-depth
       // offset r * 2 / zFar * 0.6 is tabulated per radius byte
@@ -416,11 +426,13 @@ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, T
 // survives the max-depth test is `keep`, and a fill needs an occluding kept
 // neighbour, so `wacc > 0 && occluded` is just `occluded`. Sums over the kept
 // terms in neighbour order are the sums of push_quad (its zero terms add +0).
-// The pixels' original samples come from the ISM's atlas tile src, the
-// results go to its result tile dst (both of row stride `stride`).
+// The pixels' original samples come from the ISM's tile at texel `base` of
+// the atlas, the results go to the same texels of the result (both of row
+// stride `stride`).
 template <int S>
-__device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coarse, const uint32_t* src,
-                                                uint16_t* dst, int stride, const float* rcp) {
+__device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coarse, __amdgpu_buffer_rsrc_t atlas,
+                                                __amdgpu_buffer_rsrc_t out, uint32_t base, int stride,
+                                                const float* rcp) {
   constexpr int so = S, sc = S / 2;
   constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
   bool usable[4];
@@ -460,8 +472,9 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
       minimum = take ? cd[i] : minimum;
       max_depth = take ? cmd[i] : max_depth;
     }
-    const int t = py * stride + px;
-    const float orig = (float)(src[t] >> 8) * (1.0f / 16777216.0f);
+    const uint32_t t = base + (uint32_t)(py * stride + px);
+    const float orig =
+        (float)(__builtin_amdgcn_raw_buffer_load_b32(atlas, t * 4u, 0, 0) >> 8) * (1.0f / 16777216.0f);
     bool occluded = false;
     float wacc = 0.0f, dacc = 0.0f;
 #pragma unroll
@@ -478,7 +491,7 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
     }
     // rcp[0] = 0 keeps the unused wacc = 0 case finite
     const float depth = occluded ? div_by_recip(dacc, wacc, rcp[(int)wacc]) : orig;
-    dst[t] = (uint16_t)to_r16(depth);
+    __builtin_amdgcn_raw_buffer_store_b16((uint16_t)to_r16(depth), out, t * 2u, 0, 0);
   }
 }
 
@@ -520,8 +533,8 @@ constexpr bool kCompactPull1 = list_fits<S, L>(kGroupMax<S> * (S / 2) * (S / 2),
 // tile and result at origin(S, g) of the atlas and the result (row stride
 // atlas_side), its pull levels in lv (lv_offset).
 struct Tables {
-  const uint32_t* atlas;  // the atlas (global memory)
-  uint16_t* out;          // the unorm16 result (global memory)
+  __amdgpu_buffer_rsrc_t atlas;  // the atlas (buffer_of)
+  __amdgpu_buffer_rsrc_t out;    // the unorm16 result (buffer_of)
   Texel* lv;
   const float* rcp;
   const float* mtab;  // r * 2 / zFar * 0.6 per radius byte (r = b / 10)
@@ -529,10 +542,10 @@ struct Tables {
   Decoded empty;
   float z_far;
   int ism0, n1d_log2, groups, atlas_side;
-  // offset of group member g's tile in the atlas
-  __device__ size_t origin(int S, int g) const {
+  // texel offset of group member g's tile in the atlas
+  __device__ uint32_t origin(int S, int g) const {
     const int ism = ism0 + g;
-    return (size_t)(ism >> n1d_log2) * S * atlas_side + (size_t)(ism & ((1 << n1d_log2) - 1)) * S;
+    return (uint32_t)((ism >> n1d_log2) * S * atlas_side + (ism & ((1 << n1d_log2) - 1)) * S);
   }
 };
 
@@ -567,7 +580,7 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
       const int ism = tb.ism0 + g;
       const int ism_x = ism & ((1 << tb.n1d_log2) - 1), ism_y = ism >> tb.n1d_log2;
       tb.lv[lv_offset<S>(l) + t] =
-          pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.atlas + tb.origin(S, g), tb.atlas_side,
+          pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.atlas, tb.origin(S, g), tb.atlas_side,
                          tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.mtab, tb.empty, tb.z_far);
     };
     if constexpr (l == 1 && kCompactPull1<S, L>) {
@@ -576,10 +589,10 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
       for (int t = threadIdx.x; t < total; t += kThreads) {
         const int g = t / n, o = t - g * n;
         const int ox = o % so, oy = o / so;
-        const uint32_t* base = tb.atlas + tb.origin(S, g);
-        const uint2 a = *reinterpret_cast<const uint2*>(&base[(2 * oy) * tb.atlas_side + 2 * ox]);
-        const uint2 b = *reinterpret_cast<const uint2*>(&base[(2 * oy + 1) * tb.atlas_side + 2 * ox]);
-        const bool empty = (a.x & a.y & b.x & b.y) == 0xFFFFFFFFu;
+        const uint32_t row = (tb.origin(S, g) + (uint32_t)((2 * oy) * tb.atlas_side + 2 * ox)) * 4u;
+        const auto a = __builtin_amdgcn_raw_buffer_load_b64(tb.atlas, row, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(tb.atlas, row + (uint32_t)tb.atlas_side * 4u, 0, 0);
+        const bool empty = (a[0] & a[1] & b[0] & b[1]) == 0xFFFFFFFFu;
         if (empty) tb.lv[lv_offset<S>(1) + t] = pc.e1;
         list_append(list, &tb.counts[0], !empty, t);
       }
@@ -603,8 +616,7 @@ __device__ __forceinline__ void push_levels(const Tables& tb) {
       const int g = t / nq, q = t - g * nq;
       const Texel* coarse = tb.lv + lv_offset<S>(l + 1) + g * (sc * sc);
       if constexpr (l == 0)
-        push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.atlas + tb.origin(S, g), tb.out + tb.origin(S, g),
-                           tb.atlas_side, tb.rcp);
+        push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.atlas, tb.out, tb.origin(S, g), tb.atlas_side, tb.rcp);
       else
         push_quad<S, l>(q % nq1, q / nq1, coarse, tb.lv + lv_offset<S>(l) + g * (so * so), tb.rcp);
     };
@@ -640,7 +652,9 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
   if (tid < 17) sm.rcp[tid] = kRcp[tid];
   if (tid == 0) sm.counts[0] = 0;
   __syncthreads();
-  Tables tb{atlas, out, sm.lv, sm.rcp, sm.mtab, sm.counts, {}, z_far, ism0, n1d_log2, groups, atlas_side};
+  const uint32_t texels = (uint32_t)atlas_side * (uint32_t)atlas_side;
+  Tables tb{buffer_of(atlas, texels * 4u), buffer_of(out, texels * 2u), sm.lv, sm.rcp, sm.mtab, sm.counts, {},
+            z_far, ism0, n1d_log2, groups, atlas_side};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
   const float r255 = (float)255 / 10.0f;
   tb.empty = decode_texel(0xFFFFFFFFu, r255, r255 * 2.0f / z_far * 0.6f, z_far, S);
