@@ -86,9 +86,18 @@ __device__ __forceinline__ Candidate load_candidate(const float* __restrict__ vp
 // Radius byte of a point record's packed w (unorm4x8 .w * 25 is the radius;
 // the atlas keeps radius * 10 / sqrt(4)).
 __device__ __forceinline__ uint32_t radius_byte_of(float w) {
-  const float uw = (float)(__float_as_uint(w) >> 24) / 255.0f;
+  const float uw = dev::unorm8(__float_as_uint(w) >> 24);
   const float radius = uw * 25.0f;
   return dev::to_uint(radius * 10.0f / 2.0f);
+}
+
+// dot(a, b) < 0 for dot = (a.x b.x + a.y b.y) + a.z b.z in FP32, without
+// the last addition: rounding keeps the sign of the exact sum of its two
+// operands (a nonzero sum of two floats is a multiple of 2^-149, so it never
+// rounds to zero, and an exact 0 rounds to +0), so RN(s + t) < 0 is s < -t.
+__device__ __forceinline__ bool dot_below_zero(f3 a, f3 b) {
+  const float s = a.x * b.x + a.y * b.y;
+  return s < -(a.z * b.z);
 }
 
 // Bit i set: candidate i is one of the first kMaxCollect that pass both
@@ -97,9 +106,9 @@ __device__ __forceinline__ uint32_t keep_mask(const Candidate* cand, float4 rec)
   const f3 pos = dev::make(rec.x, rec.y, rec.z);
   const uint32_t bits = __float_as_uint(rec.w);
   // unpackUnorm4x8: c / 255
-  const float ux = (float)(bits & 0xFFu) / 255.0f;
-  const float uy = (float)((bits >> 8) & 0xFFu) / 255.0f;
-  const float uz = (float)((bits >> 16) & 0xFFu) / 255.0f;
+  const float ux = dev::unorm8(bits & 0xFFu);
+  const float uy = dev::unorm8((bits >> 8) & 0xFFu);
+  const float uz = dev::unorm8((bits >> 16) & 0xFFu);
   const f3 pn3 = dev::make(ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, uz * 2.0f - 1.0f);
   uint32_t keep = 0;
   int found = 0;
@@ -108,7 +117,7 @@ __device__ __forceinline__ uint32_t keep_mask(const Candidate* cand, float4 rec)
     const Candidate& c = cand[i];
     const f3 rel = dev::sub(pos, dev::make(c.px, c.py, c.pz));
     const f3 nrel = dev::make(-rel.x, -rel.y, -rel.z);
-    const bool cull = dev::dot(dev::make(c.nx, c.ny, c.nz), rel) < 0.0f || dev::dot(pn3, nrel) < 0.0f;
+    const bool cull = dot_below_zero(dev::make(c.nx, c.ny, c.nz), rel) || dot_below_zero(pn3, nrel);
     if (!cull && found < kMaxCollect) {
       keep |= 1u << i;
       ++found;

@@ -25,6 +25,17 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
   return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// c / 255 for a byte c (unpackUnorm4x8), correctly rounded: q = RN(c y)
+// with y = RN(1/255), the exact FMA remainder r = c - 255 q, then RN(q + r y)
+// (Markstein's form; equal to the IEEE quotient for all 256 bytes, which the
+// CPU tests check exhaustively). Three operations instead of the scaled
+// division sequence.
+__device__ __forceinline__ float unorm8(uint32_t c) {
+  const float x = (float)c, y = 1.0f / 255.0f;
+  const float q = x * y;
+  return __builtin_fmaf(__builtin_fmaf(-q, 255.0f, x), y, q);
+}
+
 // GLSL uint(float): truncation, negative values saturate to 0.
 __device__ __forceinline__ uint32_t to_uint(float v) {
   if (!(v > 0.0f)) return 0u;

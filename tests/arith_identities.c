@@ -32,6 +32,22 @@ long check_div_by_recip(float b, int y_ulps, int e0, int e1) {
   return bad;
 }
 
+/* Bytes c whose unpackUnorm4x8 value c / 255 differs from the splat's form
+ * div_by_recip(c, 255, RN(1/255)); with product_only, from RN(c RN(1/255))
+ * instead (which the harness must catch). */
+long check_unorm8(int product_only) {
+  volatile float d = 255.0f;
+  const float y = 1.0f / d;
+  long bad = 0;
+  for (uint32_t c = 0; c < 256; ++c) {
+    const float x = (float)c;
+    volatile float ref = x / d;
+    const float got = product_only ? x * y : div_by_recip(x, 255.0f, y);
+    if (f2u(got) != f2u(ref)) ++bad;
+  }
+  return bad;
+}
+
 /* m^2 with m the midpoint between R and the next float (exact in double);
  * midpoint = 0 gives the naive bound R^2 instead. */
 static double sqrt_gt_bound(float R, int midpoint) {

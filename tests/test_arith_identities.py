@@ -1,4 +1,4 @@
-"""Pins the exact arithmetic rewrites of the pull-push kernel (CPU only).
+"""Pins the exact arithmetic rewrites of the pull-push and splat kernels (CPU only).
 
 The kernel replaces two float operations of the reference's shaders by
 cheaper forms that give the same bits:
@@ -8,7 +8,9 @@ cheaper forms that give the same bits:
   * sqrt(x) > R in the radius tests becomes x > RD(m^2), m the midpoint
     between R and the next float, with RD(m^2) = RD(R next(R)) formed in
     FP32 from the rounded product and its FMA error (no FP64 on the GPU).
-Both are checked here against IEEE float division and sqrtf: the division
+The splat's unpackUnorm4x8 (c / 255 for a byte c) uses the first form with
+divisor 255, checked for all 256 bytes. Both are checked here against IEEE
+float division and sqrtf: the division
 over every significand of two binades per divisor and sign (scaling by a
 power of two leaves all three roundings unchanged), the comparison on random
 radii at the 65 floats around R^2 where the decision flips.
@@ -37,6 +39,8 @@ def lib(tmp_path_factory):
     L.check_rd_bound.restype = C.c_long
     L.check_rd_bound_sqrt.argtypes = [C.c_long, C.c_uint64]
     L.check_rd_bound_sqrt.restype = C.c_long
+    L.check_unorm8.argtypes = [C.c_int]
+    L.check_unorm8.restype = C.c_long
     return L
 
 
@@ -66,3 +70,10 @@ def test_planted_mistakes_are_caught(lib):
     assert lib.check_div_by_recip(3.0, 4096, -1, 0) > 0
     assert lib.check_div_by_recip(3.14, -1, -1, 0) > 0
     assert lib.check_sqrt_bound(20_000, 12345, 0) > 0
+
+
+def test_unorm8_from_reciprocal_is_exact(lib):
+    # the splat's unpackUnorm4x8 (point normals and radius byte): c / 255 for
+    # all 256 bytes; the reciprocal product alone is wrong for about half
+    assert lib.check_unorm8(0) == 0
+    assert lib.check_unorm8(1) > 100
