@@ -125,7 +125,13 @@ __device__ __forceinline__ Pixel load_pixel(const GatherArgs& a, int cx, int cy,
   const float d = gload(a.depth + idx);
   p.depth = d;
   const float n0 = gload(a.normal + idx * 3), n1 = gload(a.normal + idx * 3 + 1), n2 = gload(a.normal + idx * 3 + 2);
-  const float u = (float)sx / (float)a.width, v = (float)sy / (float)a.height;
+  // The texture coordinate and the perspective divide use reciprocals (the
+  // screen size's once per wave): the position moves by a few ulps against
+  // the divisions, far inside the gather's tolerance. The exact texel
+  // decision of the FP32 form rebuilds the position from (x, y, depth)
+  // itself.
+  const float rw = __builtin_amdgcn_rcpf((float)a.width), rh = __builtin_amdgcn_rcpf((float)a.height);
+  const float u = (float)sx * rw, v = (float)sy * rh;
   const float c0 = u * 2.0f - 1.0f, c1 = v * 2.0f - 1.0f, c2 = d * 2.0f - 1.0f;
   const float* m = a.vp_inv.m;
   float r[4];
@@ -133,9 +139,10 @@ __device__ __forceinline__ Pixel load_pixel(const GatherArgs& a, int cx, int cy,
   for (int row = 0; row < 4; ++row)
     r[row] = __builtin_fmaf(m[row], c0,
                             __builtin_fmaf(m[4 + row], c1, __builtin_fmaf(m[8 + row], c2, m[12 + row])));
-  p.fx = r[0] / r[3];
-  p.fy = r[1] / r[3];
-  p.fz = r[2] / r[3];
+  const float rw3 = __builtin_amdgcn_rcpf(r[3]);
+  p.fx = r[0] * rw3;
+  p.fy = r[1] * rw3;
+  p.fz = r[2] * rw3;
   p.nx = n0 * 2.0f - 1.0f;
   p.ny = n1 * 2.0f - 1.0f;
   p.nz = n2 * 2.0f - 1.0f;
