@@ -135,3 +135,30 @@ long check_rd_bound_sqrt(long n, uint64_t seed) {
   }
   return bad;
 }
+
+/* Pairs (s, t) where the splat's sign test s < -t differs from RN(s + t) < 0
+ * (the dot product's last addition): random floats of every magnitude, both
+ * signs, with exact cancellations (t = -s and neighbours of -s), signed
+ * zeros and subnormals mixed in. With naive = 1 the test is s < t instead,
+ * which the harness must catch. */
+long check_sign_of_sum(long n, uint64_t seed, int naive) {
+  long bad = 0;
+  for (long i = 0; i < n; ++i) {
+    const uint64_t r = splitmix(&seed);
+    float s = u2f((uint32_t)r), t;
+    switch ((r >> 32) & 7u) {
+      case 0: t = -s; break;                                    /* exact 0 */
+      case 1: t = u2f(f2u(-s) + 1u); break;                     /* one ulp off */
+      case 2: t = u2f(f2u(-s) - 1u); break;
+      case 3: t = u2f((uint32_t)(r >> 32) & 0x807FFFFFu); break; /* subnormal */
+      case 4: t = ((r >> 35) & 1u) ? 0.0f : -0.0f; break;
+      default: t = u2f((uint32_t)(splitmix(&seed))); break;
+    }
+    if (isnan(s) || isnan(t) || isinf(s) || isinf(t)) continue;
+    volatile float sum = s + t;
+    const int ref = sum < 0.0f;
+    const int got = naive ? (s < t) : (s < -t);
+    if (ref != got) ++bad;
+  }
+  return bad;
+}

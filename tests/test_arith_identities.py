@@ -9,7 +9,9 @@ cheaper forms that give the same bits:
     between R and the next float, with RD(m^2) = RD(R next(R)) formed in
     FP32 from the rounded product and its FMA error (no FP64 on the GPU).
 The splat's unpackUnorm4x8 (c / 255 for a byte c) uses the first form with
-divisor 255, checked for all 256 bytes. Both are checked here against IEEE
+divisor 255, checked for all 256 bytes, and its hemisphere tests decide
+RN(s + t) < 0 as s < -t, checked on random pairs with exact cancellations,
+signed zeros and subnormals. Both are checked here against IEEE
 float division and sqrtf: the division
 over every significand of two binades per divisor and sign (scaling by a
 power of two leaves all three roundings unchanged), the comparison on random
@@ -41,6 +43,8 @@ def lib(tmp_path_factory):
     L.check_rd_bound_sqrt.restype = C.c_long
     L.check_unorm8.argtypes = [C.c_int]
     L.check_unorm8.restype = C.c_long
+    L.check_sign_of_sum.argtypes = [C.c_long, C.c_uint64, C.c_int]
+    L.check_sign_of_sum.restype = C.c_long
     return L
 
 
@@ -77,3 +81,10 @@ def test_unorm8_from_reciprocal_is_exact(lib):
     # all 256 bytes; the reciprocal product alone is wrong for about half
     assert lib.check_unorm8(0) == 0
     assert lib.check_unorm8(1) > 100
+
+
+def test_sign_of_sum_without_the_addition(lib):
+    # the splat's hemisphere tests: dot < 0 as s < -t without the last
+    # addition (exact cancellations, signed zeros, subnormals included)
+    assert lib.check_sign_of_sum(2_000_000, 99, 0) == 0
+    assert lib.check_sign_of_sum(200_000, 99, 1) > 1000
