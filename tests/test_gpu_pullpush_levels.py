@@ -6,7 +6,8 @@ sides 8 and 16 and every level count from 1 (the level-0 push reads the pull
 directly) to log2(side). Texels are empty (0xFFFFFFFF) with a density that
 differs per ISM (from nearly full to entirely empty), otherwise a random
 24-bit depth and radius byte, so that pulls, occlusion tests and fills all
-take both branches.
+take both branches. One more case runs the largest atlas accepted, 16384^2,
+whose byte offsets reach 2^30.
 """
 import numpy as np
 import pytest
@@ -52,3 +53,32 @@ def test_pullpush_levels_bit_exact(ctx, ism_side, levels):
     ref = oracle.pullpush(atlas, ism_side, levels, Z_FAR)
     assert len(np.unique(ref)) > 16
     assert np.array_equal(got, ref), int((got != ref).sum())
+
+
+
+def test_pullpush_largest_atlas_bit_exact(ctx):
+    # the largest atlas the entry point accepts, 16384^2 (1 GiB of u32
+    # texels): the kernel addresses atlas and result with 32-bit byte offsets
+    # of buffers sized exactly to them, which reach 2^30 here.
+    # Sparse: every 37th row of ISMs and the last row carry samples.
+    side, ism_side, levels = 16384, 64, 4
+    print(f"[pullpush] S={ism_side} L={levels} atlas={side}", flush=True)
+    rng = np.random.default_rng(16384)
+    atlas = np.full((side, side), 0xFFFFFFFF, np.uint32)
+    rows = list(range(0, side // ism_side, 37)) + [side // ism_side - 1]
+    for r in rows:
+        band = atlas[r * ism_side:(r + 1) * ism_side]
+        depth = rng.integers(1 << 18, (1 << 24) - 1, band.shape, dtype=np.uint32)
+        radius = rng.integers(0, 256, band.shape, dtype=np.uint32)
+        band[...] = np.where(rng.random(band.shape) < 0.3, (depth << 8) | radius, np.uint32(0xFFFFFFFF))
+    ism = mlgi.ImperfectShadowmap(ctx, side, (side // ism_side) ** 2)
+    assert ism.ism_side == ism_side
+    ism.softrender.copy_from(atlas)
+    p = mlgi.default_params(atlas_side=side, z_far=Z_FAR, pull_push_levels=levels)
+    ism.pullpush(p)
+    ctx.sync()
+    got = ism.pushpull_result.numpy()
+    ref = oracle.pullpush(atlas, ism_side, levels, Z_FAR, threads=16)
+    assert np.array_equal(got, ref), int((got != ref).sum())
+    # the last row of ISMs (the largest offsets) holds filled samples
+    assert len(np.unique(ref[-ism_side:])) > 16
