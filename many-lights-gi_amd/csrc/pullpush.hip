@@ -28,7 +28,7 @@
 // Most of an atlas is empty (0xFFFFFFFF), and an empty texel decodes to a
 // fixed, valid far sample: the first pull level of an all-empty 2x2 block is
 // one constant E_1 (computed on the host by the same code and passed as an
-// argument), taken when a whole wave's blocks are empty.
+// argument), written for every such block (the two-pass level 1 below).
 //
 // Push runs over the reference's invocations: the (up to) four fine pixels
 // that share the same four coarse neighbours (push.comp:46-67), so loads,
