@@ -33,8 +33,10 @@
 // Push runs over the reference's invocations: the (up to) four fine pixels
 // that share the same four coarse neighbours (push.comp:46-67), so loads,
 // half unpacking and per-neighbour constants are formed once per quad. The
-// coarse-to-fine offset (coarse + 0.5) * 2 - (fine + 0.5) of each pair is one
-// of +-0.5, +-1.5, exact in the reference's global coordinates as well.
+// quads wrap around the ISM, so that the half-empty border invocations of
+// opposite sides share one quad and every quad holds four pixels (push_quad).
+// The coarse-to-fine offset (coarse + 0.5) * 2 - (fine + 0.5) of each pair is
+// one of +-0.5, +-1.5, exact in the reference's global coordinates as well.
 // The radius tests compare a correctly rounded square root with a float,
 // sqrt_rn(x) > R. Because sqrt(x) of a float x can never equal the midpoint m
 // between R and the next float (m has 25 significant bits, m^2 then has more
@@ -321,9 +323,35 @@ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y, 
   return pull_combine(ds, mds, rs, dx, dy, valid);
 }
 
+// Push quads wrap around the ISM. The reference's invocations g = 0 .. so/2
+// per axis (push.comp:46-67) pair the fine pixels 2g - 1, 2g with the coarse
+// neighbours g - 1, g, so the first and the last quad of a row hold one
+// in-ISM pixel each, whose other neighbour lies outside the ISM and has
+// weight 0 (push.comp:77-86). Here quads g = 0 .. so/2 - 1 take those
+// indices mod so (pixels) and mod sc = so/2 (neighbours): quad 0 holds pixel
+// so - 1 with neighbour sc - 1 and pixel 0 with neighbour 0, the two border
+// invocations in one, and each of its pixels may only use the neighbour on
+// its own side. Its pairs across the wrap, which the reference never forms,
+// get the offset kAcross instead of +-1.5: the squared distance is then +inf
+// and the radius test fails, so they are never valid and never kept, as the
+// outside neighbours they stand for. Every quad holds four in-ISM pixels, no
+// index is clamped or range-tested, and per 64^2 texels the push levels take
+// 16, 4 and 1 wave iterations instead of 18, 5 and 2.
+constexpr float kAcross = 1.0e20f;  // absorbs twice any half; its square overflows
+
+// The offset (coarse + 0.5) * 2 - (fine + 0.5) = 2 o - f - 0.5 along one axis
+// between the neighbour at o and the pixel at f of a quad (o, f in {0, 1}),
+// exact; kAcross for the pairs across the wrap of a quad 0 (`wrapped`).
+template <int o, int f>
+__device__ __forceinline__ float pair_offset(bool wrapped) {
+  constexpr float c = (float)(2 * o - f) - 0.5f;
+  if constexpr (o == f) return c;
+  else return wrapped ? kAcross : c;
+}
+
 // One push invocation (push.comp:44-165) of a level l >= 1: quad (gx, gy)
-// updates, in place, the fine pixels 2g - 1 + {0,1}^2 of level l that lie in
-// the ISM from the four coarse neighbours g - 1 + {0,1}^2 of level l + 1.
+// updates, in place, the fine pixels 2g - 1 + {0,1}^2 (mod so) of level l
+// from the four coarse neighbours g - 1 + {0,1}^2 (mod sc) of level l + 1.
 // The reference's weights become flags: w_i != 0 is `valid` (usable and
 // within the radius), the weight that survives the max-depth test is `keep`.
 // The reference sums all four w_i-weighted terms; a dropped term there is
@@ -341,33 +369,32 @@ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, T
   bool usable[4];
   uint32_t cdisp[4];
   float cd[4], cmd[4], cr[4], bound[4];
+  const bool wx = gx == 0, wy = gy == 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
-    // ISM border test: coarse neighbours outside this ISM get weight 0 (as
-    // unusable, their values are never used: they are read from the nearest
-    // texel inside)
-    const bool inside = cx >= 0 && cy >= 0 && cx < sc && cy < sc;
-    const Texel c = coarse[min(max(cy, 0), sc - 1) * sc + min(max(cx, 0), sc - 1)];
+    const int cx = (gx - 1 + ox[i]) & (sc - 1), cy = (gy - 1 + oy[i]) & (sc - 1);
+    const Texel c = coarse[cy * sc + cx];
     cd[i] = c.depth;
     cmd[i] = c.max_depth;
     cr[i] = c.radius;
     cdisp[i] = c.disp;
-    usable[i] = inside && c.depth != 1.0f;
+    usable[i] = c.depth != 1.0f;
     bound[i] = rd_bound(c.radius * scale);
   }
 #pragma unroll
   for (int op = 0; op < 4; ++op) {
-    const int px = 2 * gx - 1 + ox[op], py = 2 * gy - 1 + oy[op];
-    if (px < 0 || py < 0 || px >= so || py >= so) continue;
+    const int px = (2 * gx - 1 + ox[op]) & (so - 1), py = (2 * gy - 1 + oy[op]) & (so - 1);
     float ddx[4], ddy[4];
     bool valid[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      // (coarse + 0.5) * 2 - (fine + 0.5) = 2 o_i - o_op - 0.5, exact, plus
-      // twice the displacement
-      ddx[i] = plus_twice_half((float)(2 * ox[i] - ox[op]) - 0.5f, cdisp[i]);
-      ddy[i] = plus_twice_half((float)(2 * oy[i] - oy[op]) - 0.5f, cdisp[i] >> 16);
+      // the pair's offset plus twice the displacement
+      ddx[i] = plus_twice_half(ox[i] ? (ox[op] ? pair_offset<1, 1>(wx) : pair_offset<1, 0>(wx))
+                                     : (ox[op] ? pair_offset<0, 1>(wx) : pair_offset<0, 0>(wx)),
+                               cdisp[i]);
+      ddy[i] = plus_twice_half(oy[i] ? (oy[op] ? pair_offset<1, 1>(wy) : pair_offset<1, 0>(wy))
+                                     : (oy[op] ? pair_offset<0, 1>(wy) : pair_offset<0, 0>(wy)),
+                               cdisp[i] >> 16);
       const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
       valid[i] = usable[i] && !(x > bound[i]);  // sqrt(x) <= radius * 2^-level
     }
@@ -419,8 +446,8 @@ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, T
 }
 
 // Level-0 push (push.comp:44-165 at level 0, depth only): quad (gx, gy) writes
-// the unorm16 depth of the fine pixels 2g - 1 + {0,1}^2 from the coarse
-// neighbours g - 1 + {0,1}^2 of level 1, whose radius slots hold
+// the unorm16 depth of the fine pixels 2g - 1 + {0,1}^2 (mod S) from the
+// coarse neighbours g - 1 + {0,1}^2 (mod S/2) of level 1, whose radius slots hold
 // rd_bound(radius) (level 0 scales the radius by 2^0). The weights of
 // push_quad become per-neighbour flags: w_i != 0 is `valid`, the weight that
 // survives the max-depth test is `keep`, and a fill needs an occluding kept
@@ -437,29 +464,30 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
   constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
   bool usable[4];
   float cd[4], cd3[4], cd9[4], cmd[4], bnd[4], hx[2][4], hy[2][4];
+  const bool wx = gx == 0, wy = gy == 0;  // quads wrap as in push_quad
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int cx = gx - 1 + ox[i], cy = gy - 1 + oy[i];
-    const bool inside = cx >= 0 && cy >= 0 && cx < sc && cy < sc;
-    const Texel c = coarse[min(max(cy, 0), sc - 1) * sc + min(max(cx, 0), sc - 1)];  // as in push_quad
+    const int cx = (gx - 1 + ox[i]) & (sc - 1), cy = (gy - 1 + oy[i]) & (sc - 1);
+    const Texel c = coarse[cy * sc + cx];
     cd[i] = c.depth;
     cd3[i] = c.depth * 3.0f;
     cd9[i] = c.depth * 9.0f;
     cmd[i] = c.max_depth;
     bnd[i] = c.radius;
-    usable[i] = inside && c.depth != 1.0f;
-#pragma unroll
-    for (int f = 0; f < 2; ++f) {  // fine offset f along x / y
-      const float ex = plus_twice_half((float)(2 * ox[i] - f) - 0.5f, c.disp);
-      const float ey = plus_twice_half((float)(2 * oy[i] - f) - 0.5f, c.disp >> 16);
-      hx[f][i] = ex * ex;
-      hy[f][i] = ey * ey;
-    }
+    usable[i] = c.depth != 1.0f;
+    // squared offsets to the pixels at fine offset 0 and 1 along x / y
+    const float ex0 = plus_twice_half(ox[i] ? pair_offset<1, 0>(wx) : pair_offset<0, 0>(wx), c.disp);
+    const float ex1 = plus_twice_half(ox[i] ? pair_offset<1, 1>(wx) : pair_offset<0, 1>(wx), c.disp);
+    const float ey0 = plus_twice_half(oy[i] ? pair_offset<1, 0>(wy) : pair_offset<0, 0>(wy), c.disp >> 16);
+    const float ey1 = plus_twice_half(oy[i] ? pair_offset<1, 1>(wy) : pair_offset<0, 1>(wy), c.disp >> 16);
+    hx[0][i] = ex0 * ex0;
+    hx[1][i] = ex1 * ex1;
+    hy[0][i] = ey0 * ey0;
+    hy[1][i] = ey1 * ey1;
   }
 #pragma unroll
   for (int op = 0; op < 4; ++op) {
-    const int px = 2 * gx - 1 + ox[op], py = 2 * gy - 1 + oy[op];
-    if (px < 0 || py < 0 || px >= so || py >= so) continue;
+    const int px = (2 * gx - 1 + ox[op]) & (so - 1), py = (2 * gy - 1 + oy[op]) & (so - 1);
     bool valid[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) valid[i] = usable[i] && !(hx[ox[op]][i] + hy[oy[op]][i] > bnd[i]);
@@ -610,7 +638,7 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
 template <int S, int l, int L>
 __device__ __forceinline__ void push_levels(const Tables& tb) {
   if constexpr (l >= 0) {
-    constexpr int so = S >> l, sc = so / 2, nq1 = so / 2 + 1, nq = nq1 * nq1;
+    constexpr int so = S >> l, sc = so / 2, nq1 = so / 2, nq = nq1 * nq1;  // quads wrap (push_quad)
     const int total = group_count<S>(tb.groups) * nq;
     auto full = [&](int t) {
       const int g = t / nq, q = t - g * nq;
