@@ -272,11 +272,11 @@ __device__ __forceinline__ float plus_twice_half(float c, uint32_t bits) {
 }
 
 // Pull of output (ox, oy) of level l >= 1 (pull.comp:20-126). Level 1 reads
-// the atlas tile at texel `base` of the atlas (row stride `stride`) and
+// the atlas tile at texel `base` of the atlas (row stride 2^stride_log2) and
 // decodes it; higher levels read pull level l - 1.
 template <int S, int l>
 __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y, __amdgpu_buffer_rsrc_t atlas,
-                                          uint32_t base, int stride, const Texel* src, const float* mtab,
+                                          uint32_t base, int stride_log2, const Texel* src, const float* mtab,
                                           const Decoded& empty, float z_far) {
   constexpr int so = S >> l, si = so * 2;
   constexpr float scale = 1.0f / (float)(1 << l);  // pow(2, -level)
@@ -291,7 +291,7 @@ __device__ __forceinline__ Texel pull_one(int ox, int oy, int ism_x, int ism_y, 
     const int offx = i >> 1, offy = i & 1;
     const int ix = ox * 2 + offx, iy = oy * 2 + offy;
     if constexpr (l == 1) {
-      const uint32_t s = __builtin_amdgcn_raw_buffer_load_b32(atlas, (base + iy * stride + ix) * 4u, 0, 0);
+      const uint32_t s = __builtin_amdgcn_raw_buffer_load_b32(atlas, (base + ((uint32_t)iy << stride_log2) + ix) * 4u, 0, 0);
       const uint32_t b = s & 0xFFu;
       // r = b / 10 (div_by_recip is exact for the divisor 10); the max-depth
       // offset r * 2 / zFar * 0.6 is tabulated per radius byte
@@ -455,10 +455,10 @@ __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, T
 // terms in neighbour order are the sums of push_quad (its zero terms add +0).
 // The pixels' original samples come from the ISM's tile at texel `base` of
 // the atlas, the results go to the same texels of the result (both of row
-// stride `stride`).
+// stride 2^stride_log2; atlas sides are powers of two).
 template <int S>
 __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coarse, __amdgpu_buffer_rsrc_t atlas,
-                                                __amdgpu_buffer_rsrc_t out, uint32_t base, int stride,
+                                                __amdgpu_buffer_rsrc_t out, uint32_t base, int stride_log2,
                                                 const float* rcp) {
   constexpr int so = S, sc = S / 2;
   constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
@@ -500,7 +500,7 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
       minimum = take ? cd[i] : minimum;
       max_depth = take ? cmd[i] : max_depth;
     }
-    const uint32_t t = base + (uint32_t)(py * stride + px);
+    const uint32_t t = base + ((uint32_t)py << stride_log2) + (uint32_t)px;
     const float orig =
         (float)(__builtin_amdgcn_raw_buffer_load_b32(atlas, t * 4u, 0, 0) >> 8) * (1.0f / 16777216.0f);
     bool occluded = false;
@@ -559,7 +559,7 @@ constexpr bool kCompactPull1 = list_fits<S, L>(kGroupMax<S> * (S / 2) * (S / 2),
 
 // One workgroup's group of ISMs ism0 .. ism0 + groups - 1: member g's atlas
 // tile and result at origin(S, g) of the atlas and the result (row stride
-// atlas_side), its pull levels in lv (lv_offset).
+// 2^side_log2), its pull levels in lv (lv_offset).
 struct Tables {
   __amdgpu_buffer_rsrc_t atlas;  // the atlas (buffer_of)
   __amdgpu_buffer_rsrc_t out;    // the unorm16 result (buffer_of)
@@ -569,11 +569,11 @@ struct Tables {
   int* counts;  // list length of the two-pass pull level 1
   Decoded empty;
   float z_far;
-  int ism0, n1d_log2, groups, atlas_side;
+  int ism0, n1d_log2, groups, side_log2;
   // texel offset of group member g's tile in the atlas
   __device__ uint32_t origin(int S, int g) const {
     const int ism = ism0 + g;
-    return (uint32_t)((ism >> n1d_log2) * S * atlas_side + (ism & ((1 << n1d_log2) - 1)) * S);
+    return ((uint32_t)((ism >> n1d_log2) * S) << side_log2) + (uint32_t)((ism & ((1 << n1d_log2) - 1)) * S);
   }
 };
 
@@ -608,7 +608,7 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
       const int ism = tb.ism0 + g;
       const int ism_x = ism & ((1 << tb.n1d_log2) - 1), ism_y = ism >> tb.n1d_log2;
       tb.lv[lv_offset<S>(l) + t] =
-          pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.atlas, tb.origin(S, g), tb.atlas_side,
+          pull_one<S, l>(o % so, o / so, ism_x, ism_y, tb.atlas, tb.origin(S, g), tb.side_log2,
                          tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.mtab, tb.empty, tb.z_far);
     };
     if constexpr (l == 1 && kCompactPull1<S, L>) {
@@ -617,9 +617,9 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
       for (int t = threadIdx.x; t < total; t += kThreads) {
         const int g = t / n, o = t - g * n;
         const int ox = o % so, oy = o / so;
-        const uint32_t row = (tb.origin(S, g) + (uint32_t)((2 * oy) * tb.atlas_side + 2 * ox)) * 4u;
+        const uint32_t row = (tb.origin(S, g) + ((uint32_t)(2 * oy) << tb.side_log2) + (uint32_t)(2 * ox)) * 4u;
         const auto a = __builtin_amdgcn_raw_buffer_load_b64(tb.atlas, row, 0, 0);
-        const auto b = __builtin_amdgcn_raw_buffer_load_b64(tb.atlas, row + (uint32_t)tb.atlas_side * 4u, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(tb.atlas, row + (4u << tb.side_log2), 0, 0);
         const bool empty = (a[0] & a[1] & b[0] & b[1]) == 0xFFFFFFFFu;
         if (empty) tb.lv[lv_offset<S>(1) + t] = pc.e1;
         list_append(list, &tb.counts[0], !empty, t);
@@ -644,7 +644,7 @@ __device__ __forceinline__ void push_levels(const Tables& tb) {
       const int g = t / nq, q = t - g * nq;
       const Texel* coarse = tb.lv + lv_offset<S>(l + 1) + g * (sc * sc);
       if constexpr (l == 0)
-        push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.atlas, tb.out, tb.origin(S, g), tb.atlas_side, tb.rcp);
+        push_quad_depth<S>(q % nq1, q / nq1, coarse, tb.atlas, tb.out, tb.origin(S, g), tb.side_log2, tb.rcp);
       else
         push_quad<S, l>(q % nq1, q / nq1, coarse, tb.lv + lv_offset<S>(l) + g * (so * so), tb.rcp);
     };
@@ -682,7 +682,7 @@ __device__ __forceinline__ void pullpush_tile(TileLds<S, L>& sm, const uint32_t*
   __syncthreads();
   const uint32_t texels = (uint32_t)atlas_side * (uint32_t)atlas_side;
   Tables tb{buffer_of(atlas, texels * 4u), buffer_of(out, texels * 2u), sm.lv, sm.rcp, sm.mtab, sm.counts, {},
-            z_far, ism0, n1d_log2, groups, atlas_side};
+            z_far, ism0, n1d_log2, groups, 31 - __builtin_clz(atlas_side)};
   // decode of an empty texel (0xFFFFFFFF), shared by every empty texel
   const float r255 = (float)255 / 10.0f;
   tb.empty = decode_texel(0xFFFFFFFFu, r255, r255 * 2.0f / z_far * 0.6f, z_far, S);

@@ -82,3 +82,57 @@ def test_pullpush_largest_atlas_bit_exact(ctx):
     assert np.array_equal(got, ref), int((got != ref).sum())
     # the last row of ISMs (the largest offsets) holds filled samples
     assert len(np.unique(ref[-ism_side:])) > 16
+
+
+def border_atlas(ism_side, seed):
+    # 4 x 4 ISMs whose samples lie only along their borders: one edge, one
+    # corner, two opposite edges, none or all, with the radius byte at its
+    # maximum (fills reach as far as the reference allows) or random. A push
+    # quad pairs pixels on one side of an ISM with neighbours on the other
+    # side only if its border handling is wrong, and then these atlases fill
+    # texels the oracle leaves untouched.
+    rng = np.random.default_rng(seed)
+    s = ism_side
+    side = 4 * s
+    atlas = np.full((side, side), 0xFFFFFFFF, np.uint32)
+    masks = []
+    for k in range(16):
+        m = np.zeros((s, s), bool)
+        if k == 0: m[:, 0] = True
+        elif k == 1: m[:, -1] = True
+        elif k == 2: m[0, :] = True
+        elif k == 3: m[-1, :] = True
+        elif k == 4: m[0, 0] = True
+        elif k == 5: m[0, -1] = True
+        elif k == 6: m[-1, 0] = True
+        elif k == 7: m[-1, -1] = True
+        elif k == 8: m[:, [0, -1]] = True
+        elif k == 9: m[[0, -1], :] = True
+        elif k == 11: m[...] = True
+        elif k >= 12: m[:, :2] = m[:2, :] = True  # two-texel bands along two edges
+        masks.append(m)
+    for k, m in enumerate(masks):
+        y0, x0 = (k // 4) * s, (k % 4) * s
+        depth = rng.integers(1 << 18, (1 << 24) - 1, (s, s), dtype=np.uint32)
+        radius = np.full((s, s), 255, np.uint32) if k < 12 else rng.integers(0, 256, (s, s), dtype=np.uint32)
+        atlas[y0:y0 + s, x0:x0 + s] = np.where(m, (depth << 8) | radius, np.uint32(0xFFFFFFFF))
+    return atlas
+
+
+@pytest.mark.parametrize("ism_side,levels", [(s, l) for s in (8, 16, 32, 64)
+                                             for l in sorted({1, 2, int(np.log2(s))})])
+def test_pullpush_border_samples_bit_exact(ctx, ism_side, levels):
+    side = 4 * ism_side
+    print(f"[pullpush border] S={ism_side} L={levels}", flush=True)
+    atlas = border_atlas(ism_side, seed=ism_side + levels)
+    ism = mlgi.ImperfectShadowmap(ctx, side, N_VPL)
+    ism.softrender.copy_from(atlas)
+    p = mlgi.default_params(atlas_side=side, z_far=Z_FAR, pull_push_levels=levels)
+    ism.pullpush(p)
+    ctx.sync()
+    got = ism.pushpull_result.numpy()
+    ref = oracle.pullpush(atlas, ism_side, levels, Z_FAR)
+    # the border samples fill part of their ISMs (not all of them: the
+    # opposite border stays empty in the one-edge ISMs at one level)
+    assert (ref < 65535).sum() > (atlas != 0xFFFFFFFF).sum()
+    assert np.array_equal(got, ref), int((got != ref).sum())
