@@ -341,108 +341,151 @@ constexpr float kAcross = 1.0e20f;  // absorbs twice any half; its square overfl
 
 // The offset (coarse + 0.5) * 2 - (fine + 0.5) = 2 o - f - 0.5 along one axis
 // between the neighbour at o and the pixel at f of a quad (o, f in {0, 1}),
-// exact; kAcross for the pairs across the wrap of a quad 0 (`wrapped`).
-template <int o, int f>
-__device__ __forceinline__ float pair_offset(bool wrapped) {
-  constexpr float c = (float)(2 * o - f) - 0.5f;
-  if constexpr (o == f) return c;
-  else return wrapped ? kAcross : c;
+// exact: +-0.5 on the same side, +-1.5 across; kAcross for the pairs across
+// the wrap of a quad 0 (`wrapped`).
+__device__ __forceinline__ float pair_offset(int o, int f, bool wrapped) {
+  const float same = o ? 0.5f : -0.5f, across = wrapped ? kAcross : (o ? 1.5f : -1.5f);
+  return o == f ? same : across;
 }
 
-// One push invocation (push.comp:44-165) of a level l >= 1: quad (gx, gy)
-// updates, in place, the fine pixels 2g - 1 + {0,1}^2 (mod so) of level l
-// from the four coarse neighbours g - 1 + {0,1}^2 (mod sc) of level l + 1.
+// The reference's weightsX = {9, 3, 1, 3} of neighbour i for fine pixel op.
+__device__ __forceinline__ float push_weight(int i, int op) {
+  const int wt = (i - op + 4) & 3;
+  return wt == 0 ? 9.0f : (wt == 2 ? 1.0f : 3.0f);
+}
+
+// Offsets {0,0},{0,1},{1,1},{1,0} of a quad's neighbours and of its fine
+// pixels, in the reference's invocation order.
+constexpr int kOx[4] = {0, 0, 1, 1}, kOy[4] = {0, 1, 1, 0};
+
+// The four coarse neighbours of push quad (gx, gy) of a level l >= 1 (kept
+// as separate arrays: flags held in a struct are stored as bytes and cost
+// conversions once the struct is taken apart).
+struct Coarse4 {
+  float cd[4], cmd[4], cr[4], bound[4];
+  uint32_t cdisp[4];
+};
+
+template <int S, int l>
+__device__ __forceinline__ void load_coarse(int gx, int gy, const Texel* coarse, Coarse4& c, bool (&usable)[4]) {
+  constexpr int sc = (S >> l) / 2;
+  constexpr float scale = 1.0f / (float)(1 << l);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int cx = (gx - 1 + kOx[i]) & (sc - 1), cy = (gy - 1 + kOy[i]) & (sc - 1);
+    const Texel t = coarse[cy * sc + cx];
+    c.cd[i] = t.depth;
+    c.cmd[i] = t.max_depth;
+    c.cr[i] = t.radius;
+    c.cdisp[i] = t.disp;
+    usable[i] = t.depth != 1.0f;
+    c.bound[i] = rd_bound(t.radius * scale);
+  }
+}
+
+// One pixel of a push invocation (push.comp:44-165) of a level l >= 1: fine
+// pixel op (0..3, kOx / kOy) of a quad whose neighbours are c, texel t of
+// `same`, which holds its pull value and receives its push value in place.
+// wx / wy: the quad is quad 0 of its row / column (the wrap).
 // The reference's weights become flags: w_i != 0 is `valid` (usable and
 // within the radius), the weight that survives the max-depth test is `keep`.
 // The reference sums all four w_i-weighted terms; a dropped term there is
 // v * 0 = +-0, and a fill always has a kept term, which is nonzero or +0
 // (a displacement c + d with c != 0 that cancels rounds to +0), so summing
 // the kept terms over +0 gives the same bits.
+template <int l>
+__device__ __forceinline__ void push_pixel(const Coarse4& c, const bool (&usable)[4], int op, bool wx, bool wy,
+                                           const float* rcp, Texel* same, int t) {
+  const int fx = op >> 1, fy = (op ^ (op >> 1)) & 1;  // kOx[op], kOy[op]
+  const float* cd = c.cd;
+  const float* cmd = c.cmd;
+  const float* cr = c.cr;
+  float ddx[4], ddy[4];
+  bool valid[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    // the pair's offset plus twice the displacement
+    ddx[i] = plus_twice_half(pair_offset(kOx[i], fx, wx), c.cdisp[i]);
+    ddy[i] = plus_twice_half(pair_offset(kOy[i], fy, wy), c.cdisp[i] >> 16);
+    const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
+    valid[i] = usable[i] && !(x > c.bound[i]);  // sqrt(x) <= radius * 2^-level
+  }
+  // the reference starts the minimum at 9001; every depth is at most
+  // 1 + 2^-22, so the first valid neighbour is always taken
+  float minimum = 9001.0f, max_depth = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {  // fminf(cd, minimum) == cd is cd <= minimum
+    const bool take = valid[i] && (i == 0 || !(cd[i] > minimum));
+    minimum = take ? cd[i] : minimum;
+    max_depth = take ? cmd[i] : max_depth;
+  }
+  Texel res = same[t];
+  bool keep[4], any = false, occluded = false;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {  // bitwise, so that no flag becomes a branch
+    keep[i] = valid[i] & !(cd[i] > max_depth);
+    any |= keep[i];
+    occluded |= keep[i] & (res.depth > cmd[i]);
+  }
+  if (any && (res.depth == 1.0f || occluded)) {
+    // kept terms as fma(k, term, acc), k = 1 or 0 (pull_combine); each
+    // weighted term w v is rounded first, as the reference's products are
+    float wacc = 0.0f, dacc = 0.0f, macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float w = push_weight(i, op);
+      const float k = keep[i] ? 1.0f : 0.0f;
+      wacc = __builtin_fmaf(k, w, wacc);
+      dacc = __builtin_fmaf(k, cd[i] * w, dacc);
+      macc = __builtin_fmaf(k, cmd[i] * w, macc);
+      racc = __builtin_fmaf(k, cr[i] * w, racc);
+      xacc = __builtin_fmaf(k, ddx[i] * w, xacc);
+      yacc = __builtin_fmaf(k, ddy[i] * w, yacc);
+    }
+    const float y = rcp[(int)wacc];
+    res.depth = div_by_recip(dacc, wacc, y);
+    res.max_depth = div_by_recip(macc, wacc, y);
+    res.radius = div_by_recip(racc, wacc, y);
+    res.disp = pack_half2_hd(div_by_recip(xacc, wacc, y), div_by_recip(yacc, wacc, y));
+  }
+  // level 1 is read once more, by the depth-only level-0 push, which needs
+  // its radius only as the bound of the distance test
+  if constexpr (l == 1) res.radius = rd_bound(res.radius);
+  same[t] = res;
+}
+
+// Push quad (gx, gy) of a level l >= 1: updates, in place, the fine pixels
+// 2g - 1 + {0,1}^2 (mod so) of level l from the four coarse neighbours
+// g - 1 + {0,1}^2 (mod sc) of level l + 1.
 template <int S, int l>
 __device__ __forceinline__ void push_quad(int gx, int gy, const Texel* coarse, Texel* same,
                                           const float* rcp) {
   static_assert(l >= 1, "level 0 is push_quad_depth");
-  constexpr int so = S >> l, sc = so / 2;
-  constexpr float scale = 1.0f / (float)(1 << l);
-  // offsets {0,0},{0,1},{1,1},{1,0} for both neighbours and fine pixels
-  constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
-  bool usable[4];
-  uint32_t cdisp[4];
-  float cd[4], cmd[4], cr[4], bound[4];
+  constexpr int so = S >> l;
   const bool wx = gx == 0, wy = gy == 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int cx = (gx - 1 + ox[i]) & (sc - 1), cy = (gy - 1 + oy[i]) & (sc - 1);
-    const Texel c = coarse[cy * sc + cx];
-    cd[i] = c.depth;
-    cmd[i] = c.max_depth;
-    cr[i] = c.radius;
-    cdisp[i] = c.disp;
-    usable[i] = c.depth != 1.0f;
-    bound[i] = rd_bound(c.radius * scale);
-  }
+  Coarse4 c;
+  bool usable[4];
+  load_coarse<S, l>(gx, gy, coarse, c, usable);
 #pragma unroll
   for (int op = 0; op < 4; ++op) {
-    const int px = (2 * gx - 1 + ox[op]) & (so - 1), py = (2 * gy - 1 + oy[op]) & (so - 1);
-    float ddx[4], ddy[4];
-    bool valid[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      // the pair's offset plus twice the displacement
-      ddx[i] = plus_twice_half(ox[i] ? (ox[op] ? pair_offset<1, 1>(wx) : pair_offset<1, 0>(wx))
-                                     : (ox[op] ? pair_offset<0, 1>(wx) : pair_offset<0, 0>(wx)),
-                               cdisp[i]);
-      ddy[i] = plus_twice_half(oy[i] ? (oy[op] ? pair_offset<1, 1>(wy) : pair_offset<1, 0>(wy))
-                                     : (oy[op] ? pair_offset<0, 1>(wy) : pair_offset<0, 0>(wy)),
-                               cdisp[i] >> 16);
-      const float x = ddx[i] * ddx[i] + ddy[i] * ddy[i];
-      valid[i] = usable[i] && !(x > bound[i]);  // sqrt(x) <= radius * 2^-level
-    }
-    // the reference starts the minimum at 9001; every depth is at most
-    // 1 + 2^-22, so the first valid neighbour is always taken
-    float minimum = 9001.0f, max_depth = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {  // fminf(cd, minimum) == cd is cd <= minimum
-      const bool take = valid[i] && (i == 0 || !(cd[i] > minimum));
-      minimum = take ? cd[i] : minimum;
-      max_depth = take ? cmd[i] : max_depth;
-    }
-    const int t = py * so + px;
-    Texel res = same[t];
-    bool keep[4], any = false, occluded = false;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      keep[i] = valid[i] && !(cd[i] > max_depth);
-      any = any || keep[i];
-      occluded = occluded || (keep[i] && res.depth > cmd[i]);
-    }
-    if (any && (res.depth == 1.0f || occluded)) {
-      // kept terms as fma(k, term, acc), k = 1 or 0 (pull_combine); each
-      // weighted term w v is rounded first, as the reference's products are
-      float wacc = 0.0f, dacc = 0.0f, macc = 0.0f, racc = 0.0f, xacc = 0.0f, yacc = 0.0f;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int wt = (i - op + 4) & 3;  // weightsX = {9, 3, 1, 3}
-        const float w = (wt == 0) ? 9.0f : (wt == 2 ? 1.0f : 3.0f);
-        const float k = keep[i] ? 1.0f : 0.0f;
-        wacc = __builtin_fmaf(k, w, wacc);
-        dacc = __builtin_fmaf(k, cd[i] * w, dacc);
-        macc = __builtin_fmaf(k, cmd[i] * w, macc);
-        racc = __builtin_fmaf(k, cr[i] * w, racc);
-        xacc = __builtin_fmaf(k, ddx[i] * w, xacc);
-        yacc = __builtin_fmaf(k, ddy[i] * w, yacc);
-      }
-      const float y = rcp[(int)wacc];
-      res.depth = div_by_recip(dacc, wacc, y);
-      res.max_depth = div_by_recip(macc, wacc, y);
-      res.radius = div_by_recip(racc, wacc, y);
-      res.disp = pack_half2_hd(div_by_recip(xacc, wacc, y), div_by_recip(yacc, wacc, y));
-    }
-    // level 1 is read once more, by the depth-only level-0 push, which needs
-    // its radius only as the bound of the distance test
-    if constexpr (l == 1) res.radius = rd_bound(res.radius);
-    same[t] = res;
+    const int px = (2 * gx - 1 + kOx[op]) & (so - 1), py = (2 * gy - 1 + kOy[op]) & (so - 1);
+    push_pixel<l>(c, usable, op, wx, wy, rcp, same, py * so + px);
   }
+}
+
+// Pixel op of push quad (gx, gy), for levels whose quads take four lanes
+// each: every lane loads the quad's neighbours and pushes one pixel.
+template <int S, int l>
+__device__ __forceinline__ void push_quad_pixel(int gx, int gy, int op, const Texel* coarse, Texel* same,
+                                                const float* rcp) {
+  static_assert(l >= 1, "level 0 is push_quad_depth");
+  constexpr int so = S >> l;
+  const bool wx = gx == 0, wy = gy == 0;
+  Coarse4 c;
+  bool usable[4];
+  load_coarse<S, l>(gx, gy, coarse, c, usable);
+  const int px = (2 * gx - 1 + (op >> 1)) & (so - 1), py = (2 * gy - 1 + ((op ^ (op >> 1)) & 1)) & (so - 1);
+  push_pixel<l>(c, usable, op, wx, wy, rcp, same, py * so + px);
 }
 
 // Level-0 push (push.comp:44-165 at level 0, depth only): quad (gx, gy) writes
@@ -461,13 +504,12 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
                                                 __amdgpu_buffer_rsrc_t out, uint32_t base, int stride_log2,
                                                 const float* rcp) {
   constexpr int so = S, sc = S / 2;
-  constexpr int ox[4] = {0, 0, 1, 1}, oy[4] = {0, 1, 1, 0};
   bool usable[4];
   float cd[4], cd3[4], cd9[4], cmd[4], bnd[4], hx[2][4], hy[2][4];
   const bool wx = gx == 0, wy = gy == 0;  // quads wrap as in push_quad
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int cx = (gx - 1 + ox[i]) & (sc - 1), cy = (gy - 1 + oy[i]) & (sc - 1);
+    const int cx = (gx - 1 + kOx[i]) & (sc - 1), cy = (gy - 1 + kOy[i]) & (sc - 1);
     const Texel c = coarse[cy * sc + cx];
     cd[i] = c.depth;
     cd3[i] = c.depth * 3.0f;
@@ -476,10 +518,10 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
     bnd[i] = c.radius;
     usable[i] = c.depth != 1.0f;
     // squared offsets to the pixels at fine offset 0 and 1 along x / y
-    const float ex0 = plus_twice_half(ox[i] ? pair_offset<1, 0>(wx) : pair_offset<0, 0>(wx), c.disp);
-    const float ex1 = plus_twice_half(ox[i] ? pair_offset<1, 1>(wx) : pair_offset<0, 1>(wx), c.disp);
-    const float ey0 = plus_twice_half(oy[i] ? pair_offset<1, 0>(wy) : pair_offset<0, 0>(wy), c.disp >> 16);
-    const float ey1 = plus_twice_half(oy[i] ? pair_offset<1, 1>(wy) : pair_offset<0, 1>(wy), c.disp >> 16);
+    const float ex0 = plus_twice_half(pair_offset(kOx[i], 0, wx), c.disp);
+    const float ex1 = plus_twice_half(pair_offset(kOx[i], 1, wx), c.disp);
+    const float ey0 = plus_twice_half(pair_offset(kOy[i], 0, wy), c.disp >> 16);
+    const float ey1 = plus_twice_half(pair_offset(kOy[i], 1, wy), c.disp >> 16);
     hx[0][i] = ex0 * ex0;
     hx[1][i] = ex1 * ex1;
     hy[0][i] = ey0 * ey0;
@@ -487,10 +529,10 @@ __device__ __forceinline__ void push_quad_depth(int gx, int gy, const Texel* coa
   }
 #pragma unroll
   for (int op = 0; op < 4; ++op) {
-    const int px = (2 * gx - 1 + ox[op]) & (so - 1), py = (2 * gy - 1 + oy[op]) & (so - 1);
+    const int px = (2 * gx - 1 + kOx[op]) & (so - 1), py = (2 * gy - 1 + kOy[op]) & (so - 1);
     bool valid[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) valid[i] = usable[i] && !(hx[ox[op]][i] + hy[oy[op]][i] > bnd[i]);
+    for (int i = 0; i < 4; ++i) valid[i] = usable[i] && !(hx[kOx[op]][i] + hy[kOy[op]][i] > bnd[i]);
     // the reference starts the minimum at 9001; every depth is at most
     // 1 + 2^-22, so the first valid neighbour is always taken
     float minimum = 9001.0f, max_depth = 0.0f;
@@ -648,7 +690,18 @@ __device__ __forceinline__ void push_levels(const Tables& tb) {
       else
         push_quad<S, l>(q % nq1, q / nq1, coarse, tb.lv + lv_offset<S>(l) + g * (so * so), tb.rcp);
     };
-    for (int t = threadIdx.x; t < total; t += kThreads) full(t);
+    if constexpr (l >= 1 && kGroupMax<S> * nq * 4 <= 64) {
+      // the level's quads fill at most one wave with four lanes each: one
+      // pixel per lane (fewer instructions than a wave with a quad per lane)
+      const int t = threadIdx.x >> 2, op = threadIdx.x & 3;
+      if (t < total) {
+        const int g = t / nq, q = t - g * nq;
+        push_quad_pixel<S, l>(q % nq1, q / nq1, op, tb.lv + lv_offset<S>(l + 1) + g * (sc * sc),
+                              tb.lv + lv_offset<S>(l) + g * (so * so), tb.rcp);
+      }
+    } else {
+      for (int t = threadIdx.x; t < total; t += kThreads) full(t);
+    }
     __syncthreads();
     push_levels<S, l - 1, L>(tb);
   }
