@@ -110,20 +110,23 @@ __device__ __forceinline__ uint32_t keep_mask(const Candidate* cand, float4 rec)
   const float uy = dev::unorm8((bits >> 8) & 0xFFu);
   const float uz = dev::unorm8((bits >> 16) & 0xFFu);
   const f3 pn3 = dev::make(ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, uz * 2.0f - 1.0f);
-  uint32_t keep = 0;
-  int found = 0;
+  // bit i of `pass`: candidate i passes both tests, shifted in from the
+  // last candidate down (one add with carry per candidate)
+  uint32_t pass = 0;
 #pragma unroll
-  for (int i = 0; i < kMaxTest; ++i) {
+  for (int i = kMaxTest - 1; i >= 0; --i) {
     const Candidate& c = cand[i];
     const f3 rel = dev::sub(pos, dev::make(c.px, c.py, c.pz));
     const f3 nrel = dev::make(-rel.x, -rel.y, -rel.z);
     const bool cull = dot_below_zero(dev::make(c.nx, c.ny, c.nz), rel) || dot_below_zero(pn3, nrel);
-    if (!cull && found < kMaxCollect) {
-      keep |= 1u << i;
-      ++found;
-    }
+    pass = pass + pass + (cull ? 0u : 1u);
   }
-  return keep;
+  // the first kMaxCollect of them: clear the lowest set bit kMaxCollect
+  // times, and keep what was cleared
+  uint32_t rest = pass;
+#pragma unroll
+  for (int k = 0; k < kMaxCollect; ++k) rest &= rest - 1u;
+  return pass ^ rest;
 }
 
 // Projection of a point onto candidate c (paraboloid_project with

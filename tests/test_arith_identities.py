@@ -11,7 +11,10 @@ cheaper forms that give the same bits:
 The splat's unpackUnorm4x8 (c / 255 for a byte c) uses the first form with
 divisor 255, checked for all 256 bytes, and its hemisphere tests decide
 RN(s + t) < 0 as s < -t, checked on random pairs with exact cancellations,
-signed zeros and subnormals. Both are checked here against IEEE
+signed zeros and subnormals; its keep mask (the first four of the 16
+candidates that pass, ism.comp:80-103) is built from the mask of all passing
+candidates by clearing the lowest set bit four times, checked for all 2^16
+masks. The float rewrites are checked here against IEEE
 float division and sqrtf: the division
 over every significand of two binades per divisor and sign (scaling by a
 power of two leaves all three roundings unchanged), the comparison on random
@@ -88,3 +91,38 @@ def test_sign_of_sum_without_the_addition(lib):
     # addition (exact cancellations, signed zeros, subnormals included)
     assert lib.check_sign_of_sum(2_000_000, 99, 0) == 0
     assert lib.check_sign_of_sum(200_000, 99, 1) > 1000
+
+
+def first_collect_counted(passing, n_test=16, n_collect=4):
+    # ism.comp:80-103 as written: candidates in order, the first n_collect
+    # that pass are kept
+    keep, found = 0, 0
+    for i in range(n_test):
+        if (passing >> i) & 1 and found < n_collect:
+            keep |= 1 << i
+            found += 1
+    return keep
+
+
+def first_collect_bits(passing, n_collect=4):
+    # the splat kernel's form: clear the lowest set bit n_collect times and
+    # keep what was cleared
+    rest = passing
+    for _ in range(n_collect):
+        rest &= rest - 1
+    return passing ^ rest
+
+
+def test_first_four_passing_candidates_by_bit_clearing():
+    # the splat's keep mask: every 16-bit set of passing candidates
+    masks = np.arange(1 << 16, dtype=np.uint32)
+    rest = masks.copy()
+    for _ in range(4):
+        rest &= rest - np.uint32(1)
+    got = masks ^ rest
+    want = np.array([first_collect_counted(int(m)) for m in masks], dtype=np.uint32)
+    assert np.array_equal(got, want)
+    # set bits 0, 9, 10, 12, 13, 15: the first four are 0, 9, 10, 12
+    assert first_collect_bits(0b1011_0110_0000_0001) == 0b0001_0110_0000_0001
+    # a planted mistake (three clears) is caught
+    assert any(first_collect_bits(int(m), 3) != int(w) for m, w in zip(masks[:64], want[:64]))
