@@ -626,18 +626,24 @@ __device__ __forceinline__ int group_count(int groups) {
   return kGroupMax<S> == 1 ? 1 : groups;
 }
 
-// Appends v to an LDS list for the lanes with `take` set. Called by every
-// active lane of a wave together; in the strided loops below a wave's lane 0
-// holds its lowest index, so it is active whenever any lane is.
-__device__ __forceinline__ void list_append(uint16_t* list, int* count, bool take, int v) {
-  const unsigned long long m = __ballot(take);
-  if (m == 0ull) return;
+// Lanes of a wave below this one with their bit of m set.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Appends v0 and v1 to an LDS list for the lanes with take0 / take1 set (the
+// wave's v0 entries first), with one atomic per wave. Called by every active
+// lane of a wave together; in the strided loop below a wave's lane 0 holds
+// its lowest index, so it is active whenever any lane is.
+__device__ __forceinline__ void list_append2(uint16_t* list, int* count, bool take0, int v0, bool take1, int v1) {
+  const unsigned long long m0 = __ballot(take0), m1 = __ballot(take1);
+  if ((m0 | m1) == 0ull) return;
+  const int n0 = __popcll(m0);
   int first = 0;
-  if ((threadIdx.x & 63) == 0) first = atomicAdd(count, __popcll(m));
+  if ((threadIdx.x & 63) == 0) first = atomicAdd(count, n0 + __popcll(m1));
   first = __shfl(first, 0, 64);
-  const uint32_t below =
-      __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-  if (take) list[first + (int)below] = (uint16_t)v;
+  if (take0) list[first + lanes_below(m0)] = (uint16_t)v0;
+  if (take1) list[first + n0 + lanes_below(m1)] = (uint16_t)v1;
 }
 
 template <int S, int l, int L>
@@ -654,17 +660,22 @@ __device__ __forceinline__ void pull_levels(const Tables& tb, const PullConsts& 
                          tb.lv + lv_offset<S>(l - 1) + g * (4 * n), tb.mtab, tb.empty, tb.z_far);
     };
     if constexpr (l == 1 && kCompactPull1<S, L>) {
-      // pass 1: all-empty 2x2 blocks are E_1, the others go to the list
+      // pass 1: all-empty 2x2 blocks are E_1, the others go to the list.
+      // A lane takes two neighbouring blocks of a row (so is even), whose
+      // two texel rows it reads with 16-byte loads.
       uint16_t* list = reinterpret_cast<uint16_t*>(tb.lv + lv_offset<S>(2));
-      for (int t = threadIdx.x; t < total; t += kThreads) {
+      for (int t2 = threadIdx.x; t2 < total / 2; t2 += kThreads) {
+        const int t = 2 * t2;  // blocks t and t + 1
         const int g = t / n, o = t - g * n;
         const int ox = o % so, oy = o / so;
         const uint32_t row = (tb.origin(S, g) + ((uint32_t)(2 * oy) << tb.side_log2) + (uint32_t)(2 * ox)) * 4u;
-        const auto a = __builtin_amdgcn_raw_buffer_load_b64(tb.atlas, row, 0, 0);
-        const auto b = __builtin_amdgcn_raw_buffer_load_b64(tb.atlas, row + (4u << tb.side_log2), 0, 0);
-        const bool empty = (a[0] & a[1] & b[0] & b[1]) == 0xFFFFFFFFu;
-        if (empty) tb.lv[lv_offset<S>(1) + t] = pc.e1;
-        list_append(list, &tb.counts[0], !empty, t);
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(tb.atlas, row, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(tb.atlas, row + (4u << tb.side_log2), 0, 0);
+        const bool empty0 = (a[0] & a[1] & b[0] & b[1]) == 0xFFFFFFFFu;
+        const bool empty1 = (a[2] & a[3] & b[2] & b[3]) == 0xFFFFFFFFu;
+        if (empty0) tb.lv[lv_offset<S>(1) + t] = pc.e1;
+        if (empty1) tb.lv[lv_offset<S>(1) + t + 1] = pc.e1;
+        list_append2(list, &tb.counts[0], !empty0, t, !empty1, t + 1);
       }
       __syncthreads();
       const int count = tb.counts[0];
