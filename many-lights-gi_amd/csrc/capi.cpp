@@ -371,6 +371,28 @@ int mlgi_probe_valu(mlgi_context* c, double* fma_per_s) {
   return MLGI_OK;
 }
 
+int mlgi_check_division(mlgi_context* c, uint32_t seed, uint64_t* reciprocal_mismatches,
+                        uint64_t* quotient_mismatches, uint64_t* quotients_tested) {
+  if (!c) return MLGI_ERR_INVALID_ARGUMENT;
+  if (!reciprocal_mismatches || !quotient_mismatches || !quotients_tested)
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "an output of mlgi_check_division is NULL");
+  uint32_t* counts = nullptr;
+  uint32_t host[2] = {0u, 0u};
+  int rc = hip_check(c, hipMalloc(&counts, sizeof(host)), "hipMalloc(division check)");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(counts, host, sizeof(host), hipMemcpyHostToDevice, c->stream),
+                          "hipMemcpyAsync(HtoD)");
+  if (!rc) rc = hip_check(c, mlgi::launch_division_check(c->stream, counts, seed), "division_check");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(host, counts, sizeof(host), hipMemcpyDeviceToHost, c->stream),
+                          "hipMemcpyAsync(DtoH)");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (counts) (void)hipFree(counts);
+  if (rc) return rc;
+  *reciprocal_mismatches = host[0];
+  *quotient_mismatches = host[1];
+  *quotients_tested = mlgi::division_check_quotients();
+  return MLGI_OK;
+}
+
 int mlgi_shuffle_indices(int32_t n, uint32_t seed, int32_t* out) {
   if (n < 0 || (!out && n)) return MLGI_ERR_INVALID_ARGUMENT;
   std::vector<int> v(n);

@@ -142,13 +142,20 @@ __device__ __forceinline__ bool project(const Candidate& c, f3 pos, uint32_t rad
   float vy = dev::dot(dev::make(c.ux, c.uy, c.uz), rel);
   float vz = dev::dot(dev::make(-c.nx, -c.ny, -c.nz), rel);
   const float sign_v = vz > 0.0f ? 1.0f : (vz < 0.0f ? -1.0f : 0.0f);
-  vx = vx / dist;
-  vy = vy / dist;
-  vz = vz / dist;
+  // the six IEEE divisions of the reference as quotients from the
+  // correctly rounded reciprocals of their three divisors (dev::div_rn: the
+  // same bits; a zero divisor gives NaN where the division gives +-inf or
+  // NaN, and both make the texel test below fail; a zero quotient's sign is
+  // lost in the additions of 1 that follow each quotient)
+  const float rdist = dev::recip_rn(dist);
+  vx = dev::div_rn(vx, dist, rdist);
+  vy = dev::div_rn(vy, dist, rdist);
+  vz = dev::div_rn(vz, dist, rdist);
   vz = 1.0f - vz;
-  vx = vx / vz;
-  vy = vy / vz;
-  vz = dist / z_far;
+  const float rvz = dev::recip_rn(vz);
+  vx = dev::div_rn(vx, vz, rvz);
+  vy = dev::div_rn(vy, vz, rvz);
+  vz = dev::div_rn(dist, z_far, dev::recip_rn(z_far));
   vz = vz * -sign_v;
   vx = (vx + 1.0f) * 0.5f;
   vy = (vy + 1.0f) * 0.5f;

@@ -122,6 +122,9 @@ hipError_t launch_composite(hipStream_t s, const CompositeArgs& a);
 // diag.hip: FP32 FMA rate probe (iters must be a multiple of 4)
 uint64_t valu_probe_fmas(int blocks, int iters);
 hipError_t launch_valu_probe(hipStream_t s, float* out, int blocks, int iters);
+// counts[0..1]: reciprocal and quotient mismatches (zeroed by the caller)
+uint64_t division_check_quotients();
+hipError_t launch_division_check(hipStream_t s, uint32_t* counts, uint32_t seed);
 
 // raycast.hip: G-buffer (rsm = 0: a = depth, b = normal), RSM (rsm = 1:
 // a = diffuse, b = normal, c = depth) or surface (rsm = 2: a = albedo,

@@ -36,6 +36,27 @@ __device__ __forceinline__ float unorm8(uint32_t c) {
   return __builtin_fmaf(__builtin_fmaf(-q, 255.0f, x), y, q);
 }
 
+// RN(1 / b) for a normal float b: the hardware reciprocal (within one ulp)
+// refined by one Newton step, y + y (1 - b y) in two FMAs. Equal to the IEEE
+// quotient 1 / b for every significand (mlgi_check_division runs that check
+// on the device: all 2^23 significands at five exponents from 2^-60 to
+// 2^60). A zero, infinite or NaN b gives NaN.
+__device__ __forceinline__ float recip_rn(float b) {
+  const float y = __builtin_amdgcn_rcpf(b);
+  return __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+}
+
+// RN(a / b) from y = RN(1 / b): q = RN(a y) lies within an ulp of a / b, the
+// remainder a - q b is exact under an FMA, and RN(q + r y) is the correctly
+// rounded quotient (Markstein's theorem), for quotients and remainders
+// clear of overflow and underflow. Three operations; the reciprocal is
+// shared by every quotient of one divisor. An exact zero quotient may
+// carry the other sign than the IEEE one.
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+  const float q = a * y;
+  return __builtin_fmaf(__builtin_fmaf(-q, b, a), y, q);
+}
+
 // GLSL uint(float): truncation, negative values saturate to 0.
 __device__ __forceinline__ uint32_t to_uint(float v) {
   if (!(v > 0.0f)) return 0u;

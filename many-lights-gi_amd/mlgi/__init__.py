@@ -153,6 +153,15 @@ class Context:
         self.check(lib.mlgi_probe_valu(self.h, C.byref(v)), "mlgi_probe_valu")
         return float(v.value)
 
+    def check_division(self, seed: int = 0) -> tuple:
+        """(reciprocal mismatches, quotient mismatches, quotients tested) of
+        the splat's division form against IEEE division on this device
+        (mlgi_check_division); both counts are 0 where the atlas is exact."""
+        r, q, n = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        self.check(lib.mlgi_check_division(self.h, seed & 0xFFFFFFFF, C.byref(r), C.byref(q), C.byref(n)),
+                   "mlgi_check_division")
+        return int(r.value), int(q.value), int(n.value)
+
     # -- profiling (PerfCounter analogue)
     def profile(self, on: bool = True) -> None:
         self.check(lib.mlgi_profile_enable(self.h, 1 if on else 0))

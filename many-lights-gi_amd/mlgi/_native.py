@@ -198,6 +198,7 @@ SIGNATURES = {
     "mlgi_splat_spill_stats": (C.c_int, [vp, P(C.c_uint32), P(C.c_int32)]),
     "mlgi_splat_count": (C.c_int, [vp, P(C.c_uint64)]),
     "mlgi_probe_valu": (C.c_int, [vp, P(C.c_double)]),
+    "mlgi_check_division": (C.c_int, [vp, C.c_uint32, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]),
     "mlgi_deferred_composite": (C.c_int, [vp, P(CompositeInputs), P(CompositeParams), P(Image)]),
     "mlgi_ism_pullpush": (C.c_int, [vp, P(Atlas), P(Params), P(IsmDepth)]),
     "mlgi_ism_process": (C.c_int, [vp, P(Points), P(Vpls), P(Params), P(Atlas), P(IsmDepth)]),

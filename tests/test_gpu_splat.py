@@ -187,3 +187,15 @@ def test_forced_drain_of_a_long_sparse_bin(ctx):
         ism.render(mlgi.points_desc(ctx.upload(pts), ctx.upload(off)), vp, p)
         ctx.sync()
         assert np.array_equal(ism.softrender.numpy(), ref), blk
+
+
+def test_division_form_matches_ieee(ctx):
+    # the projection's six divisions are quotients from refined reciprocals
+    # of their three divisors (recip_rn, div_rn in mlgi_device.hpp); the
+    # device check compares both with IEEE division: every float significand
+    # at five exponents, four numerators per divisor, two numerator seeds
+    for seed in (0, 12345):
+        print(f"[division check] seed={seed}", flush=True)
+        rec, quo, n = ctx.check_division(seed)
+        assert n == (1 << 23) * 5 * 4
+        assert rec == 0 and quo == 0, (rec, quo)
