@@ -190,7 +190,15 @@ __host__ __device__ inline Decoded decode_texel(uint32_t s, float r, float madd,
   d.depth = (float)(s >> 8) / 16777216.0f;
   d.max_depth = d.depth + madd;
   const float dist = d.depth * z_far;
-  float radius = div_by_recip(r / dist, 3.14f, kRcp314) * (float)S;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // r / dist from the correctly rounded reciprocal of dist (mlgi_device.hpp;
+  // the IEEE quotient's bits). A zero dist gives NaN where the division gives
+  // +inf or NaN; both end as the radius 15 of the clamp below.
+  const float q = dev::div_rn(r, dist, dev::recip_rn(dist));
+#else
+  const float q = r / dist;
+#endif
+  float radius = div_by_recip(q, 3.14f, kRcp314) * (float)S;
   radius = radius * 1.3f;
   d.radius = fminf(radius, 15.0f);
   return d;
