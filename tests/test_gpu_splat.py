@@ -199,3 +199,35 @@ def test_division_form_matches_ieee(ctx):
         rec, quo, n = ctx.check_division(seed)
         assert n == (1 << 23) * 5 * 4
         assert rec == 0 and quo == 0, (rec, quo)
+
+
+def test_points_at_the_vpl_position(ctx):
+    # points exactly at the VPLs' position: the projection divides 0 by a
+    # distance of 0 (NaN under IEEE division; NaN from the reciprocal form),
+    # and the splat must be dropped as the oracle drops it (its texel
+    # conversion of NaN lands outside the atlas), in both splat forms
+    print("[splat] points at the VPL position", flush=True)
+    pts, off = point_set(64, 2000, seed=7)
+    rng = np.random.default_rng(8)
+    extra = []
+    for k in range(32):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        extra.append([0.0, 0.0, 0.0, unorm4x8(*(n * 0.5 + 0.5), rng.uniform())])
+    bins = np.repeat(np.arange(N_VPL), np.diff(off))
+    bins = np.concatenate([bins, rng.integers(0, N_VPL, len(extra))])
+    allp = np.concatenate([pts, np.array(extra, np.float32)])
+    order = np.argsort(bins, kind="stable")
+    allp, bins = allp[order], bins[order]
+    off2 = np.zeros(N_VPL + 1, np.uint32)
+    np.add.at(off2, bins + 1, 1)
+    off2 = np.cumsum(off2).astype(np.uint32)
+    rec = vpl_records()
+    ref = oracle.ism_splat(allp, off2, rec[:, 0:3], rec[:, 4:7], ATLAS, Z_FAR)
+    base = oracle.ism_splat(pts, off, rec[:, 0:3], rec[:, 4:7], ATLAS, Z_FAR)
+    # the extra points change nothing in the oracle's atlas either
+    assert np.array_equal(ref, base)
+    tiled, _ = run(ctx, allp, off2, 0)
+    point_major, _ = run(ctx, allp, off2, 256)
+    assert np.array_equal(tiled, ref)
+    assert np.array_equal(point_major, ref)
