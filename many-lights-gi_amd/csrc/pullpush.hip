@@ -366,9 +366,8 @@ __device__ __forceinline__ float push_weight(int i, int op) {
 // pixels, in the reference's invocation order.
 constexpr int kOx[4] = {0, 0, 1, 1}, kOy[4] = {0, 1, 1, 0};
 
-// The four coarse neighbours of push quad (gx, gy) of a level l >= 1 (kept
-// as separate arrays: flags held in a struct are stored as bytes and cost
-// conversions once the struct is taken apart).
+// The four coarse neighbours of push quad (gx, gy) of a level l >= 1; their
+// usable flags (in the ISM's neighbourhood and not empty) are returned apart.
 struct Coarse4 {
   float cd[4], cmd[4], cr[4], bound[4];
   uint32_t cdisp[4];
@@ -430,7 +429,7 @@ __device__ __forceinline__ void push_pixel(const Coarse4& c, const bool (&usable
   Texel res = same[t];
   bool keep[4], any = false, occluded = false;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {  // bitwise, so that no flag becomes a branch
+  for (int i = 0; i < 4; ++i) {  // bitwise: the short-circuit form compiled to more instructions
     keep[i] = valid[i] & !(cd[i] > max_depth);
     any |= keep[i];
     occluded |= keep[i] & (res.depth > cmd[i]);
