@@ -367,7 +367,7 @@ __device__ __forceinline__ float push_weight(int i, int op) {
 constexpr int kOx[4] = {0, 0, 1, 1}, kOy[4] = {0, 1, 1, 0};
 
 // The four coarse neighbours of push quad (gx, gy) of a level l >= 1; their
-// usable flags (in the ISM's neighbourhood and not empty) are returned apart.
+// usable flags (the neighbour is not empty: depth != 1) are returned apart.
 struct Coarse4 {
   float cd[4], cmd[4], cr[4], bound[4];
   uint32_t cdisp[4];
