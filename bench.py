@@ -27,7 +27,10 @@ frames in flight (fp32_pipelined, mixed_pipelined): S frame slots, each
 with its own buffers and its own context and stream, frame i on slot
 i mod S, so that one frame's last workgroups of a stage share the GPU with
 the next frame's work; every frame does the whole work. `value` is
-mixed_pipelined (config.form, config.frames_in_flight); serial_ms_per_frame
+mixed_pipelined (config.form, config.frames_in_flight); K frames in flight
+take about (K - 1) steady-state frames plus one frame's latency, so the
+default K = 200 keeps that fill and drain to a few percent of the figure
+(K = 50 reports ~5% more per frame). serial_ms_per_frame
 is the mixed form one frame after another on one stream, through the faster
 of its two paths (both drop-in entry points for one frame). Precision is
 compared within a path: mixed_speedup_vs_fp32 = fp32 / mixed_single,
@@ -643,7 +646,9 @@ def trace_table(root: str) -> str:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200,
+                    help="timed steps K per form (200: the pipelined forms' fill and drain, about three "
+                         "frames' time, then weigh ~2%% in the per-frame figure)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="1080p", choices=["1080p", "small", "4k", "batch64", "serve"])
     ap.add_argument("--form", default="both", choices=["fp32", "mixed", "both"])
