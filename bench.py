@@ -36,6 +36,9 @@ of its two paths (both drop-in entry points for one frame). Precision is
 compared within a path: mixed_speedup_vs_fp32 = fp32 / mixed_single,
 mixed_speedup_same_path = fp32_batched / mixed (and its graph replays),
 mixed_speedup_pipelined = fp32_pipelined / mixed_pipelined.
+The default 1080p run then runs the other configurations briefly and adds
+their headline figures as other_configs (the 4K stress frames, the 64-frame
+batch, a 10 s served trace; --no-others leaves them out).
 
 roofline: for the kernel with the largest mean duration in the timed frames:
 its VALU issue rate ("valu": SQ_INSTS_VALU per launch from the committed
@@ -496,7 +499,7 @@ def batch_bench(a, ctx):
                        "autotuned": tuned}}
 
 
-def serve_bench(a, ctx):
+def serve_bench(a, ctx, sweep_seconds: float = 2.0):
     import numpy as np
     import mlgi
     from mlgi import workloads as W
@@ -527,7 +530,7 @@ def serve_bench(a, ctx):
     # higher arrival rates (short traces): where batching and overlap engage
     sweep = []
     for rate in (500.0, 1000.0, 2000.0, 4000.0, 8000.0):
-        r = trace(rate, 2.0, 100 + int(rate))
+        r = trace(rate, sweep_seconds, 100 + int(rate))
         s2, _ = srv.run(r)
         sweep.append({"arrival_rate_per_s": rate, "requests": len(r), "req_per_s": round(s2["req_per_s"], 2),
                       "latency_p50_ms": round(s2["latency_p50_ms"], 3), "latency_p99_ms": round(s2["latency_p99_ms"], 3),
@@ -547,9 +550,39 @@ def serve_bench(a, ctx):
                                           "peak_bytes", "wall_s", "overlapped_batches", "streams")},
             "rate_sweep": sweep, "batching_engages_at_per_s": engages[0] if engages else None,
             "config": {"workload": "serve", "arrival_rate_per_s": a.rate, "seconds": a.seconds,
+                       "sweep_seconds": sweep_seconds,
                        "scenes_triangles": [4_012, 40_012, 160_012],
                        "scene_points": [256_000, 1_000_000, 4_000_000], "resolutions": sizes, "vpls": vpls,
                        "hbm_budget_gib": a.budget_gib, "rsm": 128}}
+
+
+def other_configs(a, ctx) -> dict:
+    """The other configurations, briefly, in the default run, so that its one
+    line carries them too: the 4K stress frames (K frames each way), the
+    64-frame batch (50 replays) and a 10 s served trace at the arrival rate
+    (1 s per sweep rate); each entry holds the headline figures of that
+    workload's own line (`--workload 4k | batch64 | serve` runs them in full)."""
+    import argparse
+    from mlgi import workloads as W
+
+    def keep(d, keys):
+        return {k: d[k] for k in keys if k in d}
+
+    b = argparse.Namespace(**vars(a))
+    b.seconds = 10.0
+    k4 = stress_bench(b, ctx, W.WORKLOADS["4k"])
+    bb = argparse.Namespace(**vars(a))
+    bb.steps = 50
+    b64 = batch_bench(bb, ctx)
+    sv = serve_bench(b, ctx, sweep_seconds=1.0)
+    return {"4k": keep(k4, ("metric", "value", "unit", "steps", "serial_ms_per_frame")),
+            "batch64": keep(b64, ("metric", "value", "unit", "steps", "ms_per_frame")),
+            "serve": dict(keep(sv, ("metric", "value", "unit", "latency_ms", "latency_budget_ms",
+                                    "batching_engages_at_per_s")),
+                          seconds=b.seconds, sweep_seconds=1.0,
+                          mean_batch=sv["server"]["mean_batch"],
+                          overlapped_batches=sv["server"]["overlapped_batches"],
+                          completed=sv["server"]["completed"], rejected=sv["server"]["rejected"])}
 
 
 # ---------------------------------------------------------------- summaries
@@ -653,6 +686,8 @@ def main() -> int:
     ap.add_argument("--workload", default="1080p", choices=["1080p", "small", "4k", "batch64", "serve"])
     ap.add_argument("--form", default="both", choices=["fp32", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="1080p: leave out the brief runs of the 4k, batch64 and serve workloads (other_configs)")
     ap.add_argument("--slots", type=int, default=3,
                     help="frame workloads: frames in flight for the *_pipelined forms (1: not run)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: host_threads())")
@@ -700,6 +735,8 @@ def main() -> int:
             out = stress_bench(a, ctx, W.WORKLOADS["4k"])
         else:
             out = frame_bench(a, ctx, W.WORKLOADS[a.workload])
+            if a.workload == "1080p" and not a.no_others:
+                out["other_configs"] = other_configs(a, ctx)
     print(json.dumps(out), flush=True)
     return 0
 
