@@ -220,12 +220,12 @@ int mlgi_probe_valu(mlgi_context* ctx, double* fma_per_s);
 /* Exactness check of the division form the splat's projection uses in place
  * of IEEE division (csrc/mlgi_device.hpp: recip_rn, div_rn): the device
  * reciprocal refined by one Newton step against the IEEE 1 / b for every
- * float significand at five exponents (2^-60 .. 2^60), and the quotient
+ * float significand at eleven exponents (2^-100 .. 2^100), and the quotient
  * from it against the IEEE a / b for four numerators per divisor (random
- * significand and sign, 2^-40 .. 2^40 times the divisor's scale; `seed`
- * picks them). Writes both mismatch counts (0 and 0 on a device where the
- * splat's atlas is exact) and the number of quotients tested; waits for the
- * stream. A diagnostic; no reference counterpart. */
+ * significand and sign, 2^-40 .. 2^40 times the divisor's scale and within
+ * 2^-100 .. 2^100; `seed` picks them). Writes both mismatch counts (0 and 0
+ * on a device where the splat's atlas is exact) and the number of quotients
+ * tested; waits for the stream. A diagnostic; no reference counterpart. */
 int mlgi_check_division(mlgi_context* ctx, uint32_t seed, uint64_t* reciprocal_mismatches,
                         uint64_t* quotient_mismatches, uint64_t* quotients_tested);
 

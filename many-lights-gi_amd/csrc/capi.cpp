@@ -128,7 +128,11 @@ int check_params(mlgi_context* c, const mlgi_params* p) {
   if (!p) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "params is NULL");
   if (!p->use_push_pull)
     return fail(c, MLGI_ERR_UNSUPPORTED, "usePushPull=false (raster ISM path) is not supported");
-  if (!(p->z_far > 0.0f)) return fail(c, MLGI_ERR_INVALID_ARGUMENT, "z_far must be > 0");
+  // a finite, normal far plane: the splat's and the pull-push's divisions by
+  // distances and by zFar are exact (IEEE quotients) for divisors in the
+  // range the device division check covers
+  if (!(p->z_far >= 0x1p-60f && p->z_far <= 0x1p60f))
+    return fail(c, MLGI_ERR_INVALID_ARGUMENT, "z_far must lie in [2^-60, 2^60]");
   return MLGI_OK;
 }
 

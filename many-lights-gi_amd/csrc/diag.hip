@@ -39,12 +39,16 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   return x;
 }
 
-// Thread m (< 2^23) takes significand m at the exponents of kDivExps: the
-// refined reciprocal against the IEEE 1 / b, and kDivNums quotients a / b
-// for numerators of random significand, sign and scale (2^-40 .. 2^40 times
-// that of 1) against the IEEE a / b. counts: reciprocal and quotient
+// Thread m (< 2^23) takes significand m at the exponents of kDivExps
+// (2^-100 .. 2^100 in steps of 2^20): the refined reciprocal against the IEEE
+// 1 / b, and kDivNums quotients a / b for numerators of random significand,
+// sign and scale (2^-40 .. 2^40 times that of b, within 2^-100 .. 2^100, so
+// that quotient and remainder stay normal floats, where the correction step
+// is exact) against the IEEE a / b. counts: reciprocal and quotient
 // mismatches.
-constexpr int kDivExps[5] = {127, 127 - 20, 127 + 20, 127 - 60, 127 + 60};
+constexpr int kNumDivExps = 11;
+constexpr int kDivExps[kNumDivExps] = {127 - 100, 127 - 80, 127 - 60, 127 - 40, 127 - 20, 127,
+                                       127 + 20,  127 + 40, 127 + 60, 127 + 80, 127 + 100};
 constexpr int kDivNums = 4;
 
 __global__ __launch_bounds__(256) void division_check_kernel(uint32_t* __restrict__ counts, uint32_t seed) {
@@ -52,15 +56,15 @@ __global__ __launch_bounds__(256) void division_check_kernel(uint32_t* __restric
   if (m >= (1u << 23)) return;
   uint32_t bad_r = 0, bad_q = 0;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < kNumDivExps; ++k) {
     const float b = __uint_as_float(((uint32_t)kDivExps[k] << 23) | m);
     const float y = dev::recip_rn(b);
     if (__float_as_uint(y) != __float_as_uint(1.0f / b)) ++bad_r;
 #pragma unroll
     for (int j = 0; j < kDivNums; ++j) {
       const uint32_t h = mix32(m * 32u + (uint32_t)(k * kDivNums + j) + seed);
-      const uint32_t e = 127u - 40u + h % 81u;
-      const float a = __uint_as_float((h & 0x80000000u) | (e << 23) | (mix32(h) & 0x7FFFFFu));
+      const int e = min(127 + 100, max(127 - 100, kDivExps[k] - 40 + (int)(h % 81u)));
+      const float a = __uint_as_float((h & 0x80000000u) | ((uint32_t)e << 23) | (mix32(h) & 0x7FFFFFu));
       if (__float_as_uint(dev::div_rn(a, b, y)) != __float_as_uint(a / b)) ++bad_q;
     }
   }
@@ -70,7 +74,7 @@ __global__ __launch_bounds__(256) void division_check_kernel(uint32_t* __restric
 
 }  // namespace
 
-uint64_t division_check_quotients() { return (uint64_t)(1u << 23) * 5u * kDivNums; }
+uint64_t division_check_quotients() { return (uint64_t)(1u << 23) * kNumDivExps * kDivNums; }
 
 hipError_t launch_division_check(hipStream_t s, uint32_t* counts, uint32_t seed) {
   hipLaunchKernelGGL(division_check_kernel, dim3((1u << 23) / 256u), dim3(256), 0, s, counts, seed);

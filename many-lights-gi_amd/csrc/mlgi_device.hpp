@@ -39,8 +39,8 @@ __device__ __forceinline__ float unorm8(uint32_t c) {
 // RN(1 / b) for a normal float b: the hardware reciprocal (within one ulp)
 // refined by one Newton step, y + y (1 - b y) in two FMAs. Equal to the IEEE
 // quotient 1 / b for every significand (mlgi_check_division runs that check
-// on the device: all 2^23 significands at five exponents from 2^-60 to
-// 2^60). A zero, infinite or NaN b gives NaN.
+// on the device: all 2^23 significands at eleven exponents from 2^-100 to
+// 2^100). A zero, infinite or NaN b gives NaN.
 __device__ __forceinline__ float recip_rn(float b) {
   const float y = __builtin_amdgcn_rcpf(b);
   return __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);

@@ -159,6 +159,13 @@ def test_invalid_arguments_rejected(ctx):
     with pytest.raises(mlgi.MlgiError) as e:
         st.ism.render(pts, st.vpl_processor, st.params)
     assert e.value.code == N.MLGI_ERR_INVALID_ARGUMENT
+    # a far plane outside [2^-60, 2^60] (infinite, denormal, zero): the
+    # splat's divisions by zFar are exact only within that range
+    for z in (float("inf"), 1e-40, 0.0, 2.0 ** 61):
+        st.params = wl.params(z_far=z)
+        with pytest.raises(mlgi.MlgiError) as e:
+            st.ism.render(pts, st.vpl_processor, st.params)
+        assert e.value.code == N.MLGI_ERR_INVALID_ARGUMENT, z
     st.params = wl.params()
     wrong_bins = mlgi.points_desc(ctx.upload(inp.points), ctx.upload(np.zeros(11, np.uint32)))
     with pytest.raises(mlgi.MlgiError) as e:

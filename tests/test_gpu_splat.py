@@ -193,11 +193,11 @@ def test_division_form_matches_ieee(ctx):
     # the projection's six divisions are quotients from refined reciprocals
     # of their three divisors (recip_rn, div_rn in mlgi_device.hpp); the
     # device check compares both with IEEE division: every float significand
-    # at five exponents, four numerators per divisor, two numerator seeds
+    # at eleven exponents, four numerators per divisor, two numerator seeds
     for seed in (0, 12345):
         print(f"[division check] seed={seed}", flush=True)
         rec, quo, n = ctx.check_division(seed)
-        assert n == (1 << 23) * 5 * 4
+        assert n == (1 << 23) * 11 * 4
         assert rec == 0 and quo == 0, (rec, quo)
 
 
