@@ -86,7 +86,7 @@ typedef struct mlgi_params {
   int32_t enable_shadowing;       /* ENABLE_SHADOWING */
   int32_t use_interleaving;       /* USE_INTERLEAVING */
   int32_t interleave;             /* interleave side I (power of two, reference: 4) */
-  float z_far;                    /* ISM / camera zFar */
+  float z_far;                    /* ISM / camera zFar, in [2^-60, 2^60] */
   int32_t atlas_side;             /* ISM atlas side (reference totalIsmPixelSize 2048) */
   int32_t pull_push_levels;       /* pull levels L (reference: 6) */
   int32_t precision;              /* mlgi_precision */
